@@ -1,0 +1,228 @@
+/*
+ * omega_afqmc.h -- C ABI of the MI355X (gfx950) phaseless-AFQMC walker step.
+ *
+ * Every entry point replaces one function of the reference's Python/JAX
+ * operator API (myung-group/OmegaQMC); the citation above each declaration
+ * names the function it stands in for.  Conventions:
+ *
+ *   - All array arguments are DEVICE pointers on the context's device,
+ *     caller-owned, row-major (C order) with the reference's shapes.
+ *   - "complex" arrays are interleaved complex128: (re, im) double pairs, so a
+ *     complex (W, M, N) array is W*M*N*2 doubles.  "real" arrays are float64.
+ *   - Calls are asynchronous on the context's stream; nothing synchronises
+ *     unless the name says so.  Scratch memory is owned by the context and
+ *     grows on demand (afq_ctx_reserve pre-sizes it so steady-state calls
+ *     never allocate).
+ *   - Return value: AFQ_OK (0) or a negative afq_status; afq_last_error()
+ *     gives a message (thread-local).  Invalid shapes are rejected before any
+ *     launch.  Numerical failures propagate as NaN/inf exactly as in the
+ *     reference (no error code).
+ *   - The reference draws auxiliary fields and comb offsets with jax.random
+ *     inside the step; here they are inputs (xi, r) or come from the
+ *     counter-based Philox stream of afq_aux_fields / afq_comb_offset.
+ */
+#ifndef OMEGA_AFQMC_H
+#define OMEGA_AFQMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afq_ctx afq_ctx;
+typedef struct afq_system afq_system;
+
+typedef enum {
+  AFQ_OK = 0,
+  AFQ_EINVAL = -1,       /* bad argument / shape */
+  AFQ_ENOMEM = -2,       /* device allocation failed */
+  AFQ_EUNSUPPORTED = -3, /* shape outside what the kernels handle */
+  AFQ_EHIP = -4          /* HIP runtime error */
+} afq_status;
+
+typedef enum {
+  AFQ_FP64 = 0,   /* every contraction in FP64 */
+  AFQ_FP32 = 1,   /* FP32 MFMA contractions, FP64 det/inv/QR/weights */
+  AFQ_BF16X3 = 2  /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulate */
+} afq_precision;
+
+typedef enum { AFQ_REAL = 0, AFQ_COMPLEX = 1 } afq_dtype;
+
+int afq_version(void);
+const char* afq_last_error(void);
+
+/* ---- context ------------------------------------------------------------ */
+int afq_ctx_create(afq_ctx** out, int device, void* hip_stream);
+int afq_ctx_destroy(afq_ctx* ctx);
+int afq_ctx_set_stream(afq_ctx* ctx, void* hip_stream);
+int afq_ctx_synchronize(afq_ctx* ctx);
+int afq_ctx_reserve(afq_ctx* ctx, size_t scratch_bytes);
+
+/* ---- Green's functions and overlaps (OmegaQMC/observables/greens.py) ---- */
+
+/* greens.py:104-129 greens_function_force_bias(phia, phib, trial_up, trial_dn)
+ * -> (Ghalfa (W,Na,M), Ghalfb (W,Nb,M), overlap (W,)), all complex.
+ * phia (W,M,Na), phib (W,M,Nb), trial_up (M,Na), trial_dn (M,Nb): complex. */
+int afq_greens_function_force_bias(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                   const double* phia, const double* phib,
+                                   const double* trial_up, const double* trial_dn,
+                                   double* Ghalfa, double* Ghalfb, double* overlap);
+
+/* greens.py:132-154 greens_function_overlap -> overlap (W,) complex. */
+int afq_greens_function_overlap(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                const double* phia, const double* phib,
+                                const double* trial_up, const double* trial_dn,
+                                double* overlap);
+
+/* greens.py:157-198 greens_function -> (Ga (W,M,M), Gb (W,M,M), Ghalfa, Ghalfb, overlap). */
+int afq_greens_function(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                        const double* phia, const double* phib,
+                        const double* trial_up, const double* trial_dn,
+                        double* Ga, double* Gb, double* Ghalfa, double* Ghalfb,
+                        double* overlap);
+
+/* ---- local energy (OmegaQMC/observables/energy.py) ----------------------- */
+
+/* energy.py:51-91 local_energy_2body(Ghalfa, Ghalfb, rchol_a, rchol_b)
+ * -> (e_coul (W,), e_exch (W,)) complex.  rchol_{a,b} (G,N,M) are real or
+ * complex according to rchol_dtype. */
+int afq_local_energy_2body(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                           const double* Ghalfa, const double* Ghalfb,
+                           const double* rchol_a, const double* rchol_b, int rchol_dtype,
+                           double* e_coul, double* e_exch);
+
+/* energy.py:94-130 local_energy(h1e, Ga, Gb, Ghalfa, Ghalfb, rchol_a, rchol_b, enuc)
+ * -> (e_tot, e_1b, e_2b) (W,) complex.  h1e (M,M) and rchol share dtype. */
+int afq_local_energy(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                     const double* h1e, const double* Ga, const double* Gb,
+                     const double* Ghalfa, const double* Ghalfb,
+                     const double* rchol_a, const double* rchol_b, int dtype, double enuc,
+                     double* e_tot, double* e_1b, double* e_2b);
+
+/* energy.py:218-252 local_energy_streamed(h1e_trial_a (M,Na), h1e_trial_b (M,Nb),
+ * Ghalfa, Ghalfb, rchol_a, rchol_b, enuc, e_chunk_g).  h1e_trial_* complex.
+ * e_chunk_g only fixes the g-slab order of the exchange sum (any value >= 1). */
+int afq_local_energy_streamed(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                              const double* h1e_trial_a, const double* h1e_trial_b,
+                              const double* Ghalfa, const double* Ghalfb,
+                              const double* rchol_a, const double* rchol_b, int rchol_dtype,
+                              double enuc, int e_chunk_g,
+                              double* e_tot, double* e_1b, double* e_2b);
+
+/* ---- propagation pieces (OmegaQMC/afqmc_gto.py) ------------------------- */
+
+/* afqmc_gto.py:472-482 _build_vhs(xshifted, chol, dt, chol_chunk_g)
+ * xshifted (W,G) complex, chol (G,M,M) real -> VHS (W,M,M) complex
+ * = i sqrt(dt) sum_g xshifted[w,g] chol[g]. */
+int afq_build_vhs(afq_ctx* ctx, int nwalkers, int nbasis, int naux,
+                  const double* xshifted, const double* chol, double dt, double* VHS);
+
+/* afqmc_gto.py:451-469 _apply_exp_vhs(VHS, phi, nmax): phi_out = sum_n VHS^n phi / n!.
+ * VHS (W,M,M), phi (W,M,N) complex.  phi_out may alias phi. */
+int afq_apply_exp_vhs(afq_ctx* ctx, int nwalkers, int nbasis, int nocc,
+                      const double* VHS, const double* phi, int nmax, double* phi_out);
+
+/* afqmc_gto.py:678-679 one-body half step phi_out[w] = expH1 @ phi[w];
+ * expH1 (M,M) real, phi (W,M,N) complex.  phi_out must not alias phi. */
+int afq_apply_one_body(afq_ctx* ctx, int nwalkers, int nbasis, int nocc,
+                       const double* expH1, const double* phi, double* phi_out);
+
+/* afqmc_gto.py:682-706 force bias and shifted fields:
+ * vbias[g,w] = sum_s sum_iq rchol_s[g,i,q] Ghalf_s[w,i,q];
+ * xbar = -sqrt(dt) (i vbias - mf_shift)^T capped at |xbar| <= fbbound;
+ * xshifted = xi - xbar; cmf = -sqrt(dt) sum_g xshifted mf_shift;
+ * cfb = sum_g xi xbar - 1/2 sum_g xbar^2.
+ * rchol real (G,N,M); Ghalf complex; mf_shift (G,) complex; xi (W,G) real.
+ * Outputs complex: xbar, xshifted (W,G); cfb, cmf (W,).  xbar may be NULL. */
+int afq_force_bias(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                   const double* rchol_a, const double* rchol_b,
+                   const double* Ghalfa, const double* Ghalfb,
+                   const double* mf_shift, const double* xi, double dt, double fbbound,
+                   double* xbar, double* xshifted, double* cfb, double* cmf);
+
+/* afqmc_gto.py:527-571 _update_weights_phaseless(weights, ovlp_old, ovlp_new, cfb,
+ * cmf, e_hybrid_old, eshift, dt) -> (weights_new, e_hybrid_new).
+ * weights real (W,); the rest complex (W,).  Outputs may alias inputs. */
+int afq_update_weights_phaseless(afq_ctx* ctx, int nwalkers, const double* weights,
+                                 const double* ovlp_old, const double* ovlp_new,
+                                 const double* cfb, const double* cmf,
+                                 const double* e_hybrid_old, double eshift, double dt,
+                                 double* weights_new, double* e_hybrid_new);
+
+/* afqmc_gto.py:160-181 orthogonalize_walkers(phia, phib) -> (phia, phib, log_detR,
+ * sign_detR).  In place on phia/phib; log_detR (W,) real, sign_detR (W,)
+ * complex (may be NULL).  Thin QR with positive real diagonal R (modified
+ * Gram-Schmidt with one re-orthogonalisation pass). */
+int afq_orthogonalize_walkers(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                              double* phia, double* phib, double* log_detR, double* sign_detR);
+
+/* afqmc_gto.py:184-238 population_control_comb(weights, phia, phib, rng_key, e_hybrid)
+ * with the comb offset r in [0,1) given.  Writes weights_new (= 1), the
+ * resampled walkers phia_new/phib_new (must not alias the inputs),
+ * e_hybrid_new (if e_hybrid != NULL) and the survivor indices idx (W,) int32.
+ * The running weight sum is sequential FP64, left to right. */
+int afq_population_control_comb(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                const double* weights, const double* phia, const double* phib,
+                                double r, const double* e_hybrid,
+                                double* weights_new, double* phia_new, double* phib_new,
+                                double* e_hybrid_new, int32_t* idx);
+
+/* afqmc_gto.py:1428-1432 weight clip w <- clip(w, 0, frac * sum|w|) in place. */
+int afq_clip_weights(afq_ctx* ctx, int nwalkers, double* weights, double frac);
+
+/* afqmc_gto.py:1448-1451 per-step accumulators: acc[0] += sum|w|,
+ * acc[1] += sum |w| Re(e_hybrid).  acc is a device array of 2 doubles. */
+int afq_accumulate_hybrid(afq_ctx* ctx, int nwalkers, const double* weights,
+                          const double* e_hybrid, double* acc);
+
+/* afqmc_gto.py:1488-1495 block statistics: out[0] = sum|w| Re(e)/sum|w|,
+ * out[1] = sum|w|, out[2] = sum w^2, out[3] = max w.  out: 4 device doubles. */
+int afq_block_stats(afq_ctx* ctx, int nwalkers, const double* weights, const double* e_tot,
+                    double* out);
+
+/* ---- counter-based auxiliary fields ------------------------------------ */
+
+/* xi[w,g] ~ N(0,1): Philox4x32-10 counter (g/2, w, step, 0), key = seed,
+ * Box-Muller.  xi (W,G) real. */
+int afq_aux_fields(afq_ctx* ctx, uint64_t seed, int64_t step, int nwalkers, int naux, double* xi);
+/* Comb offset r in [0,1) for a step (host computation, no device work). */
+double afq_comb_offset(uint64_t seed, int64_t step);
+
+/* ---- prepared system and the fused step -------------------------------- */
+
+/* Setup counterpart of build_propagator (afqmc_gto.py:574-627) + the half
+ * rotation (cholesky.py:470-510): keeps references to the caller's device
+ * arrays chol (G,M,M) real, rchol_a (G,Na,M) / rchol_b (G,Nb,M) real,
+ * expH1 (M,M) real, mf_shift (G,) complex, trial_up (M,Na) / trial_dn
+ * (M,Nb) complex, and builds the reduced-precision copies the chosen
+ * precision needs.  The caller keeps those arrays alive and unchanged. */
+int afq_system_create(afq_ctx* ctx, afq_system** out, int nbasis, int nup, int ndown, int naux,
+                      const double* chol, const double* rchol_a, const double* rchol_b,
+                      const double* expH1, const double* mf_shift,
+                      const double* trial_up, const double* trial_dn,
+                      double dt, double fbbound, int exp_nmax, int precision);
+int afq_system_destroy(afq_system* sys);
+
+/* afqmc_gto.py:630-725 propagate_walkers(...) for one step, in place on
+ * phia/phib/weights/e_hybrid; overlap_new (W,) complex is written.  The
+ * reference ignores its `overlap` argument, so there is none here.  xi (W,G)
+ * real; NULL draws it from the Philox stream (seed, step). */
+int afq_propagate_walkers(afq_system* sys, int nwalkers, double* phia, double* phib,
+                          double* weights, double* e_hybrid, double eshift,
+                          const double* xi, uint64_t seed, int64_t step,
+                          double* overlap_new);
+
+/* Block-end mixed-estimator energy of the system's walkers
+ * (afqmc_gto.py:1466-1473: greens_function + local_energy) without full G:
+ * E_1b via h1e Psi* (energy.py:193-215), exchange with the system's
+ * precision.  h1e (M,M) real.  Outputs (W,) complex. */
+int afq_system_local_energy(afq_system* sys, int nwalkers, const double* phia,
+                            const double* phib, const double* h1e, double enuc,
+                            double* e_tot, double* e_1b, double* e_2b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMEGA_AFQMC_H */

@@ -1,0 +1,17 @@
+"""omegaqmc_amd -- MI355X-native phaseless-AFQMC walker propagator.
+
+Hand-written gfx950 HIP kernels behind a C ABI (include/omega_afqmc.h), with
+a Python mirror of the reference's ``OmegaQMC.afqmc_gto`` operator API.
+"""
+
+from .afqmc_gto import (  # noqa: F401
+    FBBOUND_DEFAULT, WEIGHT_CLIP_FRACTION, PhiloxKey, System, _apply_exp_vhs,
+    _apply_exp_vhs_from_chol, _build_vhs, _update_weights_phaseless, apply_one_body, aux_fields,
+    build_propagator, comb_offset, determinant_energy_from_integrals, do_binning_analysis,
+    force_bias, greens_function, greens_function_force_bias, greens_function_overlap,
+    half_rotate_cholesky, local_energy, local_energy_2body, local_energy_streamed,
+    one_body_chol_correction, orthogonalize_walkers, population_control_comb,
+    prepare_afqmc_integrals_from_arrays, propagate_walkers)
+from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
+
+__version__ = "0.1.0"

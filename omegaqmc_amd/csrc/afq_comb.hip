@@ -1,0 +1,224 @@
+// Comb population control and the driver's per-step weight glue
+// (OmegaQMC/afqmc_gto.py:184-238, 1428-1432, 1448-1451, 1488-1495).
+//
+// Comb: ws = w / (sum w / W); c = running sum of ws; tooth j = j + r;
+// idx_j = first i with c_i >= tooth (searchsorted 'left'), clipped to W-1.
+// The total and the running sum are accumulated left to right by a single
+// lane in FP64, so the indices are bit-for-bit those of a sequential host
+// loop; the searches and the walker gather are parallel.
+#include "afq_kernels.h"
+
+namespace afq {
+
+namespace {
+
+constexpr int NT = 1024;
+constexpr int LDS_W = 16384;  // weights staged in LDS up to this many walkers
+
+__global__ __launch_bounds__(NT) void k_comb_index(int W, const double* __restrict__ w, double r,
+                                                   double* __restrict__ cum_g,
+                                                   int32_t* __restrict__ idx,
+                                                   double* __restrict__ w_new) {
+  extern __shared__ double cum_s[];
+  const bool in_lds = (W <= LDS_W);
+  double* cum = in_lds ? cum_s : cum_g;
+  for (int i = threadIdx.x; i < W; i += NT) cum[i] = w[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double total = 0.0;
+    for (int i = 0; i < W; ++i) total += cum[i];
+    const double scale = total / (double)W;
+    double c = 0.0;
+    for (int i = 0; i < W; ++i) {
+      c += cum[i] / scale;
+      cum[i] = c;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < W; j += NT) {
+    const double tooth = (double)j + r;
+    int lo = 0, hi = W;  // first index with cum >= tooth
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (cum[mid] < tooth) lo = mid + 1; else hi = mid;
+    }
+    idx[j] = lo > W - 1 ? W - 1 : lo;
+    w_new[j] = 1.0;
+  }
+}
+
+// One workgroup per destination walker: copy both spins and e_hybrid.
+__global__ __launch_bounds__(256) void k_comb_gather(long na, long nb,
+                                                     const int32_t* __restrict__ idx,
+                                                     const double4* __restrict__ pa,
+                                                     const double4* __restrict__ pb,
+                                                     const cplx* __restrict__ eh,
+                                                     double4* __restrict__ pa_new,
+                                                     double4* __restrict__ pb_new,
+                                                     cplx* __restrict__ eh_new) {
+  const long dst = blockIdx.x;
+  const long src = idx[dst];
+  // na, nb in units of complex pairs (double4 = 2 complex); odd tails handled below
+  const cplx* pac = (const cplx*)pa;
+  const cplx* pbc = (const cplx*)pb;
+  cplx* pan = (cplx*)pa_new;
+  cplx* pbn = (cplx*)pb_new;
+  for (long i = threadIdx.x; i < na; i += 256) pan[dst * na + i] = pac[src * na + i];
+  for (long i = threadIdx.x; i < nb; i += 256) pbn[dst * nb + i] = pbc[src * nb + i];
+  if (eh && threadIdx.x == 0) eh_new[dst] = eh[src];
+}
+
+__global__ __launch_bounds__(1024) void k_sum_abs(int W, const double* __restrict__ w,
+                                                  double* __restrict__ out) {
+  __shared__ double red[32];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < W; i += 1024) s += fabs(w[i]);
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_clip(int W, double* __restrict__ w, const double* __restrict__ total, double frac) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  const double b = *total * frac;
+  w[i] = jmin(jmax(w[i], 0.0), b);
+}
+
+__global__ __launch_bounds__(1024) void k_accumulate(int W, const double* __restrict__ w,
+                                                     const cplx* __restrict__ eh,
+                                                     double* __restrict__ acc) {
+  __shared__ double red[32];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < W; i += 1024) {
+    double a = fabs(w[i]);
+    s0 += a;
+    s1 += a * eh[i].x;
+  }
+  s0 = block_sum(s0, red);
+  s1 = block_sum(s1, red + 16);
+  if (threadIdx.x == 0) {
+    acc[0] += s0;
+    acc[1] += s1;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_block_stats(int W, const double* __restrict__ w,
+                                                      const cplx* __restrict__ e,
+                                                      double* __restrict__ out) {
+  __shared__ double red[32];
+  __shared__ double mx[16];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, m = -INFINITY;
+  for (int i = threadIdx.x; i < W; i += 1024) {
+    double a = fabs(w[i]);
+    s0 += a;
+    s1 += a * e[i].x;
+    s2 += a * a;
+    m = jmax(m, w[i]);
+  }
+  s0 = block_sum(s0, red);
+  s1 = block_sum(s1, red + 16);
+  double s2b = block_sum(s2, red);
+  for (int off = 32; off > 0; off >>= 1) m = jmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = mx[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mm = jmax(mm, mx[i]);
+    out[0] = s1 / s0;
+    out[1] = s0;
+    out[2] = s2b;
+    out[3] = mm;
+  }
+}
+
+}  // namespace
+
+int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
+                const double* phia, const double* phib, double r, const double* eh,
+                double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
+                double* cum_scratch) {
+  if (W <= 0) return AFQ_OK;
+  const size_t sm = W <= LDS_W ? sizeof(double) * W : 0;
+  static bool attr = false;
+  if (!attr) {
+    AFQ_CHECK_HIP(hipFuncSetAttribute((const void*)k_comb_index,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_comb_index, dim3(1), dim3(NT), sm, s, W, weights, r, cum_scratch, idx, w_new);
+  AFQ_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, (long)M * Na, (long)M * Nb, idx,
+                     (const double4*)phia, (const double4*)phib, (const cplx*)eh,
+                     (double4*)phia_new, (double4*)phib_new, (cplx*)eh_new);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(1024), 0, s, W, weights, tmp);
+  AFQ_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_clip, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, weights, tmp, frac);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int accumulate_launch(hipStream_t s, int W, const double* weights, const double* eh, double* acc) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(1024), 0, s, W, weights, (const cplx*)eh, acc);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int block_stats_launch(hipStream_t s, int W, const double* weights, const double* e, double* out) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_block_stats, dim3(1), dim3(1024), 0, s, W, weights, (const cplx*)e, out);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+}  // namespace afq
+
+extern "C" {
+
+int afq_population_control_comb(afq_ctx* ctx, int W, int M, int Na, int Nb, const double* weights,
+                                const double* phia, const double* phib, double r,
+                                const double* eh, double* w_new, double* phia_new,
+                                double* phib_new, double* eh_new, int32_t* idx) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0, "bad shape");
+  AFQ_REQUIRE(r >= 0.0 && r < 1.0, "comb offset r=%g outside [0,1)", r);
+  AFQ_REQUIRE(phia_new != phia && phib_new != phib, "comb outputs must not alias inputs");
+  AFQ_REQUIRE(idx != nullptr && w_new != nullptr, "idx and weights_new are required");
+  AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
+  double* cum = nullptr;
+  if (W > 16384) {
+    cum = (double*)afq::scratch(ctx, sizeof(double) * W);
+    if (!cum) return AFQ_ENOMEM;
+  }
+  return afq::comb_launch(ctx->stream, W, M, Na, Nb, weights, phia, phib, r, eh, w_new, phia_new,
+                          phib_new, eh_new, idx, cum);
+}
+
+int afq_clip_weights(afq_ctx* ctx, int W, double* weights, double frac) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0, "bad shape");
+  double* tmp = (double*)afq::scratch(ctx, 256);
+  if (!tmp) return AFQ_ENOMEM;
+  return afq::clip_weights_launch(ctx->stream, W, weights, frac, tmp);
+}
+
+int afq_accumulate_hybrid(afq_ctx* ctx, int W, const double* weights, const double* eh,
+                          double* acc) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0, "bad shape");
+  return afq::accumulate_launch(ctx->stream, W, weights, eh, acc);
+}
+
+int afq_block_stats(afq_ctx* ctx, int W, const double* weights, const double* e, double* out) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W > 0, "bad shape");
+  return afq::block_stats_launch(ctx->stream, W, weights, e, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// Shared helpers of the gfx950 AFQMC kernels: complex128 arithmetic on
+// double2, the context/system structs and launch-checking macros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "../../include/omega_afqmc.h"
+
+// ---------------------------------------------------------------------------
+// complex128 as double2 (x = re, y = im)
+// ---------------------------------------------------------------------------
+typedef double2 cplx;
+
+__host__ __device__ __forceinline__ cplx mkc(double r, double i) { return make_double2(r, i); }
+__host__ __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mkc(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ cplx csub(cplx a, cplx b) { return mkc(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return mkc(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ cplx cscale(cplx a, double s) { return mkc(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ cplx cconj(cplx a) { return mkc(a.x, -a.y); }
+// a += b * c
+__host__ __device__ __forceinline__ void cfma(cplx& a, cplx b, cplx c) {
+  a.x = fma(b.x, c.x, a.x);
+  a.x = fma(-b.y, c.y, a.x);
+  a.y = fma(b.x, c.y, a.y);
+  a.y = fma(b.y, c.x, a.y);
+}
+// a -= b * c
+__host__ __device__ __forceinline__ void cfms(cplx& a, cplx b, cplx c) {
+  a.x = fma(-b.x, c.x, a.x);
+  a.x = fma(b.y, c.y, a.x);
+  a.y = fma(-b.x, c.y, a.y);
+  a.y = fma(-b.y, c.x, a.y);
+}
+__device__ __forceinline__ double cabs_(cplx a) { return hypot(a.x, a.y); }
+// Smith's algorithm for a / b.
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  if (fabs(b.y) <= fabs(b.x)) {
+    double r = b.y / b.x, d = b.x + b.y * r;
+    return mkc((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  } else {
+    double r = b.x / b.y, d = b.x * r + b.y;
+    return mkc((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+  }
+}
+// NaN-propagating max/min (jnp.maximum / jnp.minimum semantics).
+__device__ __forceinline__ double jmax(double a, double b) {
+  return (a != a || b != b) ? (a + b) : (a > b ? a : b);
+}
+__device__ __forceinline__ double jmin(double a, double b) {
+  return (a != a || b != b) ? (a + b) : (a < b ? a : b);
+}
+
+// ---------------------------------------------------------------------------
+// context / system
+// ---------------------------------------------------------------------------
+struct afq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+struct afq_system {
+  afq_ctx* ctx = nullptr;
+  int M = 0, Na = 0, Nb = 0, G = 0;
+  const double* chol = nullptr;
+  const double* rchol_a = nullptr;
+  const double* rchol_b = nullptr;
+  const double* expH1 = nullptr;
+  const double* mf_shift = nullptr;
+  const double* trial_up = nullptr;
+  const double* trial_dn = nullptr;
+  double dt = 0.005, fbbound = 1.0;
+  int nmax = 6;
+  int precision = AFQ_FP64;
+  // reduced-precision operand copies (owned)
+  void* chol_lp = nullptr;     // precision-specific packed L
+  void* rchol_lp = nullptr;    // precision-specific packed Lhalf (both spins)
+  // step workspace (owned), sized for max_walkers
+  int cap_walkers = 0;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+};
+
+namespace afq {
+
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+// Ensure the context scratch holds at least `bytes`; returns pointer or null.
+void* scratch(afq_ctx* ctx, size_t bytes);
+
+}  // namespace afq
+
+#define AFQ_CHECK_HIP(expr)                                       \
+  do {                                                            \
+    hipError_t _e = (expr);                                       \
+    if (_e != hipSuccess) return afq::hip_fail(_e, #expr);        \
+  } while (0)
+
+#define AFQ_CHECK_LAUNCH() AFQ_CHECK_HIP(hipGetLastError())
+
+#define AFQ_REQUIRE(cond, ...)                                    \
+  do {                                                            \
+    if (!(cond)) return afq::fail(AFQ_EINVAL, __VA_ARGS__);       \
+  } while (0)
+
+static inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+static inline int cdiv_int(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Block reductions (blockDim.x a multiple of 64, <= 1024).  Fixed tree order:
+// results are deterministic for a given launch shape.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// All threads receive the block sum.  `red` is LDS of >= 16 doubles.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+__device__ __forceinline__ cplx block_sum_c(cplx v, double* red) {
+  double re = block_sum(v.x, red);
+  double im = block_sum(v.y, red + 16);
+  return mkc(re, im);
+}

@@ -1,0 +1,301 @@
+// Mixed-estimator local energy (OmegaQMC/observables/energy.py:32-252).
+//
+//   E_1b   = sum_s Tr(h G_s) + E_nuc          (or sum_pi (h Psi*)_pi Ghalf_ip)
+//   X_s[g] = sum_iq Lhalf_s[g,i,q] Ghalf_s[i,q]
+//   E_coul = 1/2 sum_g (X_a[g] + X_b[g])^2    (complex square)
+//   T_s[g] = Lhalf_s[g] Ghalf_s^T  (N x N)
+//   E_exch = 1/2 sum_s sum_g sum_ij T_ij T_ji  (= Tr(T T), not Tr(T T^T))
+//
+// FP64 exchange: g is processed in slabs; each slab's T block for all
+// walkers is one GEMM (Lhalf slab (gs N x M) times Ghalf^T (M x W N)) into
+// scratch, contracted per walker by k_exch_contract in a fixed order.
+#include "afq_gemm.h"
+#include "afq_kernels.h"
+
+namespace afq {
+
+namespace {
+
+constexpr int NT = 256;
+
+// e[w] (+)= 1/2 sum_{g<gs} sum_ij T[(g,i)][(w,j)] T[(g,j)][(w,i)],
+// T row-major (gs*N) x (W*N).  One workgroup per walker.
+__global__ __launch_bounds__(NT) void k_exch_contract(int W, int N, int gs,
+                                                      const cplx* __restrict__ T,
+                                                      cplx* __restrict__ e, int accumulate) {
+  __shared__ double red[32];
+  const long w = blockIdx.x;
+  const long ld = (long)W * N;
+  cplx acc = mkc(0, 0);
+  const int per = N * N;
+  for (long e2 = threadIdx.x; e2 < (long)gs * per; e2 += NT) {
+    int g = (int)(e2 / per), r = (int)(e2 % per);
+    int i = r / N, j = r % N;
+    cplx a = T[((long)g * N + i) * ld + w * N + j];
+    cplx b = T[((long)g * N + j) * ld + w * N + i];
+    cfma(acc, a, b);
+  }
+  acc = block_sum_c(acc, red);
+  if (threadIdx.x == 0) {
+    cplx v = cscale(acc, 0.5);
+    e[w] = accumulate ? cadd(e[w], v) : v;
+  }
+}
+
+// e_coul[w] = 1/2 sum_g X[w][g]^2, X (W,G) complex.
+__global__ __launch_bounds__(NT) void k_coulomb(int G, const cplx* __restrict__ X,
+                                                cplx* __restrict__ ec) {
+  __shared__ double red[32];
+  const long w = blockIdx.x;
+  cplx acc = mkc(0, 0);
+  for (int g = threadIdx.x; g < G; g += NT) {
+    cplx x = X[w * G + g];
+    cfma(acc, x, x);
+  }
+  acc = block_sum_c(acc, red);
+  if (threadIdx.x == 0) ec[w] = cscale(acc, 0.5);
+}
+
+// e1b[w] = sum_{p,q} h[p][q] (Ga[w][q][p] + Gb[w][q][p]) + enuc
+template <bool HC>
+__global__ __launch_bounds__(NT) void k_e1b_fullg(int M, const double* __restrict__ h,
+                                                  const cplx* __restrict__ Ga,
+                                                  const cplx* __restrict__ Gb, double enuc,
+                                                  cplx* __restrict__ e1b) {
+  __shared__ double red[32];
+  const long w = blockIdx.x;
+  const long MM = (long)M * M;
+  cplx sa = mkc(0, 0), sb = mkc(0, 0);
+  for (long e2 = threadIdx.x; e2 < MM; e2 += NT) {
+    int p = (int)(e2 / M), q = (int)(e2 % M);
+    cplx hv = HC ? ((const cplx*)h)[e2] : mkc(h[e2], 0.0);
+    cfma(sa, hv, Ga[w * MM + (long)q * M + p]);
+    cfma(sb, hv, Gb[w * MM + (long)q * M + p]);
+  }
+  sa = block_sum_c(sa, red);
+  sb = block_sum_c(sb, red);
+  if (threadIdx.x == 0) e1b[w] = mkc(sa.x + sb.x + enuc, sa.y + sb.y);
+}
+
+// e1b[w] = sum_{p,i} hTa[p][i] Gha[w][i][p] + sum hTb Ghb + enuc
+__global__ __launch_bounds__(NT) void k_e1b_half(int M, int Na, int Nb,
+                                                 const cplx* __restrict__ hTa,
+                                                 const cplx* __restrict__ hTb,
+                                                 const cplx* __restrict__ Gha,
+                                                 const cplx* __restrict__ Ghb, double enuc,
+                                                 cplx* __restrict__ e1b) {
+  __shared__ double red[32];
+  const long w = blockIdx.x;
+  cplx sa = mkc(0, 0), sb = mkc(0, 0);
+  for (long e2 = threadIdx.x; e2 < (long)Na * M; e2 += NT) {
+    int i = (int)(e2 / M), p = (int)(e2 % M);
+    cfma(sa, hTa[(long)p * Na + i], Gha[w * Na * M + e2]);
+  }
+  for (long e2 = threadIdx.x; e2 < (long)Nb * M; e2 += NT) {
+    int i = (int)(e2 / M), p = (int)(e2 % M);
+    cfma(sb, hTb[(long)p * Nb + i], Ghb[w * Nb * M + e2]);
+  }
+  sa = block_sum_c(sa, red);
+  sb = block_sum_c(sb, red);
+  if (threadIdx.x == 0) e1b[w] = mkc(sa.x + sb.x + enuc, sa.y + sb.y);
+}
+
+__global__ void k_combine(int W, const cplx* __restrict__ e1b, const cplx* __restrict__ ec,
+                          const cplx* __restrict__ ex, cplx* __restrict__ etot,
+                          cplx* __restrict__ e2b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  cplx d = csub(ec[i], ex[i]);
+  if (e2b) e2b[i] = d;
+  etot[i] = cadd(e1b[i], d);
+}
+
+}  // namespace
+
+// Exchange energy of one spin, FP64.  partial: scratch of >= gchunk*N*W*N complex.
+int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
+                  const double* rchol, bool rchol_cplx, int gchunk, double* partial,
+                  double* e_exch, bool accumulate) {
+  if (W <= 0) return AFQ_OK;
+  bool acc = accumulate;
+  if (N == 0) {
+    if (!acc) AFQ_CHECK_HIP(hipMemsetAsync(e_exch, 0, sizeof(cplx) * W, s));
+    return AFQ_OK;
+  }
+  for (int g0 = 0; g0 < G; g0 += gchunk) {
+    int gs = std::min(gchunk, G - g0);
+    GemmArgs ga;
+    ga.m = gs * N; ga.n = W * N; ga.k = M; ga.batch = 1;
+    ga.A = rchol_cplx ? (const void*)((const cplx*)rchol + (long)g0 * N * M)
+                      : (const void*)(rchol + (long)g0 * N * M);
+    ga.lda_m = M; ga.lda_k = 1; ga.a_cplx = rchol_cplx;
+    ga.B = Ghalf; ga.ldb_k = 1; ga.ldb_n = M; ga.b_cplx = true;
+    ga.C = (cplx*)partial; ga.ldc = (long)W * N;
+    int rc = gemm_fp64(s, ga);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_exch_contract, dim3(W), dim3(NT), 0, s, W, N, gs, (const cplx*)partial,
+                       (cplx*)e_exch, acc ? 1 : 0);
+    AFQ_CHECK_LAUNCH();
+    acc = true;
+  }
+  if (G == 0 && !accumulate) AFQ_CHECK_HIP(hipMemsetAsync(e_exch, 0, sizeof(cplx) * W, s));
+  return AFQ_OK;
+}
+
+// X (W,G) = sum_s Ghalf_s . rchol_s^T (rchol real or complex)
+static int coulomb_x(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
+                     const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
+                     double* X) {
+  const int Ns[2] = {Na, Nb};
+  const double* R[2] = {ra, rb};
+  const double* Gh[2] = {Gha, Ghb};
+  bool first = true;
+  for (int sp = 0; sp < 2; ++sp) {
+    if (Ns[sp] == 0) continue;
+    const long K = (long)Ns[sp] * M;
+    GemmArgs g;
+    g.m = W; g.n = G; g.k = (int)K;
+    g.A = Gh[sp]; g.lda_m = K; g.lda_k = 1; g.a_cplx = true;
+    g.B = R[sp]; g.ldb_k = 1; g.ldb_n = K; g.b_cplx = rc_cplx;
+    g.C = (cplx*)X; g.ldc = G;
+    g.beta = first ? 0.0 : 1.0;
+    int rc = gemm_fp64(s, g);
+    if (rc) return rc;
+    first = false;
+  }
+  if (first) AFQ_CHECK_HIP(hipMemsetAsync(X, 0, sizeof(cplx) * (size_t)W * G, s));
+  return AFQ_OK;
+}
+
+// Scratch layout of the two-body energy: X (W,G) | 3 slots of W complex |
+// T slab (gchunk*N x W*N complex, <= 256 MiB).
+struct E2bScratch {
+  double* X = nullptr;
+  double* slot[3] = {nullptr, nullptr, nullptr};
+  double* part = nullptr;
+  int gchunk = 1;
+};
+
+static int e2b_scratch(afq_ctx* ctx, int W, int Na, int Nb, int G, int gchunk_req, E2bScratch* out) {
+  const int N = std::max(Na, Nb);
+  long per_g = (long)N * W * N * 16;
+  int gc = per_g > 0 ? (int)std::max<long>(1, (256L << 20) / per_g) : std::max(G, 1);
+  gc = std::min(gc, std::max(G, 1));
+  if (gchunk_req > 0) gc = std::min(gc, gchunk_req);
+  size_t sx = align256(16L * W * G), sw = align256(16L * W);
+  size_t need = sx + 3 * sw + align256((long)gc * per_g);
+  char* sc = (char*)scratch(ctx, need);
+  if (!sc) return AFQ_ENOMEM;
+  out->X = (double*)sc;
+  for (int i = 0; i < 3; ++i) out->slot[i] = (double*)(sc + sx + i * sw);
+  out->part = (double*)(sc + sx + 3 * sw);
+  out->gchunk = gc;
+  return AFQ_OK;
+}
+
+static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, int Nb, int G,
+                    const double* Gha, const double* Ghb, const double* ra, const double* rb,
+                    bool rc_cplx, double* e_coul, double* e_exch) {
+  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, sc.X);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)sc.X, (cplx*)e_coul);
+  AFQ_CHECK_LAUNCH();
+  rc = exchange_fp64(s, W, M, Na, G, Gha, ra, rc_cplx, sc.gchunk, sc.part, e_exch, false);
+  if (rc) return rc;
+  return exchange_fp64(s, W, M, Nb, G, Ghb, rb, rc_cplx, sc.gchunk, sc.part, e_exch, true);
+}
+
+int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
+                   const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
+                   double* X, double* e_coul) {
+  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, X);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)X, (cplx*)e_coul);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int e1b_half_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* hTa,
+                    const double* hTb, const double* Gha, const double* Ghb, double enuc,
+                    double* e1b) {
+  hipLaunchKernelGGL(k_e1b_half, dim3(W), dim3(NT), 0, s, M, Na, Nb, (const cplx*)hTa,
+                     (const cplx*)hTb, (const cplx*)Gha, (const cplx*)Ghb, enuc, (cplx*)e1b);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int combine_launch(hipStream_t s, int W, const double* e1b, const double* ec, const double* ex,
+                   double* etot, double* e2b) {
+  hipLaunchKernelGGL(k_combine, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, (const cplx*)e1b,
+                     (const cplx*)ec, (const cplx*)ex, (cplx*)etot, (cplx*)e2b);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+}  // namespace afq
+
+extern "C" {
+
+int afq_local_energy_2body(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* Gha,
+                           const double* Ghb, const double* ra, const double* rb, int dtype,
+                           double* e_coul, double* e_exch) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
+  AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
+  if (W == 0) return AFQ_OK;
+  afq::E2bScratch sc;
+  int rc = afq::e2b_scratch(ctx, W, Na, Nb, G, 0, &sc);
+  if (rc) return rc;
+  return afq::two_body(ctx->stream, sc, W, M, Na, Nb, G, Gha, Ghb, ra, rb, dtype == AFQ_COMPLEX,
+                       e_coul, e_exch);
+}
+
+int afq_local_energy(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* h1e,
+                     const double* Ga, const double* Gb, const double* Gha, const double* Ghb,
+                     const double* ra, const double* rb, int dtype, double enuc, double* e_tot,
+                     double* e_1b, double* e_2b) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
+  AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
+  AFQ_REQUIRE(e_1b != nullptr, "e_1b is required");
+  if (W == 0) return AFQ_OK;
+  hipStream_t s = ctx->stream;
+  if (dtype == AFQ_COMPLEX)
+    hipLaunchKernelGGL((afq::k_e1b_fullg<true>), dim3(W), dim3(afq::NT), 0, s, M, h1e,
+                       (const cplx*)Ga, (const cplx*)Gb, enuc, (cplx*)e_1b);
+  else
+    hipLaunchKernelGGL((afq::k_e1b_fullg<false>), dim3(W), dim3(afq::NT), 0, s, M, h1e,
+                       (const cplx*)Ga, (const cplx*)Gb, enuc, (cplx*)e_1b);
+  AFQ_CHECK_LAUNCH();
+  afq::E2bScratch sc;
+  int rc = afq::e2b_scratch(ctx, W, Na, Nb, G, 0, &sc);
+  if (rc) return rc;
+  rc = afq::two_body(s, sc, W, M, Na, Nb, G, Gha, Ghb, ra, rb, dtype == AFQ_COMPLEX, sc.slot[0],
+                     sc.slot[1]);
+  if (rc) return rc;
+  return afq::combine_launch(s, W, e_1b, sc.slot[0], sc.slot[1], e_tot, e_2b);
+}
+
+int afq_local_energy_streamed(afq_ctx* ctx, int W, int M, int Na, int Nb, int G,
+                              const double* hTa, const double* hTb, const double* Gha,
+                              const double* Ghb, const double* ra, const double* rb, int dtype,
+                              double enuc, int e_chunk_g, double* e_tot, double* e_1b,
+                              double* e_2b) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0 && e_chunk_g >= 1, "bad shape");
+  AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
+  AFQ_REQUIRE(e_1b != nullptr, "e_1b is required");
+  if (W == 0) return AFQ_OK;
+  hipStream_t s = ctx->stream;
+  int rc = afq::e1b_half_launch(s, W, M, Na, Nb, hTa, hTb, Gha, Ghb, enuc, e_1b);
+  if (rc) return rc;
+  afq::E2bScratch sc;
+  rc = afq::e2b_scratch(ctx, W, Na, Nb, G, e_chunk_g, &sc);
+  if (rc) return rc;
+  rc = afq::two_body(s, sc, W, M, Na, Nb, G, Gha, Ghb, ra, rb, dtype == AFQ_COMPLEX, sc.slot[0],
+                     sc.slot[1]);
+  if (rc) return rc;
+  return afq::combine_launch(s, W, e_1b, sc.slot[0], sc.slot[1], e_tot, e_2b);
+}
+
+}  // extern "C"

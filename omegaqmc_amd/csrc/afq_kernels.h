@@ -1,0 +1,59 @@
+// Internal launchers shared between the kernel files (not part of the ABI).
+#pragma once
+#include "afq_common.h"
+
+namespace afq {
+
+int copy_async(hipStream_t s, const double* src, double* dst, long n);
+
+// greens (afq_greens.hip)
+int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,
+                  const double* phib, const double* tu, const double* td, double* Gha,
+                  double* Ghb, double* ovlp);
+int full_g_launch(hipStream_t s, int W, int M, int N, const double* trial, const double* Ghalf,
+                  double* G);
+
+// propagation (afq_propagate.hip)
+int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
+                         const double* rb, const double* Gha, const double* Ghb, double* vbiasT);
+int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
+                 const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
+                 double* cmf);
+int build_vhs_fp64(hipStream_t s, int W, int M, int G, const double* xs, const double* chol,
+                   double dt, double* VHS);
+int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, const double* phi,
+                       int nmax, double* phi_out, double* tmp0, double* tmp1);
+int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const double* phi,
+                  double* out);
+int weights_launch(hipStream_t s, int W, const double* w, const double* oo, const double* on,
+                   const double* cfb, const double* cmf, const double* eh, double eshift, double dt,
+                   double* w_out, double* eh_out);
+
+// rng (afq_rng.hip)
+int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, double* xi);
+
+// stabilisation / population control (afq_ortho.hip, afq_comb.hip)
+int ortho_launch(hipStream_t s, int W, int M, int Na, int Nb, double* phia, double* phib,
+                 double* log_detR, double* sign_detR);
+int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
+                const double* phia, const double* phib, double r, const double* eh,
+                double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
+                double* cum_scratch);
+int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp);
+int block_stats_launch(hipStream_t s, int W, const double* weights, const double* e, double* out);
+int accumulate_launch(hipStream_t s, int W, const double* weights, const double* eh, double* acc);
+
+// energy (afq_energy.hip)
+int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
+                  const double* rchol, bool rchol_cplx, int gchunk, double* partial,
+                  double* e_exch, bool accumulate);
+int e1b_half_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* hTa,
+                    const double* hTb, const double* Gha, const double* Ghb, double enuc,
+                    double* e1b);
+int combine_launch(hipStream_t s, int W, const double* e1b, const double* ec, const double* ex,
+                   double* etot, double* e2b);
+int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
+                   const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
+                   double* X, double* e_coul);
+
+}  // namespace afq

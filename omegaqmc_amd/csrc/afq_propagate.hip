@@ -1,0 +1,235 @@
+// Propagation pieces of the phaseless step (OmegaQMC/afqmc_gto.py:451-725):
+// force bias + shifted fields, VHS build, Taylor exp(VHS) phi, one-body half
+// step and the phaseless weight update, each as a reference-shaped ABI
+// entry point.  The FP64 contractions run on the generic tiled GEMM; the
+// fused, precision-selectable step lives in afq_step.hip.
+#include "afq_gemm.h"
+#include "afq_kernels.h"
+
+namespace afq {
+
+namespace {
+
+constexpr int NT = 256;
+
+// afqmc_gto.py:687-706 per walker (one workgroup): cap, shift, cfb, cmf.
+__global__ __launch_bounds__(NT) void k_shift(int G, const cplx* __restrict__ vbias,  // (W,G)
+                                              const cplx* __restrict__ mf,            // (G,)
+                                              const double* __restrict__ xi,          // (W,G)
+                                              double sqdt, double fb, cplx* __restrict__ xbar_out,
+                                              cplx* __restrict__ xs_out, cplx* __restrict__ cfb,
+                                              cplx* __restrict__ cmf) {
+  __shared__ double red[32];
+  const long w = blockIdx.x;
+  cplx s_mf = mkc(0, 0), s_xixb = mkc(0, 0), s_xb2 = mkc(0, 0);
+  for (int g = threadIdx.x; g < G; g += NT) {
+    const cplx v = vbias[w * G + g];
+    const cplx m = mf[g];
+    cplx xb = mkc(sqdt * (v.y + m.x), -sqdt * (v.x - m.y));
+    const double a = hypot(xb.x, xb.y);
+    if (a > fb) xb = mkc((xb.x * fb) / a, (xb.y * fb) / a);
+    const double x = xi[w * G + g];
+    const cplx xs = mkc(x - xb.x, -xb.y);
+    if (xbar_out) xbar_out[w * G + g] = xb;
+    xs_out[w * G + g] = xs;
+    s_mf = cadd(s_mf, cmul(xs, m));
+    s_xixb = cadd(s_xixb, cscale(xb, x));
+    s_xb2 = cadd(s_xb2, cmul(xb, xb));
+  }
+  s_mf = block_sum_c(s_mf, red);
+  s_xixb = block_sum_c(s_xixb, red);
+  s_xb2 = block_sum_c(s_xb2, red);
+  if (threadIdx.x == 0) {
+    cmf[w] = cscale(s_mf, -sqdt);
+    cfb[w] = csub(s_xixb, cscale(s_xb2, 0.5));
+  }
+}
+
+// afqmc_gto.py:527-571, one thread per walker.
+__global__ void k_weights(int W, const double* __restrict__ w_in, const cplx* __restrict__ o_old,
+                          const cplx* __restrict__ o_new, const cplx* __restrict__ cfb,
+                          const cplx* __restrict__ cmf, const cplx* __restrict__ eh_old,
+                          double eshift, double dt, double* __restrict__ w_out,
+                          cplx* __restrict__ eh_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  const cplx r = cdiv(o_new[i], o_old[i]);
+  const cplx lr = mkc(log(hypot(r.x, r.y)), atan2(r.y, r.x));
+  const cplx f = cfb[i], m = cmf[i];
+  cplx e = mkc(-((lr.x + f.x) + m.x) / dt, -((lr.y + f.y) + m.y) / dt);
+  const double eb = sqrt(2.0 / dt);
+  e.x = jmin(jmax(e.x, eshift - eb), eshift + eb);
+  const cplx eo = eh_old[i];
+  const cplx avg = mkc(0.5 * (e.x + eo.x), 0.5 * (e.y + eo.y));
+  const double zr = -dt * (avg.x - eshift), zi = -dt * avg.y;
+  const double ez = exp(zr);
+  const double magn = hypot(ez * cos(zi), ez * sin(zi));
+  const double dth = -dt * e.y - f.y;
+  const double cf = jmax(0.0, cos(dth));
+  w_out[i] = w_in[i] * magn * cf;
+  eh_out[i] = e;
+}
+
+__global__ void k_copy(long n, const double* __restrict__ a, double* __restrict__ b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+}  // namespace
+
+int copy_async(hipStream_t s, const double* src, double* dst, long n) {
+  if (n <= 0 || src == dst) return AFQ_OK;
+  int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, n, src, dst);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+// vbias^T (W,G) = sum_s Ghalf_s (W, N_s M) . rchol_s (G, N_s M)^T  (real rchol)
+int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
+                         const double* rb, const double* Gha, const double* Ghb, double* vbiasT) {
+  const int Ns[2] = {Na, Nb};
+  const double* R[2] = {ra, rb};
+  const double* Gh[2] = {Gha, Ghb};
+  bool first = true;
+  for (int sp = 0; sp < 2; ++sp) {
+    if (Ns[sp] == 0) continue;
+    const long K = (long)Ns[sp] * M;
+    GemmArgs g;
+    g.m = W; g.n = G; g.k = (int)K; g.batch = 1;
+    g.A = Gh[sp]; g.lda_m = K; g.lda_k = 1; g.a_cplx = true;
+    g.B = R[sp]; g.ldb_k = 1; g.ldb_n = K; g.b_cplx = false;
+    g.C = (cplx*)vbiasT; g.ldc = G;
+    g.beta = first ? 0.0 : 1.0;
+    int rc = gemm_fp64(s, g);
+    if (rc) return rc;
+    first = false;
+  }
+  if (first) {  // no electrons: vbias = 0
+    AFQ_CHECK_HIP(hipMemsetAsync(vbiasT, 0, sizeof(cplx) * (size_t)W * G, s));
+  }
+  return AFQ_OK;
+}
+
+int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
+                 const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
+                 double* cmf) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, (const cplx*)vbiasT, (const cplx*)mf, xi,
+                     sqrt(dt), fb, (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int build_vhs_fp64(hipStream_t s, int W, int M, int G, const double* xs, const double* chol,
+                   double dt, double* VHS) {
+  GemmArgs g;
+  g.m = W; g.n = M * M; g.k = G; g.batch = 1;
+  g.A = xs; g.lda_m = G; g.lda_k = 1; g.a_cplx = true;
+  g.B = chol; g.ldb_k = (long)M * M; g.ldb_n = 1; g.b_cplx = false;
+  g.C = (cplx*)VHS; g.ldc = (long)M * M;
+  g.alpha = mkc(0.0, sqrt(dt));
+  return gemm_fp64(s, g);
+}
+
+// phi_out = sum_{n<=nmax} VHS^n phi / n!  (tmp0, tmp1: W*M*N complex each)
+int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, const double* phi,
+                       int nmax, double* phi_out, double* tmp0, double* tmp1) {
+  if (N == 0 || W == 0) return AFQ_OK;
+  const long n = (long)W * M * N * 2;
+  int rc = copy_async(s, phi, tmp0, n);
+  if (rc) return rc;
+  rc = copy_async(s, phi, phi_out, n);
+  if (rc) return rc;
+  double* tin = tmp0;
+  double* tout = tmp1;
+  for (int k = 1; k <= nmax; ++k) {
+    GemmArgs g;
+    g.m = M; g.n = N; g.k = M; g.batch = W;
+    g.A = VHS; g.lda_m = M; g.lda_k = 1; g.sA = (long)M * M; g.a_cplx = true;
+    g.B = tin; g.ldb_k = N; g.ldb_n = 1; g.sB = (long)M * N; g.b_cplx = true;
+    g.C = (cplx*)tout; g.ldc = N; g.sC = (long)M * N;
+    g.R = (cplx*)phi_out; g.ldr = N; g.sR = (long)M * N;
+    g.alpha = mkc(1.0 / k, 0.0);
+    rc = gemm_fp64(s, g);
+    if (rc) return rc;
+    double* t = tin; tin = tout; tout = t;
+  }
+  return AFQ_OK;
+}
+
+int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const double* phi,
+                  double* out) {
+  if (N == 0 || W == 0) return AFQ_OK;
+  GemmArgs g;
+  g.m = M; g.n = N; g.k = M; g.batch = W;
+  g.A = expH1; g.lda_m = M; g.lda_k = 1; g.sA = 0; g.a_cplx = false;
+  g.B = phi; g.ldb_k = N; g.ldb_n = 1; g.sB = (long)M * N; g.b_cplx = true;
+  g.C = (cplx*)out; g.ldc = N; g.sC = (long)M * N;
+  return gemm_fp64(s, g);
+}
+
+int weights_launch(hipStream_t s, int W, const double* w, const double* oo, const double* on,
+                   const double* cfb, const double* cmf, const double* eh, double eshift, double dt,
+                   double* w_out, double* eh_out) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_weights, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, w, (const cplx*)oo,
+                     (const cplx*)on, (const cplx*)cfb, (const cplx*)cmf, (const cplx*)eh, eshift, dt,
+                     w_out, (cplx*)eh_out);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+}  // namespace afq
+
+extern "C" {
+
+int afq_build_vhs(afq_ctx* ctx, int W, int M, int G, const double* xs, const double* chol,
+                  double dt, double* VHS) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && G >= 0, "bad shape");
+  return afq::build_vhs_fp64(ctx->stream, W, M, G, xs, chol, dt, VHS);
+}
+
+int afq_apply_exp_vhs(afq_ctx* ctx, int W, int M, int N, const double* VHS, const double* phi,
+                      int nmax, double* phi_out) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && N >= 0 && nmax >= 0, "bad shape");
+  size_t one = align256(sizeof(double) * 2 * (size_t)W * M * N);
+  double* tmp = (double*)afq::scratch(ctx, 2 * one);
+  if (!tmp && one) return AFQ_ENOMEM;
+  return afq::apply_exp_vhs_fp64(ctx->stream, W, M, N, VHS, phi, nmax, phi_out, tmp,
+                                 (double*)((char*)tmp + one));
+}
+
+int afq_apply_one_body(afq_ctx* ctx, int W, int M, int N, const double* expH1, const double* phi,
+                       double* out) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && N >= 0, "bad shape");
+  AFQ_REQUIRE(out != phi || N == 0 || W == 0, "afq_apply_one_body: out must not alias phi");
+  return afq::one_body_fp64(ctx->stream, W, M, N, expH1, phi, out);
+}
+
+int afq_force_bias(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* ra,
+                   const double* rb, const double* Gha, const double* Ghb, const double* mf,
+                   const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
+                   double* cmf) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
+  double* vb = (double*)afq::scratch(ctx, sizeof(double) * 2 * (size_t)W * G);
+  if (!vb && W * (size_t)G) return AFQ_ENOMEM;
+  int rc = afq::force_bias_gemm_fp64(ctx->stream, W, M, Na, Nb, G, ra, rb, Gha, Ghb, vb);
+  if (rc) return rc;
+  return afq::shift_launch(ctx->stream, W, G, vb, mf, xi, dt, fb, xbar, xs, cfb, cmf);
+}
+
+int afq_update_weights_phaseless(afq_ctx* ctx, int W, const double* w, const double* oo,
+                                 const double* on, const double* cfb, const double* cmf,
+                                 const double* eh, double eshift, double dt, double* w_out,
+                                 double* eh_out) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && dt > 0, "bad arguments");
+  return afq::weights_launch(ctx->stream, W, w, oo, on, cfb, cmf, eh, eshift, dt, w_out, eh_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""Device-resident AFQMC driver: counterpart of ``_AFQMCDriverGTO``
+(OmegaQMC/afqmc_gto.py:1034-1631) for array-input Hamiltonians.
+
+Per step (afqmc_gto.py:1383-1451): QR every ``stabilize_freq`` steps
+(before the step), one fused phaseless step, weight clip (skipped at step
+1), comb every ``pop_control_freq`` steps, on-device accumulators.  Per
+block: mixed-estimator energy, Kish size, eshift update -- the only host
+synchronisation.  Walker buffers are preallocated; steady-state steps
+allocate nothing.
+"""
+
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr
+from .afqmc_gto import (C128, F64, PhiloxKey, System, WEIGHT_CLIP_FRACTION, _r,
+                        build_propagator, determinant_energy_from_integrals, do_binning_analysis,
+                        half_rotate_cholesky, prepare_afqmc_integrals_from_arrays)
+
+
+class AFQMCDriver:
+    """Set up once (integrals, trial, propagator), run many times."""
+
+    def __init__(self, integrals, dt=0.005, e_hf=None, eshift0=None, verbose=True,
+                 precision="fp64", fbbound=None, exp_nmax=6):
+        self.dt = float(dt)
+        self.verbose = verbose
+        self.precision = precision
+        self.h1e = integrals['h1e']
+        self.h1e_mod = integrals['h1e_mod']
+        self.chol = integrals['chol']
+        self.enuc = float(integrals['enuc'])
+        self.nbasis = int(integrals['nbasis'])
+        self.nup = int(integrals['nup'])
+        self.ndown = int(integrals['ndown'])
+        self.naux = int(self.chol.shape[0])
+        if e_hf is None:
+            e_hf = determinant_energy_from_integrals(self.h1e, self.chol, self.enuc, self.nup,
+                                                     self.ndown)
+        self.ehf = float(e_hf)
+        dev = self.chol.device
+        self.trial_up = torch.eye(self.nbasis, self.nup, dtype=F64, device=dev)
+        self.trial_dn = torch.eye(self.nbasis, self.ndown, dtype=F64, device=dev)
+        self.rchol_a, self.rchol_b = half_rotate_cholesky(self.chol, self.trial_up, self.trial_dn)
+        self.propagator = build_propagator(self.h1e_mod, self.chol, self.trial_up, self.trial_dn,
+                                           self.dt)
+        mf = self.propagator['mf_shift']
+        self.e_const = float(self.enuc + 0.5 * torch.sum(mf * mf).real)
+        self.eshift0 = eshift0
+        self.system = System(self.propagator, self.chol, self.rchol_a, self.rchol_b,
+                             self.trial_up, self.trial_dn, fbbound=fbbound, exp_nmax=exp_nmax,
+                             precision=precision)
+
+    # -- buffers ------------------------------------------------------------
+    def init_walkers(self, W):
+        dev = self.chol.device
+        M, Na, Nb = self.nbasis, self.nup, self.ndown
+        st = {
+            'phia': self.system.trial_up.unsqueeze(0).repeat(W, 1, 1).contiguous(),
+            'phib': self.system.trial_dn.unsqueeze(0).repeat(W, 1, 1).contiguous(),
+            'phia2': torch.empty((W, M, Na), dtype=C128, device=dev),
+            'phib2': torch.empty((W, M, Nb), dtype=C128, device=dev),
+            'weights': torch.ones(W, dtype=F64, device=dev),
+            'weights2': torch.empty(W, dtype=F64, device=dev),
+            'overlap': torch.ones(W, dtype=C128, device=dev),
+            'e_hybrid': torch.zeros(W, dtype=C128, device=dev),
+            'e_hybrid2': torch.empty(W, dtype=C128, device=dev),
+            'log_detR': torch.empty(W, dtype=F64, device=dev),
+            'idx': torch.empty(W, dtype=torch.int32, device=dev),
+            'acc': torch.zeros(2, dtype=F64, device=dev),
+            'stats': torch.zeros(4, dtype=F64, device=dev),
+        }
+        return st
+
+    def step(self, st, step_count, seed, eshift, stabilize_freq, pop_control_freq):
+        """One driver step (afqmc_gto.py:1387-1451) on preallocated buffers."""
+        L = _lib.load()
+        ctx = _lib.context().handle
+        W = st['phia'].shape[0]
+        M, Na, Nb = self.nbasis, self.nup, self.ndown
+        if step_count % stabilize_freq == 0:
+            check(L.afq_orthogonalize_walkers(ctx, W, M, Na, Nb, ptr(st['phia']), ptr(st['phib']),
+                                              ptr(st['log_detR']), None), "orthogonalize")
+        key = PhiloxKey(seed, step_count)
+        st['overlap'] = self.system.step(st['phia'], st['phib'], st['weights'], st['e_hybrid'],
+                                         eshift, key=key)
+        if step_count > 1:
+            check(L.afq_clip_weights(ctx, W, ptr(st['weights']), WEIGHT_CLIP_FRACTION), "clip")
+        if step_count % pop_control_freq == 0:
+            r = float(L.afq_comb_offset(seed & (2 ** 64 - 1), step_count))
+            check(L.afq_population_control_comb(ctx, W, M, Na, Nb, ptr(st['weights']),
+                                                ptr(st['phia']), ptr(st['phib']), r,
+                                                ptr(st['e_hybrid']), ptr(st['weights2']),
+                                                ptr(st['phia2']), ptr(st['phib2']),
+                                                ptr(st['e_hybrid2']), ptr(st['idx'])), "comb")
+            for a, b in (('phia', 'phia2'), ('phib', 'phib2'), ('weights', 'weights2'),
+                         ('e_hybrid', 'e_hybrid2')):
+                st[a], st[b] = st[b], st[a]
+        check(L.afq_accumulate_hybrid(ctx, W, ptr(st['weights']), ptr(st['e_hybrid']),
+                                      ptr(st['acc'])), "accumulate")
+
+    def block_energy(self, st):
+        """Mixed-estimator block energy (afqmc_gto.py:1466-1496); device values."""
+        e_tot, _, _ = self.system.local_energy(st['phia'], st['phib'], self.h1e, self.enuc)
+        L = _lib.load()
+        W = st['phia'].shape[0]
+        check(L.afq_block_stats(_lib.context().handle, W, ptr(st['weights']), ptr(e_tot),
+                                ptr(st['stats'])), "block_stats")
+        return e_tot
+
+    def __call__(self, rng_key=None, num_walkers=100, num_blocks=100, num_steps_per_block=25,
+                 stabilize_freq=5, pop_control_freq=5, num_blocks_equil=10, fname_log=None,
+                 walker_chunk_size=None, eshift0=None, local_energy_cap=False):
+        seed = rng_key.seed if isinstance(rng_key, PhiloxKey) else (42 if rng_key is None
+                                                                   else int(rng_key))
+        fout = sys.stdout if not fname_log else open(fname_log, 'w', 1)
+        st = self.init_walkers(num_walkers)
+        if eshift0 is None:
+            eshift0 = self.eshift0
+        eshift = 0.0 if eshift0 is None else float(eshift0) - self.e_const
+        ebound = float(np.sqrt(2.0 / self.dt))
+        step_count = 0
+        energy_blocks = []
+        n_capped = 0
+        t_prev = time.perf_counter()
+        for iblock in range(num_blocks_equil + num_blocks):
+            st['acc'].zero_()
+            for _ in range(num_steps_per_block):
+                step_count += 1
+                self.step(st, step_count, seed, eshift, stabilize_freq, pop_control_freq)
+            e_tot = self.block_energy(st)
+            if local_energy_cap:
+                e_ref = ((float(eshift0) if eshift0 is not None else self.ehf) if iblock == 0
+                         else eshift + self.e_const)
+                er = e_tot.real
+                n_capped += int(torch.sum(torch.abs(er - e_ref) > ebound))
+                er = torch.clamp(er, e_ref - ebound, e_ref + ebound)
+                w = torch.abs(st['weights'])
+                e_block = float(torch.sum(w * er) / torch.sum(w))
+                stats = st['stats'].cpu().numpy()
+            else:
+                stats = st['stats'].cpu().numpy()
+                e_block = float(stats[0])
+            energy_blocks.append(e_block)
+            acc = st['acc'].cpu().numpy()
+            if acc[0] > 1e-10:
+                eshift = float(acc[1] / acc[0])
+            if self.verbose:
+                t_now = time.perf_counter()
+                phase = "EQ" if iblock < num_blocks_equil else "  "
+                kish = stats[1] * stats[1] / stats[2] if stats[2] > 0 else 0.0
+                print(f"{phase}{iblock:4d} {e_block:16.10f} {eshift:16.10f} {stats[1]:12.4f} "
+                      f"{stats[3]:12.4f} {kish:>16.8e} {t_now - t_prev:>16.6f}", file=fout)
+                t_prev = t_now
+        energy_blocks = np.array(energy_blocks)
+        mean, err, std, kappa = do_binning_analysis(energy_blocks[num_blocks_equil:])
+        if fout is not sys.stdout:
+            fout.close()
+        out = {'energy_blocks': energy_blocks, 'energy_mean': float(mean),
+               'energy_err': float(err), 'energy_std': float(std), 'kappa': float(kappa),
+               'ehf': self.ehf, 'state': st}
+        if local_energy_cap:
+            out['local_energy_cap_bound'] = ebound
+            out['n_local_energy_capped'] = n_capped
+        return out
+
+
+def get_afqmc_func_from_integrals(h1, chol, ecore, nelec, mo_coeff=None, e_hf=None, dt=0.005,
+                                  trial=None, eshift0=None, verbose=True, chol_chunk_g=128,
+                                  det_chunk_size=5, multidet_method='explicit',
+                                  precision="fp64"):
+    """afqmc_gto.py:1592-1631 (single-determinant trial)."""
+    if trial is not None:
+        raise NotImplementedError("multi-determinant trials: use omegaqmc_amd.multidet")
+    integrals = prepare_afqmc_integrals_from_arrays(h1, chol, ecore, nelec, mo_coeff=mo_coeff)
+    return AFQMCDriver(integrals, dt=dt, e_hf=e_hf, eshift0=eshift0, verbose=verbose,
+                       precision=precision)
