@@ -1,0 +1,248 @@
+"""Walker-steps/s of the phaseless AFQMC step on one MI355X.
+
+Workload: M=300 orbitals, N=40+40 electrons, Nchol=1800, 1024 walkers,
+dt=0.005 (synthetic h1 / L_g generator, seed 42; random fields from the
+Philox stream).  A timed "step" is one driver step: re-orthonormalisation
+every 5 steps (before propagating), the fused phaseless propagation, weight
+clip and comb every 10 steps -- exactly afqmc_gto.py:1387-1451.  Inputs are
+resident in HBM before timing starts.
+
+Prints ONE JSON line.  ``value`` is the walker-steps/s of the fastest
+precision path that meets the energy bar; the FP64 GPU step is reported
+beside it, with the energy error of the reduced-precision path against the
+FP64 run of the same trajectory.  ``roofline`` is computed from HIP events
+recorded around each phase of the fused step during the timed region.
+``cpu_baseline`` times the NumPy FP64 oracle on this host on a bounded
+sample (a subset of walkers of the same system, one step).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "walker-steps/s at M=300 Nchol=1800: mixed-prec vs FP64 GPU at ≤0.5 mHa error"
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}  # dense TFLOP/s (MI355X)
+HBM_GBS = 8000.0
+
+
+def step_flops(M, Na, Nb, G, T=6):
+    """Algorithmic real flops per walker-step (complex x real = 4, complex x complex = 8)."""
+    f = 4.0 * G * M * M
+    for N in (Na, Nb):
+        f += (8 * T + 8) * M * M * N + 4.0 * G * N * M + 8.0 * N * N * M + 8.0 * M * N * N + 10.0 * N ** 3
+    return f
+
+
+def phase_flops(M, Na, Nb, G, W, T=6):
+    """Algorithmic flops of one launch-phase of the fused step, for W walkers."""
+    Ns = (Na, Nb)
+    return {
+        "greens": W * sum(8.0 * N * N * M + 8.0 * M * N * N + 10.0 * N ** 3 for N in Ns),
+        "one_body": W * sum(8.0 * M * M * N for N in Ns),       # both half steps
+        "force_bias": W * sum(4.0 * G * N * M for N in Ns),
+        "vhs": W * 4.0 * G * M * M,
+        "taylor": W * sum(8.0 * T * M * M * N for N in Ns),
+    }
+
+
+def cpu_info():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def run_gpu(sys_np, W, steps, warmup, precision, seed=42, timing=True):
+    import torch
+    from omegaqmc_amd import _lib
+    from omegaqmc_amd.driver import AFQMCDriver
+    dev = torch.device("cuda", 0)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    integ = {'h1e': t(sys_np.h1e), 'h1e_mod': t(sys_np.h1e_mod), 'chol': t(sys_np.chol),
+             'enuc': sys_np.enuc, 'nbasis': sys_np.nbasis, 'nup': sys_np.nup,
+             'ndown': sys_np.ndown}
+    drv = AFQMCDriver(integ, verbose=False, precision=precision)
+    st = drv.init_walkers(W)
+    L = _lib.load()
+    eshift = drv.ehf - drv.e_const  # eshift0 = E_HF (afqmc_gto.py:1357-1359)
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        drv.step(st, step, seed, eshift, 5, 10)
+    torch.cuda.synchronize()
+    if timing:
+        _lib.check(L.afq_system_timing_enable(drv.system.handle, 1), "timing")
+        tot = (_lib.ctypes.c_double * 16)()
+        cnt = (_lib.ctypes.c_int64 * 16)()
+        L.afq_system_timing_read(drv.system.handle, tot, cnt, 16, 1)  # reset
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        step += 1
+        drv.step(st, step, seed, eshift, 5, 10)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    phases = {}
+    if timing:
+        _lib.check(L.afq_system_timing_read(drv.system.handle, tot, cnt, 16, 1), "timing")
+        for i, name in enumerate(_lib.PHASES):
+            if cnt[i]:
+                phases[name] = {"ms_total": tot[i], "intervals": int(cnt[i])}
+        L.afq_system_timing_enable(drv.system.handle, 0)
+    # block energy of the final state (exchange kernel timed separately)
+    torch.cuda.synchronize()
+    te0 = torch.cuda.Event(enable_timing=True)
+    te1 = torch.cuda.Event(enable_timing=True)
+    te0.record()
+    e_tot = drv.block_energy(st)
+    te1.record()
+    torch.cuda.synchronize()
+    e_block = float(st['stats'][0].item())
+    return {"ms_per_step": ms / steps, "wall_s": wall, "phases": phases, "drv": drv, "state": st,
+            "energy_ms": te0.elapsed_time(te1), "e_block": e_block, "last_step": step}
+
+
+def cpu_baseline(sys_np, sample_walkers=8, nsteps=1):
+    """Oracle (NumPy FP64 restatement) on this host for a bounded sample."""
+    from oracle import afqmc as O
+    from oracle import philox as P
+    d = O.Driver(sys_np.h1e, sys_np.h1e_mod, sys_np.chol, sys_np.enuc, sys_np.nup, sys_np.ndown)
+    W = sample_walkers
+    pa = np.tile(d.trial_up[None].astype(complex), (W, 1, 1))
+    pb = np.tile(d.trial_dn[None].astype(complex), (W, 1, 1))
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    xi = P.aux_fields(42, 1, W, sys_np.naux)
+    O.propagate_walkers(pa, pb, w, None, eh, d.propagator, d.chol, d.rchol_a, d.rchol_b,
+                        d.trial_up, d.trial_dn, 0.0, xi)  # warm-up
+    t0 = time.perf_counter()
+    for s in range(nsteps):
+        xi = P.aux_fields(42, 2 + s, W, sys_np.naux)
+        pa, pb, w, _, eh = O.propagate_walkers(pa, pb, w, None, eh, d.propagator, d.chol,
+                                               d.rchol_a, d.rchol_b, d.trial_up, d.trial_dn,
+                                               0.0, xi)
+    dt = time.perf_counter() - t0
+    return W * nsteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--walkers", type=int, default=1024)
+    ap.add_argument("--precision", default="auto", help="fp64 | fp32 | bf16x3 | auto")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-walkers", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    import torch
+    from omegaqmc_amd import _lib
+    from omegaqmc_amd.synth import make_system
+    torch.cuda.set_device(0)
+    M, Na, Nb, G, W = 300, 40, 40, 1800, args.walkers
+    print(f"[bench] generating system M={M} N={Na}+{Nb} G={G}", file=sys.stderr, flush=True)
+    # size-normalised L_g variance: with the literal sigma the walker overlap
+    # overflows FP64 in the first step (see omegaqmc_amd.synth)
+    sys_np = make_system(M, Na, Nb, G, seed=42, rotate_device="cuda", normalise=True)
+
+    print("[bench] FP64 GPU step", file=sys.stderr, flush=True)
+    r64 = run_gpu(sys_np, W, args.steps, args.warmup, "fp64")
+    best, best_prec = r64, "fp64"
+    extra = {}
+    if args.precision in ("auto", "fp32", "bf16x3"):
+        cands = ["bf16x3", "fp32"] if args.precision == "auto" else [args.precision]
+        for prec in cands:
+            try:
+                print(f"[bench] {prec} GPU step", file=sys.stderr, flush=True)
+                r = run_gpu(sys_np, W, args.steps, args.warmup, prec)
+            except _lib.AFQError as e:
+                extra[f"{prec}_error"] = str(e)[:200]
+                continue
+            dE = abs(r["e_block"] - r64["e_block"])
+            extra[f"{prec}_walker_steps_per_s"] = W / (r["ms_per_step"] * 1e-3)
+            extra[f"{prec}_energy_err_mHa"] = dE * 1e3
+            if dE <= 0.5e-3 and r["ms_per_step"] < best["ms_per_step"]:
+                best, best_prec = r, prec
+    ws = W / (best["ms_per_step"] * 1e-3)
+
+    # roofline of the dominant phase of the fused step
+    pf = phase_flops(M, Na, Nb, G, W)
+    steps = args.steps
+    dom, dom_ms = None, -1.0
+    for name, v in best["phases"].items():
+        if name in pf and v["ms_total"] > dom_ms:
+            dom, dom_ms = name, v["ms_total"]
+    roof = None
+    if dom is not None:
+        per_launch_ms = dom_ms / best["phases"][dom]["intervals"]
+        flops = pf[dom]
+        if dom == "one_body":
+            flops /= 2  # two timed intervals per step, one half step each
+        ach = flops / (per_launch_ms * 1e-3) / 1e12
+        peak = PEAK["fp64"] if best_prec == "fp64" else (PEAK["fp32"] if best_prec == "fp32"
+                                                         else PEAK["bf16"] / 3)
+        roof = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
+                "ms_per_launch": round(per_launch_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        model, ncpu = cpu_info()
+        os.environ.setdefault("OMP_NUM_THREADS", str(ncpu))
+        print(f"[bench] CPU baseline ({args.cpu_walkers} walkers, oracle)", file=sys.stderr, flush=True)
+        rate, secs = cpu_baseline(sys_np, args.cpu_walkers)
+        cpu = {"value": round(rate, 4), "unit": "walker-steps/s", "cores": ncpu, "kind": "port",
+               "sample": f"NumPy FP64 oracle, {args.cpu_walkers} walkers x 1 step of the same "
+                         f"M=300 system ({secs:.1f} s), BLAS threads={ncpu}, {model}"}
+
+    flop_ws = step_flops(M, Na, Nb, G)
+    out = {
+        "metric": METRIC, "value": round(ws, 2), "unit": "walker-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(best["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32-mfma+f64", "bf16x3": "bf16x3-f32acc+f64"}[best_prec],
+        "data": "synthetic",
+        "config": {"workload": "phaseless AFQMC step, M=300 N=40+40 Nchol=1800 W=1024 dt=0.005",
+                   "walkers": W, "nbasis": M, "nup": Na, "ndown": Nb, "naux": G,
+                   "precision": best_prec},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64_walker_steps_per_s": round(W / (r64["ms_per_step"] * 1e-3), 2),
+        "speedup_vs_fp64": round(r64["ms_per_step"] / best["ms_per_step"], 3),
+        "algorithmic_tflops": round(flop_ws * ws / 1e12, 3),
+        "block_energy_ms": round(best["energy_ms"], 2),
+        "phases_ms_per_step": {k: round(v["ms_total"] / steps, 4) for k, v in best["phases"].items()},
+        **extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -222,6 +222,16 @@ int afq_system_local_energy(afq_system* sys, int nwalkers, const double* phia,
                             const double* phib, const double* h1e, double enuc,
                             double* e_tot, double* e_1b, double* e_2b);
 
+/* Per-phase timing of afq_propagate_walkers with HIP events on the system's
+ * stream (no counterpart in the reference, which times whole blocks with
+ * perf_counter at afqmc_gto.py:1380-1382).  Phases, in order: greens,
+ * one-body, force-bias GEMM, shift, VHS build, Taylor, overlap, weights,
+ * aux-field RNG.  read() synchronises the stream and returns the summed
+ * milliseconds and the number of timed intervals per phase. */
+int afq_system_timing_enable(afq_system* sys, int on);
+int afq_system_timing_read(afq_system* sys, double* total_ms, int64_t* launches, int nphase,
+                           int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,7 +61,11 @@ SIGNATURES = {
     "afq_system_destroy": [_P],
     "afq_propagate_walkers": [_P, _I, _P, _P, _P, _P, _D, _P, _U64, _I64, _P],
     "afq_system_local_energy": [_P, _I, _P, _P, _P, _D, _P, _P, _P],
+    "afq_system_timing_enable": [_P, _I],
+    "afq_system_timing_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64), _I, _I],
 }
+
+PHASES = ("greens", "one_body", "force_bias", "shift", "vhs", "taylor", "overlap", "weights", "rng")
 _RESTYPE = {"afq_last_error": ctypes.c_char_p, "afq_comb_offset": _D}
 
 _lib = None

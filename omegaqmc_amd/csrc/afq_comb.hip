@@ -13,14 +13,13 @@ namespace afq {
 namespace {
 
 constexpr int NT = 1024;
-constexpr int LDS_W = 16384;  // weights staged in LDS up to this many walkers
 
 __global__ __launch_bounds__(NT) void k_comb_index(int W, const double* __restrict__ w, double r,
                                                    double* __restrict__ cum_g,
                                                    int32_t* __restrict__ idx,
                                                    double* __restrict__ w_new) {
   extern __shared__ double cum_s[];
-  const bool in_lds = (W <= LDS_W);
+  const bool in_lds = (cum_g == nullptr);
   double* cum = in_lds ? cum_s : cum_g;
   for (int i = threadIdx.x; i < W; i += NT) cum[i] = w[i];
   __syncthreads();
@@ -138,13 +137,10 @@ int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weigh
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
                 double* cum_scratch) {
   if (W <= 0) return AFQ_OK;
-  const size_t sm = W <= LDS_W ? sizeof(double) * W : 0;
-  static bool attr = false;
-  if (!attr) {
-    AFQ_CHECK_HIP(hipFuncSetAttribute((const void*)k_comb_index,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr = true;
-  }
+  const bool lds = sizeof(double) * (size_t)W + 1024 <= max_lds_bytes();
+  if (!lds && !cum_scratch) return fail(AFQ_EINVAL, "comb: W=%d needs a scratch buffer", W);
+  const size_t sm = lds ? sizeof(double) * W : 0;
+  if (lds) cum_scratch = nullptr;
   hipLaunchKernelGGL(k_comb_index, dim3(1), dim3(NT), sm, s, W, weights, r, cum_scratch, idx, w_new);
   AFQ_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, (long)M * Na, (long)M * Nb, idx,
@@ -192,7 +188,7 @@ int afq_population_control_comb(afq_ctx* ctx, int W, int M, int Na, int Nb, cons
   AFQ_REQUIRE(idx != nullptr && w_new != nullptr, "idx and weights_new are required");
   AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
   double* cum = nullptr;
-  if (W > 16384) {
+  if (sizeof(double) * (size_t)W + 1024 > afq::max_lds_bytes()) {
     cum = (double*)afq::scratch(ctx, sizeof(double) * W);
     if (!cum) return AFQ_ENOMEM;
   }

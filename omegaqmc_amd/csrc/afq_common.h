@@ -82,10 +82,25 @@ struct afq_system {
   // reduced-precision operand copies (owned)
   void* chol_lp = nullptr;     // precision-specific packed L
   void* rchol_lp = nullptr;    // precision-specific packed Lhalf (both spins)
+  // FP64 VHS operand: upper triangle of every L_g (G x P) and its column map
+  double* chol_packed = nullptr;
+  int2* colmap = nullptr;
+  int packed_P = 0;
+  int fb_nsplit = 1;
+  // greens mode bits: 2 if trial_up == eye(M, Na) exactly, 4 for trial_dn
+  int eye_mask = 0;
   // step workspace (owned), sized for max_walkers
   int cap_walkers = 0;
   void* work = nullptr;
   size_t work_bytes = 0;
+  // optional per-phase HIP-event timing of the fused step
+  struct Timing* timing = nullptr;
+};
+
+// Phases of the fused step timed by afq_system_timing_*.
+enum afq_phase {
+  PH_GREENS = 0, PH_ONEBODY, PH_FORCEBIAS, PH_SHIFT, PH_VHS, PH_TAYLOR, PH_OVERLAP, PH_WEIGHTS,
+  PH_RNG, PH_NPHASE
 };
 
 namespace afq {
@@ -95,6 +110,8 @@ int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 // Ensure the context scratch holds at least `bytes`; returns pointer or null.
 void* scratch(afq_ctx* ctx, size_t bytes);
+// Largest dynamic LDS allocation a workgroup may request on the current device.
+size_t max_lds_bytes();
 
 }  // namespace afq
 

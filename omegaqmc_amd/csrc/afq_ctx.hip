@@ -47,6 +47,17 @@ void* scratch(afq_ctx* ctx, size_t bytes) {
   return ctx->scratch;
 }
 
+size_t max_lds_bytes() {
+  static size_t cached = 0;
+  if (cached) return cached;
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    v = 65536;
+  cached = (size_t)v;
+  return cached;
+}
+
 }  // namespace afq
 
 extern "C" {

@@ -110,10 +110,12 @@ __device__ void lu_inverse(const cplx* LU, const int* perm, int N, cplx* Inv) {
   }
 }
 
-// One spin of one walker.  mode 0: overlap only; 1: + Ghalf.
-__device__ void greens_spin(int mode, int M, int N, int QCH, const cplx* __restrict__ phi,
-                            const cplx* __restrict__ trial, cplx* __restrict__ Ghalf,
-                            cplx* ov_out, char* smem) {
+// One spin of one walker.  mode 0: overlap only; 1: + Ghalf.  eye != 0
+// asserts trial == eye(M, N) exactly, so S[i][j] = phi[j][i] (the products
+// with the zero entries of the trial contribute exact zeros).
+__device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
+                            const cplx* __restrict__ phi, const cplx* __restrict__ trial,
+                            cplx* __restrict__ Ghalf, cplx* ov_out, char* smem) {
   const int t = threadIdx.x;
   if (N == 0) {
     if (t == 0) *ov_out = mkc(1.0, 0.0);
@@ -123,16 +125,41 @@ __device__ void greens_spin(int mode, int M, int N, int QCH, const cplx* __restr
   cplx* LU = (cplx*)smem;
   cplx* Inv = LU + N * N;
   cplx* slab = Inv + N * N;
-  int* perm = (int*)(slab + QCH * N);
+  cplx* slab_t = slab + QCH * N;
+  int* perm = (int*)(slab_t + QCH * N);
   int* red_i = perm + 64;
   double* red_v = (double*)(red_i + 4);
 
-  // S[i][j] = sum_p phi[p][i] conj(trial[p][j])
-  for (int e = t; e < N * N; e += NT) {
-    int i = e / N, j = e % N;
-    cplx acc = mkc(0.0, 0.0);
-    for (int p = 0; p < M; ++p) cfma(acc, phi[(long)p * N + i], cconj(trial[(long)p * N + j]));
-    LU[e] = acc;
+  if (eye) {
+    for (int e = t; e < N * N; e += NT) LU[e] = phi[(long)(e % N) * N + e / N];
+  } else {
+    // S[i][j] = sum_p phi[p][i] conj(trial[p][j]), p staged through LDS
+    constexpr int RMAX = (64 * 64 + NT - 1) / NT;
+    cplx acc[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) acc[r] = mkc(0.0, 0.0);
+    for (int q0 = 0; q0 < M; q0 += QCH) {
+      const int nq = min(QCH, M - q0);
+      for (int e = t; e < nq * N; e += NT) {
+        slab[e] = phi[(long)q0 * N + e];
+        slab_t[e] = trial[(long)q0 * N + e];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r) {
+        const int e = t + NT * r;
+        if (e < N * N) {
+          const int i = e / N, j = e % N;
+          for (int q = 0; q < nq; ++q) cfma(acc[r], slab[q * N + i], cconj(slab_t[q * N + j]));
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int e = t + NT * r;
+      if (e < N * N) LU[e] = acc[r];
+    }
   }
   __syncthreads();
   lu_slogdet(LU, perm, N, ov_out, red_v, red_i);
@@ -165,20 +192,22 @@ __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, 
   extern __shared__ __align__(16) char smem[];
   __shared__ cplx ov[2];
   const long w = blockIdx.x;
-  greens_spin(mode, M, Na, qch, phia + w * M * Na, tu, mode ? Gha + w * Na * M : nullptr, &ov[0], smem);
+  greens_spin(mode & 1, (mode >> 1) & 1, M, Na, qch, phia + w * M * Na, tu,
+              (mode & 1) ? Gha + w * Na * M : nullptr, &ov[0], smem);
   __syncthreads();
-  greens_spin(mode, M, Nb, qch, phib + w * M * Nb, td, mode ? Ghb + w * Nb * M : nullptr, &ov[1], smem);
+  greens_spin(mode & 1, (mode >> 2) & 1, M, Nb, qch, phib + w * M * Nb, td,
+              (mode & 1) ? Ghb + w * Nb * M : nullptr, &ov[1], smem);
   __syncthreads();
   if (threadIdx.x == 0) ovlp[w] = cmul(ov[0], ov[1]);
 }
 
 }  // namespace
 
-static int greens_qch(int Na, int Nb) { return (Na > 48 || Nb > 48) ? 16 : 32; }
+static int greens_qch(int Na, int Nb) { return (Na > 48 || Nb > 48) ? 8 : 16; }
 
 size_t greens_smem_bytes(int Na, int Nb) {
   int N = Na > Nb ? Na : Nb;
-  return (size_t)(2 * N * N + greens_qch(Na, Nb) * N) * sizeof(cplx) + 64 * sizeof(int) + 4 * sizeof(int) +
+  return (size_t)(2 * N * N + 2 * greens_qch(Na, Nb) * N) * sizeof(cplx) + 64 * sizeof(int) + 4 * sizeof(int) +
          64 * sizeof(double) + 64;
 }
 
@@ -188,12 +217,9 @@ int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const d
   if (Na > 64 || Nb > 64) return fail(AFQ_EUNSUPPORTED, "nocc > 64 is not supported (got %d, %d)", Na, Nb);
   if (W <= 0) return AFQ_OK;
   size_t sm = greens_smem_bytes(Na, Nb);
-  static bool attr_set = false;
-  if (!attr_set) {
-    AFQ_CHECK_HIP(hipFuncSetAttribute((const void*)k_greens,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr_set = true;
-  }
+  if (sm + 64 > max_lds_bytes())
+    return fail(AFQ_EUNSUPPORTED, "greens: %zu bytes of LDS needed, device allows %zu", sm,
+                max_lds_bytes());
   hipLaunchKernelGGL(k_greens, dim3(W), dim3(NT), sm, s, mode, M, Na, Nb, greens_qch(Na, Nb),
                      (const cplx*)phia,
                      (const cplx*)phib, (const cplx*)tu, (const cplx*)td, (cplx*)Gha, (cplx*)Ghb,

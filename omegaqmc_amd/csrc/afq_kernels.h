@@ -18,7 +18,7 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
                          const double* rb, const double* Gha, const double* Ghb, double* vbiasT);
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
-                 double* cmf);
+                 double* cmf, int nsplit);
 int build_vhs_fp64(hipStream_t s, int W, int M, int G, const double* xs, const double* chol,
                    double dt, double* VHS);
 int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, const double* phi,

@@ -1,0 +1,35 @@
+// MFMA kernel launchers (FP64 in afq_mfma64.hip, reduced precision in afq_lowp.hip).
+#pragma once
+#include "afq_common.h"
+
+namespace afq {
+
+constexpr int FB_MAX_SPLITS = 8;
+
+// K-splits of the force-bias contraction; each split reads one spin.
+struct FbSplits {
+  const cplx* gh[FB_MAX_SPLITS];
+  const double* lh[FB_MAX_SPLITS];
+  long K[FB_MAX_SPLITS];
+  int kbeg[FB_MAX_SPLITS];
+  int kend[FB_MAX_SPLITS];
+  int n;
+};
+
+FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, const double* Lha,
+                   const double* Lhb, int per_spin);
+
+// VHS (W,M,M) complex from xs (W,G) complex and L (G,P) real; colmap (P) maps a
+// packed column to its two (p,q)/(q,p) offsets (NULL: P = M*M, identity).
+int vhs_mfma_f64(hipStream_t s, int W, int M, int G, const double* xs, const double* L, int P,
+                 const int2* colmap, double dt, double* VHS);
+// part[s] (W,G) complex partial force bias of split s.
+int fb_mfma_f64(hipStream_t s, int W, int G, const FbSplits& sp, double* part);
+// phi_out <- expH1 . Taylor_nmax(VHS) . expH1 . phi_in (both spins); expH1 NULL
+// skips the one-body products.  phi_out may alias phi_in.
+int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
+                  const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
+                  double* pb_out, int nmax);
+size_t prop_f64_smem(int M);
+
+}  // namespace afq

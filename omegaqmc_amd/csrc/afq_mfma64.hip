@@ -1,0 +1,519 @@
+// FP64 MFMA kernels (v_mfma_f64_16x16x4f64) for the three dense contractions
+// of the phaseless step (OmegaQMC/afqmc_gto.py:451-482, 682-683):
+//
+//   k_vhs_f64   VHS[w] = i sqrt(dt) sum_g x[w,g] L_g, as a real GEMM whose rows
+//               are (walker, re/im) pairs and whose columns are the packed upper
+//               triangle of the symmetric L_g (both triangles of VHS written).
+//   k_fb_f64    vbias[w,g] = sum_k Ghalf[w,k] Lhalf[g,k], split over K with
+//               per-split partial sums reduced in a fixed order afterwards.
+//   k_prop_f64  phi <- expH1 . sum_{n<=nmax} VHS^n/n! . expH1 . phi, fused: one
+//               workgroup owns a 16-column block of one walker for the whole
+//               chain, keeping that block in LDS between the 2 + nmax products.
+//
+// MFMA operand layout (gfx950, checked on the device): A[i][k] in lane i + 16k,
+// B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i % 4), register i / 4.
+// So a lane holds rows q, q+4, q+8, q+12 of a 16-row tile (q = lane / 16).
+// GEMM rows that pair (walker re, walker im) are therefore ordered so that a
+// lane's four rows are (w, re), (w, im), (w+4, re), (w+4, im).
+//
+// Workgroups are renumbered so that the hardware's round-robin assignment of
+// consecutive workgroups to the 8 XCDs puts tiles that share an operand
+// (same L column tile / same walker) on one XCD, where they share its L2.
+#include "afq_mfma.h"
+
+namespace afq {
+
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Virtual block index such that consecutive virtual indices run on one XCD.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int x = b & 7, slot = b >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// Row of the 16-row tile that holds component c (0 re, 1 im) of walker wi (0..7).
+__device__ __forceinline__ int pair_row(int wi, int c) { return (wi & 3) + 8 * (wi >> 2) + 4 * c; }
+
+constexpr int TM = 128, TN = 128, TK = 16, NTH = 256;
+constexpr int PNW = 8, PTH = 64 * PNW;  // waves / threads of the fused propagator
+
+// --------------------------------------------------------------------------
+// VHS build
+// --------------------------------------------------------------------------
+template <bool PACKED>
+__global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
+                                                 const cplx* __restrict__ xs,
+                                                 const double* __restrict__ L,
+                                                 const int2* __restrict__ colmap, double sqdt,
+                                                 cplx* __restrict__ VHS, int ntm, int ntn) {
+  __shared__ double As[TK][TM + 4];
+  __shared__ double Bs[TK][TN + 4];
+  const int v = xcd_remap(blockIdx.x, ntm * ntn);
+  const int tm = v % ntm, tn = v / ntm;
+  const int w0 = tm * (TM / 2), n0 = tn * TN;
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+
+  cplx ra[4];
+  double rb[8];
+  auto load = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + NTH * i, wl = e >> 4, gk = e & 15;
+      const int w = w0 + wl, g = g0 + gk;
+      ra[i] = (w < W && g < G) ? xs[(long)w * G + g] : mkc(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = t + NTH * i, kk = e >> 7, nn = e & 127;
+      const int g = g0 + kk, n = n0 + nn;
+      rb[i] = (g < G && n < P) ? L[(long)g * P + n] : 0.0;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + NTH * i, wl = e >> 4, gk = e & 15;
+      const int row = 16 * (wl >> 3) + pair_row(wl & 7, 0);
+      As[gk][row] = ra[i].x;
+      As[gk][row + 4] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = t + NTH * i;
+      Bs[e >> 7][e & 127] = rb[i];
+    }
+  };
+
+  d4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
+
+  const int nk = (G + TK - 1) / TK;
+  load(0);
+  store();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load((kt + 1) * TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kl = 4 * ks + (l >> 4);
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kl][64 * wr + 16 * i + (l & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kl][64 * wc + 16 * j + (l & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  const int q = l >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int tile = 4 * wr + i;
+    const int wA = w0 + 8 * tile + q, wB = wA + 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + 64 * wc + 16 * j + (l & 15);
+      if (n >= P) continue;
+      long o1 = n, o2 = -1;
+      if (PACKED) {
+        const int2 cm = colmap[n];
+        o1 = cm.x;
+        o2 = cm.y;
+      }
+      const d4v c = acc[i][j];
+      if (wA < W) {
+        const cplx val = mkc(-sqdt * c[1], sqdt * c[0]);
+        VHS[wA * MM + o1] = val;
+        if (o2 >= 0) VHS[wA * MM + o2] = val;
+      }
+      if (wB < W) {
+        const cplx val = mkc(-sqdt * c[3], sqdt * c[2]);
+        VHS[wB * MM + o1] = val;
+        if (o2 >= 0) VHS[wB * MM + o2] = val;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// force bias, split K
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx* __restrict__ part,
+                                                int ntm, int ntn) {
+  __shared__ double As[TK][TM + 4];
+  __shared__ double Bs[TK][TN + 4];
+  const int s = blockIdx.y;
+  const int v = xcd_remap(blockIdx.x, ntm * ntn);
+  const int tm = v % ntm, tn = v / ntm;
+  const int w0 = tm * (TM / 2), n0 = tn * TN;
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+  const cplx* __restrict__ Gh = sp.gh[s];
+  const double* __restrict__ Lh = sp.lh[s];
+  const long K = sp.K[s];
+  const int kb = sp.kbeg[s], ke = sp.kend[s];
+
+  cplx ra[4];
+  double rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + NTH * i, wl = e >> 4, kk = e & 15;
+      const int w = w0 + wl, k = k0 + kk;
+      ra[i] = (w < W && k < ke) ? Gh[(long)w * K + k] : mkc(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = t + NTH * i, gl = e >> 4, kk = e & 15;
+      const int g = n0 + gl, k = k0 + kk;
+      rb[i] = (g < G && k < ke) ? Lh[(long)g * K + k] : 0.0;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + NTH * i, wl = e >> 4, kk = e & 15;
+      const int row = 16 * (wl >> 3) + pair_row(wl & 7, 0);
+      As[kk][row] = ra[i].x;
+      As[kk][row + 4] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = t + NTH * i;
+      Bs[e & 15][e >> 4] = rb[i];
+    }
+  };
+
+  d4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
+
+  const int nk = (ke - kb + TK - 1) / TK;
+  if (nk > 0) {
+    load(kb);
+    store();
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load(kb + (kt + 1) * TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kl = 4 * ks + (l >> 4);
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kl][64 * wr + 16 * i + (l & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kl][64 * wc + 16 * j + (l & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  const int q = l >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int tile = 4 * wr + i;
+    const int wA = w0 + 8 * tile + q, wB = wA + 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int g = n0 + 64 * wc + 16 * j + (l & 15);
+      if (g >= G) continue;
+      const d4v c = acc[i][j];
+      if (wA < W) part[((long)s * W + wA) * G + g] = mkc(c[0], c[1]);
+      if (wB < W) part[((long)s * W + wB) * G + g] = mkc(c[2], c[3]);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// fused one-body / Taylor / one-body on a 16-column block of one walker
+// --------------------------------------------------------------------------
+// E == nullptr skips both one-body products (plain _apply_exp_vhs).  The
+// output may alias the input: a workgroup reads and writes only its own
+// columns, reading all of them before writing any.
+template <int MT>
+__global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, int ncb,
+                                                  const double* __restrict__ E,
+                                                  const cplx* __restrict__ V,
+                                                  const cplx* pa_in, const cplx* pb_in,
+                                                  cplx* pa_out, cplx* pb_out, int nmax) {
+  extern __shared__ __align__(16) char smem_raw[];
+  cplx* Ts = (cplx*)smem_raw;  // [Mpad][16]
+  const int RT = (M + 15) >> 4, Mpad = RT * 16;
+  const int vb = xcd_remap(blockIdx.x, W * ncb);
+  const int w = vb / ncb, cb = vb % ncb;
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int li = l & 15, lq = l >> 4;
+  const int Nt = Na + Nb;
+  const long MM = (long)M * M;
+  const cplx* __restrict__ Vw = V + (long)w * MM;
+
+  // element offset of (row, column c of this block) in the spin array, or -1
+  auto coloff = [&](int c, long row, bool* spin_b) -> long {
+    const int cg = 16 * cb + c;
+    *spin_b = cg >= Na;
+    if (cg < Na) return ((long)w * M + row) * Na + cg;
+    if (cg < Nt) return ((long)w * M + row) * Nb + (cg - Na);
+    return -1;
+  };
+
+  // stage the input block
+  for (int e = t; e < Mpad * 16; e += PTH) {
+    const int q = e >> 4, c = e & 15;
+    bool sb = false;
+    const long o = (q < M) ? coloff(c, q, &sb) : -1;
+    Ts[e] = (o >= 0) ? (sb ? pb_in[o] : pa_in[o]) : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+
+  d4v are[MT], aim[MT], rre[MT], rim[MT];
+  auto owns = [&](int u) { return wv + PNW * u < RT; };
+  auto arow = [&](int u) { return 16 * (wv + PNW * u) + li; };
+
+  // acc = E (real, M x M) . Ts; the A fragments of the next k-step are
+  // loaded before the MFMAs of the current one.
+  auto onebody = [&]() {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) { are[u] = d4v{0, 0, 0, 0}; aim[u] = d4v{0, 0, 0, 0}; }
+    double a0[MT], a1[MT];
+    auto ld = [&](double* a, int k0) {
+      const int k = k0 + lq;
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        const int row = arow(u);
+        a[u] = (row < M && k < M) ? E[(long)row * M + k] : 0.0;
+      }
+    };
+    auto mm = [&](const double* a, int k0) {
+      const cplx b = Ts[(k0 + lq) * 16 + li];
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) are[u] = mfma64(a[u], b.x, are[u]);
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) aim[u] = mfma64(a[u], b.y, aim[u]);
+    };
+    ld(a0, 0);
+    for (int k0 = 0; k0 < Mpad; k0 += 8) {
+      ld(a1, k0 + 4);
+      mm(a0, k0);
+      if (k0 + 8 < Mpad) ld(a0, k0 + 8);
+      mm(a1, k0 + 4);
+    }
+  };
+  // acc = V_w (complex) . Ts, same prefetch scheme
+  auto taylor_term = [&]() {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) { are[u] = d4v{0, 0, 0, 0}; aim[u] = d4v{0, 0, 0, 0}; }
+    cplx a0[MT], a1[MT];
+    auto ld = [&](cplx* a, int k0) {
+      const int k = k0 + lq;
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        const int row = arow(u);
+        a[u] = (row < M && k < M) ? Vw[(long)row * M + k] : mkc(0.0, 0.0);
+      }
+    };
+    auto mm = [&](const cplx* a, int k0) {
+      const cplx b = Ts[(k0 + lq) * 16 + li];
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) are[u] = mfma64(a[u].x, b.x, are[u]);
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) aim[u] = mfma64(a[u].x, b.y, aim[u]);
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) are[u] = mfma64(-a[u].y, b.y, are[u]);
+#pragma unroll
+      for (int u = 0; u < MT; ++u)
+        if (owns(u)) aim[u] = mfma64(a[u].y, b.x, aim[u]);
+    };
+    ld(a0, 0);
+    for (int k0 = 0; k0 < Mpad; k0 += 8) {
+      ld(a1, k0 + 4);
+      mm(a0, k0);
+      if (k0 + 8 < Mpad) ld(a0, k0 + 8);
+      mm(a1, k0 + 4);
+    }
+  };
+  // write acc rows (< M) into Ts
+  auto to_lds = [&](const d4v* xr, const d4v* xi) {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      if (!owns(u)) continue;
+      const int tile = wv + PNW * u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * tile + lq + 4 * r;
+        if (row < M) Ts[row * 16 + li] = mkc(xr[u][r], xi[u][r]);
+      }
+    }
+  };
+
+  if (E) {
+    onebody();
+    __syncthreads();
+    to_lds(are, aim);
+  } else {
+    // the result starts as the input block itself
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      const int tile = wv + PNW * u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * tile + lq + 4 * r;
+        const cplx x = owns(u) ? Ts[row * 16 + li] : mkc(0.0, 0.0);
+        are[u][r] = x.x;
+        aim[u][r] = x.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < MT; ++u) { rre[u] = are[u]; rim[u] = aim[u]; }
+  __syncthreads();
+  for (int n = 1; n <= nmax; ++n) {
+    taylor_term();
+    const double dn = (double)n;
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        are[u][r] = are[u][r] / dn;
+        aim[u][r] = aim[u][r] / dn;
+        rre[u][r] = rre[u][r] + are[u][r];
+        rim[u][r] = rim[u][r] + aim[u][r];
+      }
+    __syncthreads();
+    to_lds(are, aim);
+    __syncthreads();
+  }
+  if (E) {
+    to_lds(rre, rim);
+    __syncthreads();
+    onebody();
+  } else {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) { are[u] = rre[u]; aim[u] = rim[u]; }
+  }
+#pragma unroll
+  for (int u = 0; u < MT; ++u) {
+    if (!owns(u)) continue;
+    const int tile = wv + PNW * u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * tile + lq + 4 * r;
+      bool sb = false;
+      const long o = (row < M) ? coloff(li, row, &sb) : -1;
+      if (o >= 0) (sb ? pb_out : pa_out)[o] = mkc(are[u][r], aim[u][r]);
+    }
+  }
+}
+
+}  // namespace
+
+int vhs_mfma_f64(hipStream_t s, int W, int M, int G, const double* xs, const double* L, int P,
+                 const int2* colmap, double dt, double* VHS) {
+  if (W <= 0) return AFQ_OK;
+  const int ntm = cdiv_int(2L * W, TM), ntn = cdiv_int(P, TN);
+  const int nb = ntm * ntn;
+  if (colmap)
+    hipLaunchKernelGGL((k_vhs_f64<true>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
+                       (const cplx*)xs, L, colmap, sqrt(dt), (cplx*)VHS, ntm, ntn);
+  else
+    hipLaunchKernelGGL((k_vhs_f64<false>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
+                       (const cplx*)xs, L, colmap, sqrt(dt), (cplx*)VHS, ntm, ntn);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, const double* Lha,
+                   const double* Lhb, int per_spin) {
+  FbSplits sp{};
+  int n = 0;
+  const int Ns[2] = {Na, Nb};
+  const double* gh[2] = {Gha, Ghb};
+  const double* lh[2] = {Lha, Lhb};
+  for (int s = 0; s < 2; ++s) {
+    const long K = (long)Ns[s] * M;
+    if (K == 0) continue;
+    const long chunk = ((K + per_spin - 1) / per_spin + TK - 1) / TK * TK;
+    for (long k0 = 0; k0 < K && n < FB_MAX_SPLITS; k0 += chunk) {
+      sp.gh[n] = (const cplx*)gh[s];
+      sp.lh[n] = lh[s];
+      sp.K[n] = K;
+      sp.kbeg[n] = (int)k0;
+      sp.kend[n] = (int)std::min(K, k0 + chunk);
+      ++n;
+    }
+  }
+  sp.n = n;
+  return sp;
+}
+
+int fb_mfma_f64(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
+  if (W <= 0 || sp.n == 0) return AFQ_OK;
+  const int ntm = cdiv_int(2L * W, TM), ntn = cdiv_int(G, TN);
+  hipLaunchKernelGGL(k_fb_f64, dim3(ntm * ntn, sp.n), dim3(NTH), 0, s, W, G, sp, (cplx*)part, ntm,
+                     ntn);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+size_t prop_f64_smem(int M) { return (size_t)((M + 15) / 16) * 16 * 16 * sizeof(cplx); }
+
+int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
+                  const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
+                  double* pb_out, int nmax) {
+  const int Nt = Na + Nb;
+  if (W <= 0 || Nt == 0) return AFQ_OK;
+  const int RT = (M + 15) / 16;
+  const int ncb = (Nt + 15) / 16;
+  const size_t sm = prop_f64_smem(M);
+  if (sm > max_lds_bytes())
+    return fail(AFQ_EUNSUPPORTED, "fused propagator: M=%d needs %zu bytes of LDS", M, sm);
+  const int nb = W * ncb;
+  const dim3 grid(nb), blk(PTH);
+#define AFQ_LAUNCH_PROP(MT)                                                                   \
+  hipLaunchKernelGGL((k_prop_f64<MT>), grid, blk, sm, s, W, M, Na, Nb, ncb, expH1,           \
+                     (const cplx*)VHS, (const cplx*)pa_in, (const cplx*)pb_in, (cplx*)pa_out, \
+                     (cplx*)pb_out, nmax)
+  if (RT <= PNW) AFQ_LAUNCH_PROP(1);
+  else if (RT <= 2 * PNW) AFQ_LAUNCH_PROP(2);
+  else if (RT <= 3 * PNW) AFQ_LAUNCH_PROP(3);
+  else if (RT <= 4 * PNW) AFQ_LAUNCH_PROP(4);
+  else return fail(AFQ_EUNSUPPORTED, "fused propagator: M=%d > 512", M);
+#undef AFQ_LAUNCH_PROP
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+}  // namespace afq

@@ -5,6 +5,7 @@
 // fused, precision-selectable step lives in afq_step.hip.
 #include "afq_gemm.h"
 #include "afq_kernels.h"
+#include "afq_mfma.h"
 
 namespace afq {
 
@@ -13,7 +14,8 @@ namespace {
 constexpr int NT = 256;
 
 // afqmc_gto.py:687-706 per walker (one workgroup): cap, shift, cfb, cmf.
-__global__ __launch_bounds__(NT) void k_shift(int G, const cplx* __restrict__ vbias,  // (W,G)
+__global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
+                                              const cplx* __restrict__ vbias,  // nsplit x (W,G)
                                               const cplx* __restrict__ mf,            // (G,)
                                               const double* __restrict__ xi,          // (W,G)
                                               double sqdt, double fb, cplx* __restrict__ xbar_out,
@@ -23,7 +25,8 @@ __global__ __launch_bounds__(NT) void k_shift(int G, const cplx* __restrict__ vb
   const long w = blockIdx.x;
   cplx s_mf = mkc(0, 0), s_xixb = mkc(0, 0), s_xb2 = mkc(0, 0);
   for (int g = threadIdx.x; g < G; g += NT) {
-    const cplx v = vbias[w * G + g];
+    cplx v = vbias[w * G + g];
+    for (int sp = 1; sp < nsplit; ++sp) v = cadd(v, vbias[sp * sstride + w * G + g]);
     const cplx m = mf[g];
     cplx xb = mkc(sqdt * (v.y + m.x), -sqdt * (v.x - m.y));
     const double a = hypot(xb.x, xb.y);
@@ -113,9 +116,9 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
 
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
-                 double* cmf) {
+                 double* cmf, int nsplit) {
   if (W <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, (const cplx*)vbiasT, (const cplx*)mf, xi,
+  hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, nsplit, (long)W * G, (const cplx*)vbiasT, (const cplx*)mf, xi,
                      sqrt(dt), fb, (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
@@ -188,13 +191,16 @@ int afq_build_vhs(afq_ctx* ctx, int W, int M, int G, const double* xs, const dou
                   double dt, double* VHS) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && G >= 0, "bad shape");
-  return afq::build_vhs_fp64(ctx->stream, W, M, G, xs, chol, dt, VHS);
+  return afq::vhs_mfma_f64(ctx->stream, W, M, G, xs, chol, M * M, nullptr, dt, VHS);
 }
 
 int afq_apply_exp_vhs(afq_ctx* ctx, int W, int M, int N, const double* VHS, const double* phi,
                       int nmax, double* phi_out) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && N >= 0 && nmax >= 0, "bad shape");
+  if (M <= 512 && afq::prop_f64_smem(M) <= afq::max_lds_bytes())
+    return afq::prop_mfma_f64(ctx->stream, W, M, N, 0, nullptr, VHS, phi, nullptr, phi_out,
+                              nullptr, nmax);
   size_t one = align256(sizeof(double) * 2 * (size_t)W * M * N);
   double* tmp = (double*)afq::scratch(ctx, 2 * one);
   if (!tmp && one) return AFQ_ENOMEM;
@@ -216,11 +222,18 @@ int afq_force_bias(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const doub
                    double* cmf) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
-  double* vb = (double*)afq::scratch(ctx, sizeof(double) * 2 * (size_t)W * G);
+  afq::FbSplits sp = afq::fb_splits(M, Na, Nb, Gha, Ghb, ra, rb, 2);
+  const size_t nsp = std::max(1, sp.n);
+  double* vb = (double*)afq::scratch(ctx, sizeof(double) * 2 * (size_t)W * G * nsp);
   if (!vb && W * (size_t)G) return AFQ_ENOMEM;
-  int rc = afq::force_bias_gemm_fp64(ctx->stream, W, M, Na, Nb, G, ra, rb, Gha, Ghb, vb);
-  if (rc) return rc;
-  return afq::shift_launch(ctx->stream, W, G, vb, mf, xi, dt, fb, xbar, xs, cfb, cmf);
+  int rc;
+  if (sp.n == 0) {
+    AFQ_CHECK_HIP(hipMemsetAsync(vb, 0, sizeof(double) * 2 * (size_t)W * G, ctx->stream));
+  } else if ((rc = afq::fb_mfma_f64(ctx->stream, W, G, sp, vb))) {
+    return rc;
+  }
+  return afq::shift_launch(ctx->stream, W, G, vb, mf, xi, dt, fb, xbar, xs, cfb, cmf,
+                           (int)nsp);
 }
 
 int afq_update_weights_phaseless(afq_ctx* ctx, int W, const double* w, const double* oo,

@@ -12,13 +12,78 @@
 //   8. phaseless weight update                             (527-571)
 // The system owns a workspace sized for the largest walker count seen; a
 // steady-state step allocates nothing.
+#include <vector>
+
 #include "afq_gemm.h"
 #include "afq_kernels.h"
+#include "afq_mfma.h"
 #include "afq_step.h"
+
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> ev[PH_NPHASE];  // pairs (start, stop)
+  int used[PH_NPHASE] = {};
+  double total_ms[PH_NPHASE] = {};
+  int64_t count[PH_NPHASE] = {};
+};
 
 namespace afq {
 
+void phase_begin(afq_system* s, int ph) {
+  Timing* t = s->timing;
+  if (!t || !t->on) return;
+  if ((int)t->ev[ph].size() < 2 * (t->used[ph] + 1)) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      t->ev[ph].push_back(e);
+    }
+  }
+  (void)hipEventRecord(t->ev[ph][2 * t->used[ph]], s->ctx->stream);
+}
+
+void phase_end(afq_system* s, int ph) {
+  Timing* t = s->timing;
+  if (!t || !t->on || (int)t->ev[ph].size() < 2 * (t->used[ph] + 1)) return;
+  (void)hipEventRecord(t->ev[ph][2 * t->used[ph] + 1], s->ctx->stream);
+  t->used[ph] += 1;
+}
+
+static int timing_collect(afq_system* s) {
+  Timing* t = s->timing;
+  if (!t) return AFQ_OK;
+  AFQ_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+  for (int ph = 0; ph < PH_NPHASE; ++ph) {
+    for (int i = 0; i < t->used[ph]; ++i) {
+      float ms = 0.f;
+      AFQ_CHECK_HIP(hipEventElapsedTime(&ms, t->ev[ph][2 * i], t->ev[ph][2 * i + 1]));
+      t->total_ms[ph] += ms;
+      t->count[ph] += 1;
+    }
+    t->used[ph] = 0;
+  }
+  return AFQ_OK;
+}
+
 namespace {
+
+__global__ void k_sym_check(long n, int M, const double* __restrict__ L, int* __restrict__ bad) {
+  const long MM = (long)M * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long g = i / MM, r = i % MM;
+    const int p = (int)(r / M), q = (int)(r % M);
+    if (q > p && L[g * MM + (long)p * M + q] != L[g * MM + (long)q * M + p]) atomicOr(bad, 1);
+  }
+}
+
+__global__ void k_pack_upper(long n, int P, long MM, const double* __restrict__ L,
+                             const int2* __restrict__ colmap, double* __restrict__ Lp) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long g = i / P;
+    const int c = (int)(i % P);
+    Lp[i] = L[g * MM + colmap[c].x];
+  }
+}
 
 struct Layout {
   size_t off[16];
@@ -34,7 +99,7 @@ Layout layout(const afq_system* s, int W) {
   sz[GHA] = 16L * W * Na * M;
   sz[GHB] = 16L * W * Nb * M;
   sz[OVLP_OLD] = 16L * W;
-  sz[VBIAS] = 16L * W * G;
+  sz[VBIAS] = 16L * W * G * std::max(1, s->fb_nsplit);
   sz[XS] = 16L * W * G;
   sz[CFB] = 16L * W;
   sz[CMF] = 16L * W;
@@ -77,6 +142,74 @@ int ensure_work(afq_system* s, int W) {
 
 }  // namespace
 
+// FP64 operands of the step: the upper triangle of every L_g (when every L_g
+// is exactly symmetric, so that VHS_qp = VHS_pq bit for bit) and the number
+// of K splits of the force bias.
+int prepare_fp64(afq_system* s) {
+  hipStream_t st = s->ctx->stream;
+  const int M = s->M, G = s->G;
+  s->fb_nsplit = 4;
+  // identity trials make S = phi^T Psi* a transposed gather
+  const int Ns[2] = {s->Na, s->Nb};
+  const double* tr[2] = {s->trial_up, s->trial_dn};
+  s->eye_mask = 0;
+  for (int sp = 0; sp < 2; ++sp) {
+    const size_t n = 2 * (size_t)M * Ns[sp];
+    if (n == 0) continue;
+    double* h = new double[n];
+    hipError_t e = hipMemcpyAsync(h, tr[sp], n * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      delete[] h;
+      return hip_fail(e, "trial copy");
+    }
+    bool eye = true;
+    for (int p = 0; p < M && eye; ++p)
+      for (int i = 0; i < Ns[sp]; ++i) {
+        const double* z = h + 2 * ((size_t)p * Ns[sp] + i);
+        if (z[0] != (p == i ? 1.0 : 0.0) || z[1] != 0.0) { eye = false; break; }
+      }
+    delete[] h;
+    if (eye) s->eye_mask |= (sp == 0 ? 2 : 4);
+  }
+  if (G == 0) return AFQ_OK;
+  int* bad = nullptr;
+  AFQ_CHECK_HIP(hipMalloc(&bad, sizeof(int)));
+  AFQ_CHECK_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
+  const long n = (long)G * M * M;
+  hipLaunchKernelGGL(k_sym_check, dim3(4096), dim3(256), 0, st, n, M, s->chol, bad);
+  int hbad = 1;
+  AFQ_CHECK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  AFQ_CHECK_HIP(hipStreamSynchronize(st));
+  AFQ_CHECK_HIP(hipFree(bad));
+  if (hbad) return AFQ_OK;  // not exactly symmetric: full M*M columns
+  const int P = M * (M + 1) / 2;
+  int2* hmap = new int2[P];
+  int c = 0;
+  for (int p = 0; p < M; ++p)
+    for (int q = p; q < M; ++q) hmap[c++] = make_int2(p * M + q, q == p ? -1 : q * M + p);
+  int2* dmap = nullptr;
+  double* Lp = nullptr;
+  hipError_t e1 = hipMalloc(&dmap, sizeof(int2) * P);
+  hipError_t e2 = hipMalloc(&Lp, sizeof(double) * (size_t)G * P);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    delete[] hmap;
+    if (dmap) (void)hipFree(dmap);
+    if (Lp) (void)hipFree(Lp);
+    return fail(AFQ_ENOMEM, "packed Cholesky operand (%zu bytes)", sizeof(double) * (size_t)G * P);
+  }
+  AFQ_CHECK_HIP(hipMemcpyAsync(dmap, hmap, sizeof(int2) * P, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pack_upper, dim3(8192), dim3(256), 0, st, (long)G * P, P, (long)M * M,
+                     s->chol, dmap, Lp);
+  AFQ_CHECK_LAUNCH();
+  AFQ_CHECK_HIP(hipStreamSynchronize(st));
+  delete[] hmap;
+  s->colmap = dmap;
+  s->chol_packed = Lp;
+  s->packed_P = P;
+  return AFQ_OK;
+}
+
 double* work_slot(afq_system* s, int W, int slot) {
   Layout L = layout(s, s->cap_walkers);
   (void)W;
@@ -107,37 +240,56 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   const double* xi = xi_in;
   if (!xi) {
     double* xg = work_slot(sys, W, XI);
+    phase_begin(sys, PH_RNG);
     rc = aux_fields_launch(s, seed, step, W, G, xg);
+    phase_end(sys, PH_RNG);
     if (rc) return rc;
     xi = xg;
   }
   // 1
-  rc = greens_launch(s, 1, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn, gha, ghb, oo);
+  phase_begin(sys, PH_GREENS);
+  rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
+                     gha, ghb, oo);
+  phase_end(sys, PH_GREENS);
   if (rc) return rc;
-  // 2
-  if ((rc = one_body_fp64(s, W, M, Na, sys->expH1, phia, a1))) return rc;
-  if ((rc = one_body_fp64(s, W, M, Nb, sys->expH1, phib, b1))) return rc;
-  // 3
-  if ((rc = force_bias_gemm_fp64(s, W, M, Na, Nb, G, sys->rchol_a, sys->rchol_b, gha, ghb, vb)))
-    return rc;
+  (void)a1; (void)a2; (void)b1; (void)b2; (void)t0; (void)t1;
+  // 3 (the first one-body half step (2) is fused into the propagator below:
+  //    it does not feed the force bias, which uses the pre-step Ghalf)
+  phase_begin(sys, PH_FORCEBIAS);
+  FbSplits sp = fb_splits(M, Na, Nb, gha, ghb, sys->rchol_a, sys->rchol_b,
+                          std::max(1, sys->fb_nsplit / 2));
+  if ((rc = fb_mfma_f64(s, W, G, sp, vb))) return rc;
+  phase_end(sys, PH_FORCEBIAS);
+  phase_begin(sys, PH_SHIFT);
   if ((rc = shift_launch(s, W, G, vb, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr, xs, cfb,
-                         cmf)))
+                         cmf, sp.n)))
     return rc;
+  phase_end(sys, PH_SHIFT);
   // 4
-  if ((rc = build_vhs_fp64(s, W, M, G, xs, sys->chol, sys->dt, vhs))) return rc;
-  // 5
-  if ((rc = apply_exp_vhs_fp64(s, W, M, Na, vhs, a1, sys->nmax, a2, t0, t1))) return rc;
-  if ((rc = apply_exp_vhs_fp64(s, W, M, Nb, vhs, b1, sys->nmax, b2, t0, t1))) return rc;
-  // 6
-  if ((rc = one_body_fp64(s, W, M, Na, sys->expH1, a2, phia))) return rc;
-  if ((rc = one_body_fp64(s, W, M, Nb, sys->expH1, b2, phib))) return rc;
+  phase_begin(sys, PH_VHS);
+  if (sys->chol_packed)
+    rc = vhs_mfma_f64(s, W, M, G, xs, sys->chol_packed, sys->packed_P, sys->colmap, sys->dt, vhs);
+  else
+    rc = vhs_mfma_f64(s, W, M, G, xs, sys->chol, M * M, nullptr, sys->dt, vhs);
+  phase_end(sys, PH_VHS);
+  if (rc) return rc;
+  // 2, 5, 6: expH1 . exp(VHS) . expH1, in place
+  phase_begin(sys, PH_TAYLOR);
+  rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, vhs, phia, phib, phia, phib, sys->nmax);
+  phase_end(sys, PH_TAYLOR);
+  if (rc) return rc;
   // 7
-  if ((rc = greens_launch(s, 0, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn, nullptr,
+  phase_begin(sys, PH_OVERLAP);
+  if ((rc = greens_launch(s, sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn, nullptr,
                           nullptr, ovlp_new)))
     return rc;
+  phase_end(sys, PH_OVERLAP);
   // 8
-  return weights_launch(s, W, weights, oo, ovlp_new, cfb, cmf, e_hybrid, eshift, sys->dt, weights,
-                        e_hybrid);
+  phase_begin(sys, PH_WEIGHTS);
+  rc = weights_launch(s, W, weights, oo, ovlp_new, cfb, cmf, e_hybrid, eshift, sys->dt, weights,
+                      e_hybrid);
+  phase_end(sys, PH_WEIGHTS);
+  return rc;
 }
 
 int system_energy_fp64(afq_system* sys, int W, const double* phia, const double* phib,
@@ -155,7 +307,8 @@ int system_energy_fp64(afq_system* sys, int W, const double* phia, const double*
   double* ex = work_slot(sys, W, CMF);
   double* part = work_slot(sys, W, VHS);
   if ((long)M * (Na + Nb) > (long)W * G) return fail(AFQ_EUNSUPPORTED, "energy workspace too small");
-  rc = greens_launch(s, 1, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn, gha, ghb, oo);
+  rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
+                     gha, ghb, oo);
   if (rc) return rc;
   // hT_s = h1e @ conj(trial_s)  (M x N_s)
   double* hTa = hT;
@@ -206,19 +359,53 @@ int afq_system_create(afq_ctx* ctx, afq_system** out, int M, int Na, int Nb, int
   s->chol = chol; s->rchol_a = ra; s->rchol_b = rb; s->expH1 = expH1; s->mf_shift = mf;
   s->trial_up = tu; s->trial_dn = td;
   s->dt = dt; s->fbbound = fbbound; s->nmax = nmax; s->precision = precision;
-  int rc = afq::system_prepare_lowp(s);
+  int rc = afq::prepare_fp64(s);
+  if (!rc) rc = afq::system_prepare_lowp(s);
   if (rc) {
-    delete s;
+    afq_system_destroy(s);
     return rc;
   }
   *out = s;
   return AFQ_OK;
 }
 
+int afq_system_timing_enable(afq_system* s, int on) {
+  AFQ_REQUIRE(s, "system is NULL");
+  if (!s->timing) s->timing = new Timing();
+  if (!on) {
+    int rc = afq::timing_collect(s);
+    if (rc) return rc;
+  }
+  s->timing->on = (on != 0);
+  return AFQ_OK;
+}
+
+int afq_system_timing_read(afq_system* s, double* total_ms, int64_t* launches, int nphase,
+                           int reset) {
+  AFQ_REQUIRE(s && total_ms && launches, "NULL argument");
+  for (int i = 0; i < nphase; ++i) { total_ms[i] = 0.0; launches[i] = 0; }
+  if (!s->timing) return AFQ_OK;
+  int rc = afq::timing_collect(s);
+  if (rc) return rc;
+  for (int i = 0; i < nphase && i < PH_NPHASE; ++i) {
+    total_ms[i] = s->timing->total_ms[i];
+    launches[i] = s->timing->count[i];
+    if (reset) { s->timing->total_ms[i] = 0.0; s->timing->count[i] = 0; }
+  }
+  return AFQ_OK;
+}
+
 int afq_system_destroy(afq_system* s) {
   if (!s) return AFQ_OK;
   (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->timing) {
+    for (int ph = 0; ph < PH_NPHASE; ++ph)
+      for (hipEvent_t e : s->timing->ev[ph]) (void)hipEventDestroy(e);
+    delete s->timing;
+  }
   if (s->work) (void)hipFree(s->work);
+  if (s->chol_packed) (void)hipFree(s->chol_packed);
+  if (s->colmap) (void)hipFree(s->colmap);
   afq::system_release_lowp(s);
   delete s;
   return AFQ_OK;

@@ -93,23 +93,45 @@ def one_body_chol_correction(chol: np.ndarray) -> np.ndarray:
     return -0.5 * (A.T @ A)
 
 
+REF_GM = 11 * 50  # G * M of the LiH-like workload
+
+
+def size_normalised_sigma(nbasis: int, naux: int, chol_sigma: float = 0.3) -> float:
+    """sigma_0 scaled by sqrt(550 / (G M)).
+
+    With the literal sigma_0 = 0.3, ||VHS|| grows like sqrt(G M) and the
+    one-body generator like G M: at M=300, G=1800 the determinant energy is
+    -6.0e4 Ha, ||expH1|| = 2.4e7 and the walker overlap sign*exp(logdet)
+    overflows FP64 in the first step of the reference algorithm.  Scaling the
+    variance by 550/(G M) keeps ||VHS|| and the per-orbital two-body energy at
+    their values for the LiH-like workload (where the factor is exactly 1).
+    """
+    return chol_sigma * float(np.sqrt(REF_GM / (int(nbasis) * int(naux))))
+
+
 def make_system(nbasis: int, nup: int, ndown: int, naux: int, seed: int = 42,
                 decay: float | None = None, enuc: float = 1.0,
                 rotate_device: str | None = None,
-                chol_sigma: float = 0.3) -> SyntheticSystem:
+                chol_sigma: float = 0.3, normalise: bool = False) -> SyntheticSystem:
     """Build a synthetic system, rotated into the basis of the trial.
 
     ``rotate_device`` (e.g. ``"cuda"``) performs the O(G M^3) basis rotation
     and the v0 contraction with torch on that device; the default runs them
-    with NumPy/BLAS on the host.
+    with NumPy/BLAS on the host.  ``normalise`` applies
+    :func:`size_normalised_sigma` to ``chol_sigma``.
     """
+    if normalise:
+        chol_sigma = size_normalised_sigma(nbasis, naux, chol_sigma)
     h1, L, lam = raw_integrals(nbasis, naux, seed, decay, chol_sigma=chol_sigma)
     _, C = np.linalg.eigh(h1)
     C = _fix_signs(C)
     h1r = C.T @ h1 @ C
     h1r = 0.5 * (h1r + h1r.T)
+    # The rotated L_g are symmetrised exactly ((L + L^T)/2 is bitwise
+    # symmetric), which a rounded C^T L C is not.
     if rotate_device is None:
         Lr = np.matmul(np.matmul(C.T[None], L), C[None])
+        Lr = 0.5 * (Lr + Lr.transpose(0, 2, 1))
         v0 = one_body_chol_correction(Lr)
     else:
         import torch
@@ -117,6 +139,7 @@ def make_system(nbasis: int, nup: int, ndown: int, naux: int, seed: int = 42,
         Ct = torch.from_numpy(C).to(dev)
         Lt = torch.from_numpy(L).to(dev)
         Lt = torch.matmul(torch.matmul(Ct.T.unsqueeze(0), Lt), Ct.unsqueeze(0))
+        Lt = 0.5 * (Lt + Lt.transpose(1, 2))
         A = Lt.transpose(1, 2).reshape(-1, nbasis)
         v0 = (-0.5 * (A.T @ A)).cpu().numpy()
         Lr = Lt.cpu().numpy()

@@ -187,25 +187,38 @@ def e_const(enuc, mf_shift):
 # Propagation (afqmc_gto.py:451-725)
 # ---------------------------------------------------------------------------
 
+def _xr_matmul(x, Lflat):
+    """Complex (W,K) times real (K,P) as two real BLAS products."""
+    if np.iscomplexobj(Lflat):
+        return x @ Lflat
+    return (x.real @ Lflat) + 1j * (x.imag @ Lflat)
+
+
 def apply_exp_vhs(VHS, phi, nmax=6):
-    """afqmc_gto.py:451-469: sum_{n<=nmax} VHS^n phi / n! (recursive terms)."""
+    """afqmc_gto.py:451-469: sum_{n<=nmax} VHS^n phi / n! (recursive terms).
+
+    einsum('wpq,wqn->wpn') is the batched product VHS @ temp."""
     result = phi.copy()
     temp = phi.copy()
     for n in range(1, nmax + 1):
-        temp = np.einsum('wpq,wqn->wpn', VHS, temp) / n
+        temp = np.matmul(VHS, temp) / n
         result = result + temp
     return result
 
 
 def build_vhs(xshifted, chol, dt, chol_chunk_g=None):
-    """afqmc_gto.py:472-482: VHS = i sqrt(dt) sum_g xshifted[w,g] L_g."""
+    """afqmc_gto.py:472-482: VHS = i sqrt(dt) sum_g xshifted[w,g] L_g.
+
+    einsum('wg,gpq->wpq') is xshifted @ chol.reshape(G, M*M)."""
     naux, nb = chol.shape[0], chol.shape[1]
     step = naux if chol_chunk_g is None else chol_chunk_g
-    VHS = np.zeros((xshifted.shape[0], nb, nb), dtype=np.complex128)
+    W = xshifted.shape[0]
+    VHS = np.zeros((W, nb, nb), dtype=np.complex128)
     sqdt = np.sqrt(dt)
     for g0 in range(0, naux, step):
         g1 = min(g0 + step, naux)
-        VHS = VHS + 1j * sqdt * np.einsum('wg,gpq->wpq', xshifted[:, g0:g1], chol[g0:g1])
+        part = _xr_matmul(xshifted[:, g0:g1], chol[g0:g1].reshape(g1 - g0, nb * nb))
+        VHS = VHS + 1j * sqdt * part.reshape(W, nb, nb)
     return VHS
 
 
@@ -227,8 +240,14 @@ def force_bias_shift(Ghalfa, Ghalfb, rchol_a, rchol_b, mf_shift, xi, dt, fbbound
     """afqmc_gto.py:682-706 -> (xbar, xshifted, cfb, cmf, vbias)."""
     if fbbound is None:
         fbbound = FBBOUND_DEFAULT
-    vbias = (np.einsum('giq,wiq->gw', rchol_a, Ghalfa)
-             + np.einsum('giq,wiq->gw', rchol_b, Ghalfb))
+    # einsum('giq,wiq->gw', rchol, Ghalf) = (Ghalf.reshape(W, N M) @ rchol.reshape(G, N M)^T)^T
+    W, G = Ghalfa.shape[0], rchol_a.shape[0]
+    vbias = np.zeros((W, G), dtype=np.complex128)
+    for rc, gh in ((rchol_a, Ghalfa), (rchol_b, Ghalfb)):
+        K = rc.shape[1] * rc.shape[2]
+        if K:
+            vbias = vbias + _xr_matmul(gh.reshape(W, K), rc.reshape(G, K).T)
+    vbias = vbias.T
     xbar = (-np.sqrt(dt) * (1j * vbias - mf_shift[:, None])).T
     xabs = np.abs(xbar)
     with np.errstate(invalid='ignore', divide='ignore'):
@@ -252,15 +271,16 @@ def propagate_walkers(phia, phib, weights, overlap, e_hybrid, propagator, chol,
     expH1 = propagator['expH1']
     mf_shift = propagator['mf_shift']
     Gha, Ghb, ovlp = greens_function_force_bias(phia, phib, trial_up, trial_dn)
-    phia = np.einsum('pq,wqn->wpn', expH1, phia)
-    phib = np.einsum('pq,wqn->wpn', expH1, phib)
+    # einsum('pq,wqn->wpn', expH1, phi) = expH1 @ phi (batched)
+    phia = np.matmul(expH1, phia)
+    phib = np.matmul(expH1, phib)
     _, xshifted, cfb, cmf, _ = force_bias_shift(Gha, Ghb, rchol_a, rchol_b, mf_shift,
                                                 xi, dt, fbbound)
     VHS = build_vhs(xshifted, chol, dt)
     phia = apply_exp_vhs(VHS, phia, exp_nmax)
     phib = apply_exp_vhs(VHS, phib, exp_nmax)
-    phia = np.einsum('pq,wqn->wpn', expH1, phia)
-    phib = np.einsum('pq,wqn->wpn', expH1, phib)
+    phia = np.matmul(expH1, phia)
+    phib = np.matmul(expH1, phib)
     ovlp_new = greens_function_overlap(phia, phib, trial_up, trial_dn)
     w_new, e_new = update_weights_phaseless(weights, ovlp, ovlp_new, cfb, cmf,
                                             e_hybrid, eshift, dt)

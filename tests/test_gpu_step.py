@@ -119,11 +119,14 @@ def test_driver_block_energy_matches_oracle(afq):
 
 
 def test_full_size_step_properties(afq):
-    """M=300, N=40+40, G=1800, W=1024: size-independent invariants of one step."""
+    """M=300, N=40+40, G=1800, W=1024: size-independent invariants of one step.
+
+    The size-normalised generator is used: with the literal one the overlap
+    sign*exp(logdet) overflows FP64 in the first step (reference semantics)."""
     import torch
     from omegaqmc_amd.synth import make_system
     from omegaqmc_amd.driver import AFQMCDriver
-    s = make_system(300, 40, 40, 1800, seed=42, rotate_device="cuda")
+    s = make_system(300, 40, 40, 1800, seed=42, rotate_device="cuda", normalise=True)
     integ = {'h1e': _t(s.h1e), 'h1e_mod': _t(s.h1e_mod), 'chol': _t(s.chol), 'enuc': s.enuc,
              'nbasis': 300, 'nup': 40, 'ndown': 40}
     gd = AFQMCDriver(integ, verbose=False)
@@ -132,11 +135,14 @@ def test_full_size_step_properties(afq):
     e_tot, _, _ = gd.system.local_energy(st['phia'], st['phib'], gd.h1e, gd.enuc)
     ehf = O.determinant_energy_from_integrals(s.h1e, s.chol, s.enuc, 40, 40)
     np.testing.assert_allclose(_n(e_tot).real, ehf, rtol=1e-11)
-    for step in range(1, 6):
-        gd.step(st, step, 42, 0.0, 5, 10)
+    assert gd.system.handle is not None
+    for step in range(1, 11):
+        gd.step(st, step, 42, gd.ehf - gd.e_const, 5, 10)
     torch.cuda.synchronize()
     w = _n(st['weights'])
     assert np.all(np.isfinite(w)) and np.all(w >= 0)
+    # after the comb at step 10 every weight is 1 and e_hybrid is finite
+    assert np.all(w == 1.0) and np.all(np.isfinite(_n(st['e_hybrid'])))
     # step 5 re-orthonormalised before propagating: check Q of a fresh QR
     qa, qb, ld, _ = afq.orthogonalize_walkers(st['phia'][:16], st['phib'][:16])
     q = _n(qa)
