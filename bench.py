@@ -124,6 +124,41 @@ def run_gpu(sys_np, W, steps, warmup, precision, seed=42, timing=True):
             "energy_ms": te0.elapsed_time(te1), "e_block": e_block, "last_step": step}
 
 
+def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10):
+    """Block-energy error of ``precision`` against FP64 on the same trajectory.
+
+    Both runs draw the same auxiliary fields (same Philox stream); at each
+    comb step the reduced-precision run replays the FP64 run's survivor
+    indices, so the two populations stay walker-for-walker comparable and the
+    difference is the propagation error alone.  The energy estimator itself is
+    FP64 in both runs.  Returns the largest |dE| (Ha) over the checkpoints.
+    """
+    import torch
+    from omegaqmc_amd.driver import AFQMCDriver
+    dev = torch.device("cuda", 0)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    integ = {'h1e': t(sys_np.h1e), 'h1e_mod': t(sys_np.h1e_mod), 'chol': t(sys_np.chol),
+             'enuc': sys_np.enuc, 'nbasis': sys_np.nbasis, 'nup': sys_np.nup,
+             'ndown': sys_np.ndown}
+    ref = AFQMCDriver(integ, verbose=False, precision="fp64")
+    low = AFQMCDriver(integ, verbose=False, precision=precision)
+    sr, sl = ref.init_walkers(W), low.init_walkers(W)
+    eshift = ref.ehf - ref.e_const
+    worst, trace = 0.0, []
+    for step in range(1, nsteps + 1):
+        ref.step(sr, step, seed, eshift, 5, 10)
+        low.step(sl, step, seed, eshift, 5, 10, comb_idx=sr['idx'] if step % 10 == 0 else None)
+        if step % every == 0:
+            ref.block_energy(sr)
+            low.block_energy(sl)
+            d = abs(float(sl['stats'][0].item()) - float(sr['stats'][0].item()))
+            trace.append(d * 1e3)
+            worst = max(worst, d)
+    del ref, low, sr, sl
+    torch.cuda.empty_cache()
+    return worst, trace
+
+
 def cpu_baseline(sys_np, sample_walkers=8, nsteps=1):
     """Oracle (NumPy FP64 restatement) on this host for a bounded sample."""
     from oracle import afqmc as O
@@ -156,6 +191,8 @@ def main():
     ap.add_argument("--precision", default="auto", help="fp64 | fp32 | bf16x3 | auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-walkers", type=int, default=8)
+    ap.add_argument("--acc-steps", type=int, default=30,
+                    help="steps of the same-trajectory energy comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -174,6 +211,8 @@ def main():
 
     print("[bench] FP64 GPU step", file=sys.stderr, flush=True)
     r64 = run_gpu(sys_np, W, args.steps, args.warmup, "fp64")
+    del r64["drv"], r64["state"]
+    torch.cuda.empty_cache()
     best, best_prec = r64, "fp64"
     extra = {}
     if args.precision in ("auto", "fp32", "bf16x3"):
@@ -185,9 +224,16 @@ def main():
             except _lib.AFQError as e:
                 extra[f"{prec}_error"] = str(e)[:200]
                 continue
-            dE = abs(r["e_block"] - r64["e_block"])
-            extra[f"{prec}_walker_steps_per_s"] = W / (r["ms_per_step"] * 1e-3)
-            extra[f"{prec}_energy_err_mHa"] = dE * 1e3
+            extra[f"{prec}_walker_steps_per_s"] = round(W / (r["ms_per_step"] * 1e-3), 2)
+            extra[f"{prec}_phases_ms_per_step"] = {k: round(v["ms_total"] / args.steps, 4)
+                                                   for k, v in r["phases"].items()}
+            del r["drv"], r["state"]
+            torch.cuda.empty_cache()
+            print(f"[bench] {prec} vs FP64 energy, same trajectory ({args.acc_steps} steps)",
+                  file=sys.stderr, flush=True)
+            dE, trace = lockstep_energy_error(sys_np, W, args.acc_steps, prec)
+            extra[f"{prec}_energy_err_mHa"] = round(dE * 1e3, 5)
+            extra[f"{prec}_energy_err_trace_mHa"] = [round(x, 5) for x in trace]
             if dE <= 0.5e-3 and r["ms_per_step"] < best["ms_per_step"]:
                 best, best_prec = r, prec
     ws = W / (best["ms_per_step"] * 1e-3)

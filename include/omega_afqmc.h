@@ -169,6 +169,16 @@ int afq_population_control_comb(afq_ctx* ctx, int nwalkers, int nbasis, int nup,
                                 double* weights_new, double* phia_new, double* phib_new,
                                 double* e_hybrid_new, int32_t* idx);
 
+/* afqmc_gto.py:229-238 (second half of population_control_comb): gather the
+ * walkers and e_hybrid by given survivor indices idx (W,) int32 and reset the
+ * weights to 1.  Lets a caller replay the population-control decisions of
+ * another run (or of a global comb over several devices).  An index outside
+ * [0, nwalkers) is not read: that output row (and its e_hybrid) becomes NaN. */
+int afq_population_control_apply(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                 const int32_t* idx, const double* phia, const double* phib,
+                                 const double* e_hybrid, double* weights_new, double* phia_new,
+                                 double* phib_new, double* e_hybrid_new);
+
 /* afqmc_gto.py:1428-1432 weight clip w <- clip(w, 0, frac * sum|w|) in place. */
 int afq_clip_weights(afq_ctx* ctx, int nwalkers, double* weights, double frac);
 

@@ -1,0 +1,604 @@
+// Split-BF16 kernels of the reduced-precision step (v_mfma_f32_16x16x32_bf16).
+//
+// Every FP64 operand x is stored as two bf16 planes, hi = bf16(x) and
+// lo = bf16(x - hi) (round to nearest even), and a product is accumulated in
+// FP32 as ah*bh + ah*bl + al*bh; the dropped al*bl term is ~2^-16 of the
+// product.  The walkers, determinants, QR, weights and energy reductions
+// stay in FP64.
+//
+//   k_bf3_gemm<VHS>   VHS = i sqrt(dt) sum_g x_g L_g over the upper-triangle
+//                     16x16 blocks of the symmetric result; each workgroup
+//                     owns one block for 128 walkers and writes it and its
+//                     transpose in the block-major layout the propagator reads.
+//   k_bf3_gemm<FB>    split-K force bias, FP32 partials summed in FP64 later.
+//   k_prop_bf3        expH1 . sum_n VHS^n/n! . expH1 on a 48-column block of one
+//                     spin of one walker, the block kept in LDS as bf16 planes.
+//
+// MFMA layout (16x16x32 bf16, checked on the device): A[i][k] in lane
+// i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8); D[i][j] in lane
+// j + 16 (i / 4), register i % 4.  With GEMM rows ordered (walker, re/im) a
+// lane's four output rows are (w, re), (w, im), (w+1, re), (w+1, im).
+#include "afq_bf3.h"
+#include "afq_kernels.h"
+
+namespace afq {
+
+namespace {
+
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v mfma_bf(const uint4& a, const uint4& b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                 __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+
+// c += ah*bh + ah*bl + al*bh
+__device__ __forceinline__ f4v mfma3(const uint4& ah, const uint4& al, const uint4& bh,
+                                     const uint4& bl, f4v c) {
+  c = mfma_bf(al, bh, c);
+  c = mfma_bf(ah, bl, c);
+  return mfma_bf(ah, bh, c);
+}
+
+__device__ __forceinline__ uint4 neg8(uint4 a) {
+  return make_uint4(a.x ^ 0x80008000u, a.y ^ 0x80008000u, a.z ^ 0x80008000u, a.w ^ 0x80008000u);
+}
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int x = b & 7, slot = b >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// ----------------------------------------------------------------------------
+// system preparation
+// ----------------------------------------------------------------------------
+__global__ void k_prep_L(long n, int M, int G, int Gp, long NC, const int2* __restrict__ pq,
+                         const double* __restrict__ L, bf16_t* __restrict__ Lb) {
+  const long MM = (long)M * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long c = e / Gp;
+    const int g = (int)(e % Gp);
+    const int b = (int)(c >> 8), i = (int)((c >> 4) & 15), j = (int)(c & 15);
+    const int2 PQ = pq[b];
+    const int p = 16 * PQ.x + i, q = 16 * PQ.y + j;
+    double v = 0.0;
+    if (g < G && p < M && q < M) v = L[(long)g * MM + (long)p * M + q];
+    bf16_t h, l;
+    split2(v, h, l);
+    Lb[e] = h;
+    Lb[NC * Gp + e] = l;
+  }
+}
+
+// rows x cols -> [plane][rows_p][cols_p] with zero padding
+__global__ void k_prep_mat(long n, int rows, int cols, int cols_p, long plane,
+                           const double* __restrict__ A, bf16_t* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long r = e / cols_p;
+    const int c = (int)(e % cols_p);
+    const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
+    bf16_t h, l;
+    split2(v, h, l);
+    out[e] = h;
+    out[plane + e] = l;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// the GEMM core: 256 x 256 tile, K step 32, 8 waves of 64 x 128
+// ----------------------------------------------------------------------------
+constexpr int GB = 256, GK = 32, GT = 512, LROW = GK + 8;
+
+struct Bf3Split {
+  const bf16_t* a[2];  // A planes (rows x lda)
+  const bf16_t* b[2];  // B planes (cols x ldb)
+  long lda, ldb;
+  int kbeg, kend;
+};
+
+struct Bf3Gemm {
+  Bf3Split sp[8];
+  int nsplit;
+  int rows, cols, ntm, ntn;
+  // VHS epilogue
+  const int2* blk_pq;
+  int RT, W;
+  long vstride;
+  bf16_t* Vb;
+  float sqdt;
+  // FB epilogue
+  float2* part;
+  int G;
+};
+
+template <int EPI>
+__global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
+  extern __shared__ __align__(16) bf16_t lds[];
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+  const int li = l & 15, lq = l >> 4;
+  const int v = xcd_remap(blockIdx.x, a.ntm * a.ntn);
+  const int tm = v % a.ntm, tn = v / a.ntm;
+  const int row0 = tm * GB, col0 = tn * GB;
+  const Bf3Split& S = a.sp[blockIdx.y];
+
+  auto Arow = [&](int p, int r) { return lds + (p * GB + r) * LROW; };
+  auto Brow = [&](int p, int c) { return lds + ((2 + p) * GB + c) * LROW; };
+
+  uint4 ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = t + GT * i, p = c >> 10, r = (c & 1023) >> 2, kq = c & 3;
+      const int gr = row0 + r, gc = col0 + r;
+      ra[i] = gr < a.rows ? *(const uint4*)(S.a[p] + gr * S.lda + k0 + 8 * kq) : make_uint4(0, 0, 0, 0);
+      rb[i] = gc < a.cols ? *(const uint4*)(S.b[p] + gc * S.ldb + k0 + 8 * kq) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = t + GT * i, p = c >> 10, r = (c & 1023) >> 2, kq = c & 3;
+      *(uint4*)(Arow(p, r) + 8 * kq) = ra[i];
+      *(uint4*)(Brow(p, r) + 8 * kq) = rb[i];
+    }
+  };
+
+  f4v acc[4][8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int w = 0; w < 8; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (S.kend - S.kbeg) / GK;
+  if (nk > 0) {
+    load(S.kbeg);
+    store();
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load(S.kbeg + (kt + 1) * GK);
+    uint4 ah[4], al[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = 64 * wr + 16 * u + li;
+      ah[u] = *(const uint4*)(Arow(0, r) + 8 * lq);
+      al[u] = *(const uint4*)(Arow(1, r) + 8 * lq);
+    }
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      const int c = 128 * wc + 16 * w + li;
+      const uint4 bh = *(const uint4*)(Brow(0, c) + 8 * lq);
+      const uint4 bl = *(const uint4*)(Brow(1, c) + 8 * lq);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  if (EPI == 0) {
+    // one 16x16 block (P,Q) of VHS per column tile; this wave holds block
+    // rows i = 8 wc + w (w < 8), column j = li
+    const int2 PQ = a.blk_pq[tn];
+    const long offd = ((long)PQ.x * a.RT + PQ.y) * 1024;
+    const long offt = ((long)PQ.y * a.RT + PQ.x) * 1024;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int w = (row0 + 64 * wr + 16 * u + 4 * lq) / 2 + h;
+        if (w >= a.W) continue;
+        bf16_t* base = a.Vb + (long)w * a.vstride;
+        bf16_t pl[4][8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          const double re = -(double)a.sqdt * (double)acc[u][x][2 * h + 1];
+          const double im = (double)a.sqdt * (double)acc[u][x][2 * h];
+          split2(re, pl[0][x], pl[1][x]);
+          split2(im, pl[2][x], pl[3][x]);
+          const int i = 8 * wc + x;
+#pragma unroll
+          for (int p = 0; p < 4; ++p) base[offd + p * 256 + i * 16 + li] = pl[p][x];
+        }
+        if (PQ.x != PQ.y) {
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            uint4 pk;
+            pk.x = pl[p][0] | ((unsigned)pl[p][1] << 16);
+            pk.y = pl[p][2] | ((unsigned)pl[p][3] << 16);
+            pk.z = pl[p][4] | ((unsigned)pl[p][5] << 16);
+            pk.w = pl[p][6] | ((unsigned)pl[p][7] << 16);
+            *(uint4*)(base + offt + p * 256 + li * 16 + 8 * wc) = pk;
+          }
+        }
+      }
+    }
+  } else {
+    const long soff = (long)blockIdx.y * a.W * a.G;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int w = (row0 + 64 * wr + 16 * u + 4 * lq) / 2 + h;
+        if (w >= a.W) continue;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          const int g = col0 + 128 * wc + 16 * x + li;
+          if (g < a.G) a.part[soff + (long)w * a.G + g] = make_float2(acc[u][x][2 * h], acc[u][x][2 * h + 1]);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// fused propagator on a 48-column block of one spin
+// ----------------------------------------------------------------------------
+constexpr int PW = 4, PT = 64 * PW, PC = 48;
+
+template <int MT>
+__global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, int Na, int Nb,
+                                                 int ncba, int ncbb, const bf16_t* __restrict__ E,
+                                                 const bf16_t* __restrict__ Vb, double* phia,
+                                                 double* phib, int nmax) {
+  extern __shared__ __align__(16) bf16_t Ts[];  // [4][PC][Kp + 8]
+  const int KR = Kp + 8;
+  const int nb = W * (ncba + ncbb);
+  const int vb = xcd_remap(blockIdx.x, nb);
+  const int per = ncba + ncbb;
+  const int w = vb / per, cbx = vb % per;
+  const bool spin_b = cbx >= ncba;
+  const int N = spin_b ? Nb : Na;
+  const int c0 = (spin_b ? cbx - ncba : cbx) * PC;
+  double* phi = spin_b ? phib : phia;
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int li = l & 15, lq = l >> 4;
+  const long MM = (long)M * M;
+  (void)MM;
+  const bf16_t* Vw = Vb + (long)w * RT * RT * 1024;
+  const long Eplane = (long)RT * 16 * Kp;
+
+  auto T = [&](int p, int c) { return Ts + (p * PC + c) * KR; };
+
+  // stage phi (FP64) as four bf16 planes, zero outside M x N
+  for (int e = t; e < PC * Kp; e += PT) {
+    const int c = e % PC, r = e / PC;
+    double re = 0.0, im = 0.0;
+    if (r < M && c0 + c < N) {
+      const double* z = phi + 2 * (((long)w * M + r) * N + c0 + c);
+      re = z[0];
+      im = z[1];
+    }
+    bf16_t a, b, cc, d;
+    split2(re, a, b);
+    split2(im, cc, d);
+    T(0, c)[r] = a;
+    T(1, c)[r] = b;
+    T(2, c)[r] = cc;
+    T(3, c)[r] = d;
+  }
+  __syncthreads();
+
+  f4v are[MT][3], aim[MT][3], rre[MT][3], rim[MT][3];
+  auto owns = [&](int u) { return wv + PW * u < RT; };
+  auto zero = [&]() {
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) { are[u][x] = f4v{0, 0, 0, 0}; aim[u][x] = f4v{0, 0, 0, 0}; }
+  };
+  auto bfrag = [&](int x, int ks, uint4* b) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) b[p] = *(const uint4*)(T(p, 16 * x + li) + 32 * ks + 8 * lq);
+  };
+  // acc = E . T (E real, planes [2][16 RT][Kp])
+  auto onebody = [&]() {
+    zero();
+    for (int ks = 0; ks < Kp / 32; ++ks) {
+      uint4 eh[MT], el[MT];
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        const long r = 16 * (wv + PW * u) + li;
+        if (owns(u)) {
+          eh[u] = *(const uint4*)(E + r * Kp + 32 * ks + 8 * lq);
+          el[u] = *(const uint4*)(E + Eplane + r * Kp + 32 * ks + 8 * lq);
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        uint4 b[4];
+        bfrag(x, ks, b);
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          if (!owns(u)) continue;
+          are[u][x] = mfma3(eh[u], el[u], b[0], b[1], are[u][x]);
+          aim[u][x] = mfma3(eh[u], el[u], b[2], b[3], aim[u][x]);
+        }
+      }
+    }
+  };
+  // acc = V . T (V complex, block-major planes)
+  // VHS fragments of one row tile u at k-step ks (zeros outside the matrix)
+  auto ldv = [&](uint4* v, int u, int ks) {
+    const int Q = 2 * ks + (lq >> 1), j0 = 8 * (lq & 1);
+    const int P = wv + PW * u;
+    if (P < RT && Q < RT) {
+      const bf16_t* blk = Vw + ((long)P * RT + Q) * 1024 + li * 16 + j0;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) v[p] = *(const uint4*)(blk + p * 256);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) v[p] = make_uint4(0, 0, 0, 0);
+    }
+  };
+  // Row tiles are walked one at a time with the next tile's fragments (also
+  // across k-steps) in flight while the current one feeds the MFMAs.
+  auto term = [&]() {
+    zero();
+    const int nks = Kp / 32;
+    uint4 vc[4], vn[4];
+    ldv(vc, 0, 0);
+    for (int ks = 0; ks < nks; ++ks) {
+      uint4 b[3][4];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) bfrag(x, ks, b[x]);
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        if (u + 1 < MT) ldv(vn, u + 1, ks);
+        else if (ks + 1 < nks) ldv(vn, 0, ks + 1);
+        if (owns(u)) {
+          const uint4 ni = neg8(vc[2]), nl = neg8(vc[3]);
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            are[u][x] = mfma3(vc[0], vc[1], b[x][0], b[x][1], are[u][x]);
+            are[u][x] = mfma3(ni, nl, b[x][2], b[x][3], are[u][x]);
+            aim[u][x] = mfma3(vc[0], vc[1], b[x][2], b[x][3], aim[u][x]);
+            aim[u][x] = mfma3(vc[2], vc[3], b[x][0], b[x][1], aim[u][x]);
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) vc[p] = vn[p];
+      }
+    }
+  };
+  // acc -> LDS planes (rows < M)
+  auto to_lds = [&](f4v (*xr)[3], f4v (*xi)[3]) {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      if (!owns(u)) continue;
+      const int r0 = 16 * (wv + PW * u) + 4 * lq;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        bf16_t q[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool ok = r0 + r < M;
+          split2(ok ? (double)xr[u][x][r] : 0.0, q[0][r], q[1][r]);
+          split2(ok ? (double)xi[u][x][r] : 0.0, q[2][r], q[3][r]);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          uint2 pk;
+          pk.x = q[p][0] | ((unsigned)q[p][1] << 16);
+          pk.y = q[p][2] | ((unsigned)q[p][3] << 16);
+          *(uint2*)(T(p, 16 * x + li) + r0) = pk;
+        }
+      }
+    }
+  };
+
+  onebody();
+  __syncthreads();
+  to_lds(are, aim);
+#pragma unroll
+  for (int u = 0; u < MT; ++u)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) { rre[u][x] = are[u][x]; rim[u][x] = aim[u][x]; }
+  __syncthreads();
+  for (int n = 1; n <= nmax; ++n) {
+    term();
+    const float rn = 1.0f / (float)n;
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        are[u][x] *= rn;
+        aim[u][x] *= rn;
+        rre[u][x] += are[u][x];
+        rim[u][x] += aim[u][x];
+      }
+    __syncthreads();
+    to_lds(are, aim);
+    __syncthreads();
+  }
+  to_lds(rre, rim);
+  __syncthreads();
+  onebody();
+#pragma unroll
+  for (int u = 0; u < MT; ++u) {
+    if (!owns(u)) continue;
+    const int r0 = 16 * (wv + PW * u) + 4 * lq;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const int c = c0 + 16 * x + li;
+      if (c >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r0 + r >= M) continue;
+        double* z = phi + 2 * (((long)w * M + r0 + r) * N + c);
+        z[0] = (double)are[u][x][r];
+        z[1] = (double)aim[u][x][r];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+size_t bf3_vhs_bytes(const Bf3System& b, int W) { return (size_t)W * b.RT * b.RT * 1024 * 2; }
+size_t bf3_x_bytes(const Bf3System& b, int W) { return (size_t)2 * 2 * W * b.Gp * 2; }
+size_t bf3_gh_bytes(const Bf3System& b, int W) {
+  return (size_t)2 * 2 * W * ((size_t)b.Ka + b.Kb) * 2;
+}
+
+int bf3_prepare(afq_system* s, Bf3System* b) {
+  hipStream_t st = s->ctx->stream;
+  const int M = s->M, G = s->G;
+  b->M = M; b->Na = s->Na; b->Nb = s->Nb; b->G = G;
+  b->RT = (M + 15) / 16;
+  b->Gp = (G + 31) / 32 * 32;
+  b->nblk = b->RT * (b->RT + 1) / 2;
+  b->Kp = (16 * b->RT + 31) / 32 * 32;
+  b->Ka = (s->Na * M + 31) / 32 * 32;
+  b->Kb = (s->Nb * M + 31) / 32 * 32;
+  b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = nullptr; b->Lhb = nullptr; b->E = nullptr;
+  if (b->RT > 32) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: M=%d > 512", M);
+  const long NC = (long)b->nblk * 256;
+  int2* hpq = new int2[b->nblk];
+  int k = 0;
+  for (int P = 0; P < b->RT; ++P)
+    for (int Q = P; Q < b->RT; ++Q) hpq[k++] = make_int2(P, Q);
+  hipError_t e = hipMalloc(&b->blk_pq, sizeof(int2) * b->nblk);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->blk_pq, hpq, sizeof(int2) * b->nblk, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  delete[] hpq;
+  if (e != hipSuccess) return hip_fail(e, "bf16x3 block table");
+  if (hipMalloc(&b->Lb, sizeof(bf16_t) * 2 * NC * b->Gp) != hipSuccess ||
+      hipMalloc(&b->E, sizeof(bf16_t) * 2 * (size_t)16 * b->RT * b->Kp) != hipSuccess ||
+      hipMalloc(&b->Lha, sizeof(bf16_t) * 2 * (size_t)G * b->Ka + 64) != hipSuccess ||
+      hipMalloc(&b->Lhb, sizeof(bf16_t) * 2 * (size_t)G * b->Kb + 64) != hipSuccess)
+    return fail(AFQ_ENOMEM, "bf16x3 operands");
+  if (G > 0) {
+    const long n = NC * b->Gp;
+    hipLaunchKernelGGL(k_prep_L, dim3(8192), dim3(256), 0, st, n, M, G, b->Gp, NC, b->blk_pq,
+                       s->chol, b->Lb);
+    AFQ_CHECK_LAUNCH();
+  }
+  {
+    const long rows = 16L * b->RT, n = rows * b->Kp;
+    hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, st, n, M, M, b->Kp, n, s->expH1, b->E);
+    AFQ_CHECK_LAUNCH();
+  }
+  if (G > 0 && s->Na > 0) {
+    const long n = (long)G * b->Ka;
+    hipLaunchKernelGGL(k_prep_mat, dim3(4096), dim3(256), 0, st, n, G, s->Na * M, b->Ka, n,
+                       s->rchol_a, b->Lha);
+    AFQ_CHECK_LAUNCH();
+  }
+  if (G > 0 && s->Nb > 0) {
+    const long n = (long)G * b->Kb;
+    hipLaunchKernelGGL(k_prep_mat, dim3(4096), dim3(256), 0, st, n, G, s->Nb * M, b->Kb, n,
+                       s->rchol_b, b->Lhb);
+    AFQ_CHECK_LAUNCH();
+  }
+  AFQ_CHECK_HIP(hipStreamSynchronize(st));
+  return AFQ_OK;
+}
+
+void bf3_release(Bf3System* b) {
+  if (b->Lb) (void)hipFree(b->Lb);
+  if (b->blk_pq) (void)hipFree(b->blk_pq);
+  if (b->Lha) (void)hipFree(b->Lha);
+  if (b->Lhb) (void)hipFree(b->Lhb);
+  if (b->E) (void)hipFree(b->E);
+  b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = b->Lhb = nullptr; b->E = nullptr;
+}
+
+static size_t gemm_lds() { return (size_t)4 * GB * LROW * sizeof(bf16_t); }
+
+int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt, bf16_t* Vb) {
+  if (W <= 0) return AFQ_OK;
+  Bf3Gemm a{};
+  a.nsplit = 1;
+  a.sp[0].a[0] = X;
+  a.sp[0].a[1] = X + (size_t)2 * W * b.Gp;
+  a.sp[0].lda = b.Gp;
+  a.sp[0].b[0] = b.Lb;
+  a.sp[0].b[1] = b.Lb + (size_t)b.nblk * 256 * b.Gp;
+  a.sp[0].ldb = b.Gp;
+  a.sp[0].kbeg = 0;
+  a.sp[0].kend = b.Gp;
+  a.rows = 2 * W;
+  a.cols = b.nblk * 256;
+  a.ntm = cdiv_int(2L * W, GB);
+  a.ntn = b.nblk;
+  a.blk_pq = b.blk_pq;
+  a.RT = b.RT;
+  a.W = W;
+  a.vstride = (long)b.RT * b.RT * 1024;
+  a.Vb = Vb;
+  a.sqdt = (float)sqrt(dt);
+  hipLaunchKernelGGL((k_bf3_gemm<0>), dim3(a.ntm * a.ntn, 1), dim3(GT), gemm_lds(), s, a);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, const bf16_t* Ghb,
+                   float* part, int* nsplit_out) {
+  Bf3Gemm a{};
+  int n = 0;
+  const int per_spin = 2;
+  const int Ks[2] = {b.Ka, b.Kb};
+  const int Nrow[2] = {b.Na, b.Nb};
+  const bf16_t* gh[2] = {Gha, Ghb};
+  const bf16_t* lh[2] = {b.Lha, b.Lhb};
+  for (int sp = 0; sp < 2; ++sp) {
+    if (Nrow[sp] == 0 || Ks[sp] == 0) continue;
+    const int K = Ks[sp];
+    const int chunk = ((K / 32 + per_spin - 1) / per_spin) * 32;
+    for (int k0 = 0; k0 < K; k0 += chunk) {
+      Bf3Split& S = a.sp[n++];
+      S.a[0] = gh[sp];
+      S.a[1] = gh[sp] + (size_t)2 * W * K;
+      S.lda = K;
+      S.b[0] = lh[sp];
+      S.b[1] = lh[sp] + (size_t)b.G * K;
+      S.ldb = K;
+      S.kbeg = k0;
+      S.kend = std::min(K, k0 + chunk);
+    }
+  }
+  *nsplit_out = n;
+  if (W <= 0 || n == 0) return AFQ_OK;
+  a.nsplit = n;
+  a.rows = 2 * W;
+  a.cols = b.G;
+  a.ntm = cdiv_int(2L * W, GB);
+  a.ntn = cdiv_int(b.G, GB);
+  a.W = W;
+  a.G = b.G;
+  a.part = (float2*)part;
+  hipLaunchKernelGGL((k_bf3_gemm<1>), dim3(a.ntm * a.ntn, n), dim3(GT), gemm_lds(), s, a);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, double* phia,
+                  double* phib, int nmax) {
+  const int ncba = (b.Na + PC - 1) / PC, ncbb = (b.Nb + PC - 1) / PC;
+  if (W <= 0 || ncba + ncbb == 0) return AFQ_OK;
+  const size_t sm = (size_t)4 * PC * (b.Kp + 8) * sizeof(bf16_t);
+  if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: LDS %zu", sm);
+  const dim3 grid(W * (ncba + ncbb)), blk(PT);
+#define AFQ_PROP_BF3(MT)                                                                      \
+  hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, ncba, \
+                     ncbb, b.E, Vb, phia, phib, nmax)
+  if (b.RT <= 2 * PW) AFQ_PROP_BF3(2);
+  else if (b.RT <= 3 * PW) AFQ_PROP_BF3(3);
+  else if (b.RT <= 5 * PW) AFQ_PROP_BF3(5);
+  else AFQ_PROP_BF3(8);
+#undef AFQ_PROP_BF3
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+}  // namespace afq

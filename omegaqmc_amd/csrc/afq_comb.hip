@@ -47,7 +47,7 @@ __global__ __launch_bounds__(NT) void k_comb_index(int W, const double* __restri
 }
 
 // One workgroup per destination walker: copy both spins and e_hybrid.
-__global__ __launch_bounds__(256) void k_comb_gather(long na, long nb,
+__global__ __launch_bounds__(256) void k_comb_gather(int W, long na, long nb,
                                                      const int32_t* __restrict__ idx,
                                                      const double4* __restrict__ pa,
                                                      const double4* __restrict__ pb,
@@ -57,11 +57,18 @@ __global__ __launch_bounds__(256) void k_comb_gather(long na, long nb,
                                                      cplx* __restrict__ eh_new) {
   const long dst = blockIdx.x;
   const long src = idx[dst];
-  // na, nb in units of complex pairs (double4 = 2 complex); odd tails handled below
   const cplx* pac = (const cplx*)pa;
   const cplx* pbc = (const cplx*)pb;
   cplx* pan = (cplx*)pa_new;
   cplx* pbn = (cplx*)pb_new;
+  if (src < 0 || src >= W) {
+    // caller-supplied index out of range: poison the row instead of reading outside
+    const cplx nan2 = make_double2(__builtin_nan(""), __builtin_nan(""));
+    for (long i = threadIdx.x; i < na; i += 256) pan[dst * na + i] = nan2;
+    for (long i = threadIdx.x; i < nb; i += 256) pbn[dst * nb + i] = nan2;
+    if (eh && threadIdx.x == 0) eh_new[dst] = nan2;
+    return;
+  }
   for (long i = threadIdx.x; i < na; i += 256) pan[dst * na + i] = pac[src * na + i];
   for (long i = threadIdx.x; i < nb; i += 256) pbn[dst * nb + i] = pbc[src * nb + i];
   if (eh && threadIdx.x == 0) eh_new[dst] = eh[src];
@@ -132,6 +139,25 @@ __global__ __launch_bounds__(1024) void k_block_stats(int W, const double* __res
 
 }  // namespace
 
+__global__ void k_fill1(int W, double* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < W) w[i] = 1.0;
+}
+
+// Gather walkers by given survivor indices, weights reset to 1.
+int gather_launch(hipStream_t s, int W, int M, int Na, int Nb, const int32_t* idx,
+                  const double* phia, const double* phib, const double* eh, double* w_new,
+                  double* phia_new, double* phib_new, double* eh_new) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, W, (long)M * Na, (long)M * Nb, idx,
+                     (const double4*)phia, (const double4*)phib, (const cplx*)eh,
+                     (double4*)phia_new, (double4*)phib_new, (cplx*)eh_new);
+  AFQ_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fill1, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, w_new);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
 int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
                 const double* phia, const double* phib, double r, const double* eh,
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
@@ -143,7 +169,7 @@ int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weigh
   if (lds) cum_scratch = nullptr;
   hipLaunchKernelGGL(k_comb_index, dim3(1), dim3(NT), sm, s, W, weights, r, cum_scratch, idx, w_new);
   AFQ_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, (long)M * Na, (long)M * Nb, idx,
+  hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, W, (long)M * Na, (long)M * Nb, idx,
                      (const double4*)phia, (const double4*)phib, (const cplx*)eh,
                      (double4*)phia_new, (double4*)phib_new, (cplx*)eh_new);
   AFQ_CHECK_LAUNCH();
@@ -176,6 +202,19 @@ int block_stats_launch(hipStream_t s, int W, const double* weights, const double
 }  // namespace afq
 
 extern "C" {
+
+int afq_population_control_apply(afq_ctx* ctx, int W, int M, int Na, int Nb, const int32_t* idx,
+                                 const double* phia, const double* phib, const double* eh,
+                                 double* w_new, double* phia_new, double* phib_new,
+                                 double* eh_new) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0, "bad shape");
+  AFQ_REQUIRE(idx && w_new && phia_new && phib_new, "NULL argument");
+  AFQ_REQUIRE(phia_new != phia && phib_new != phib, "outputs must not alias inputs");
+  AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
+  return afq::gather_launch(ctx->stream, W, M, Na, Nb, idx, phia, phib, eh, w_new, phia_new,
+                            phib_new, eh_new);
+}
 
 int afq_population_control_comb(afq_ctx* ctx, int W, int M, int Na, int Nb, const double* weights,
                                 const double* phia, const double* phib, double r,

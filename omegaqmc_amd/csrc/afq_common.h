@@ -48,6 +48,23 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
     return mkc((a.x * r + a.y) / d, (a.y * r - a.x) / d);
   }
 }
+// bf16 split of an FP64 value: hi = bf16(x), lo = bf16(x - hi), round to nearest even.
+typedef unsigned short bf16_t;
+__device__ __forceinline__ bf16_t bf_bits(float x) { return __builtin_bit_cast(bf16_t, (__bf16)x); }
+__device__ __forceinline__ float bf_val(bf16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
+__device__ __forceinline__ void split2(double x, bf16_t& h, bf16_t& l) {
+  h = bf_bits((float)x);
+  l = bf_bits((float)(x - (double)bf_val(h)));
+}
+
+// Optional split-bf16 output of the half-rotated Green's function:
+// per spin, planes [2][2W][K_s] with rows (walker, re/im), k = i*M + q.
+struct GhalfBf16 {
+  bf16_t* a;
+  bf16_t* b;
+  long Ka, Kb;
+};
+
 // NaN-propagating max/min (jnp.maximum / jnp.minimum semantics).
 __device__ __forceinline__ double jmax(double a, double b) {
   return (a != a || b != b) ? (a + b) : (a > b ? a : b);

@@ -5,7 +5,7 @@ namespace afq {
 // mode 0: overlap only; 1: Ghalf (W,N,M) + overlap.  Complex interleaved.
 int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,
                   const double* phib, const double* tu, const double* td, double* Gha,
-                  double* Ghb, double* ovlp);
+                  double* Ghb, double* ovlp, const GhalfBf16* gbf = nullptr);
 int full_g_launch(hipStream_t s, int W, int M, int N, const double* trial, const double* Ghalf,
                   double* G);
 }  // namespace afq

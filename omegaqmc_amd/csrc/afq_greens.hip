@@ -1,5 +1,6 @@
 // Overlap matrices, LU (partial pivoting), slogdet, inverse and the
-// half-rotated Green's function, one workgroup per walker.
+// half-rotated Green's function; one workgroup (4 waves) per walker, spins
+// in sequence.
 //
 // Reference: OmegaQMC/observables/greens.py:12-198
 //   S = phi^T Psi*      (einsum 'wpi,pj->wij')
@@ -12,7 +13,11 @@
 // singular S gives ovlp = 0 (slogdet's sign 0, logdet -inf), and its
 // inverse carries inf/NaN into Ghalf as in the reference.
 //
-// LDS: LU (N x N) + S^-1 (N x N) + a 16/32-row slab of phi, complex128; N <= 64.
+// The N x N work (N <= 64) is done by one wavefront with one lane per row
+// (LU, column-major in LDS) or per column (inverse): no workgroup barriers
+// inside the factorisation.  Ghalf = S^-1 phi^T runs on FP64 MFMA
+// (v_mfma_f64_16x16x4f64: A[i][k] lane i+16k, B[k][j] lane j+16k, D[i][j]
+// lane j+16(i%4) register i/4) over all four waves.
 #include "afq_gemm.h"
 #include "afq_greens.h"
 
@@ -21,60 +26,63 @@ namespace afq {
 namespace {
 
 constexpr int NT = 256;
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 __device__ __forceinline__ double cabs1(cplx a) { return fabs(a.x) + fabs(a.y); }
 
-// Factor S (in LDS, N x N row-major) in place; returns overlap via *ov (thread 0).
-__device__ void lu_slogdet(cplx* LU, int* perm, int N, cplx* ov_out, double* red_v, int* red_i) {
-  const int t = threadIdx.x;
+// Orders the LDS accesses of one wavefront (lanes exchange data through LDS).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LU of A (column-major, A[j*LD + i] = S[i][j]) by one wavefront, lane = row.
+// Returns the overlap sign*exp(logdet) in *ov (lane 0).
+__device__ void wave_lu(cplx* A, int LD, int N, int* perm, cplx* ov) {
+  const int l = threadIdx.x & 63;
   for (int k = 0; k < N; ++k) {
-    // pivot search in column k, rows k..N-1 (wave 0)
-    if (t < 64) {
-      double best = -1.0;
-      int bi = k;
-      for (int r = k + t; r < N; r += 64) {
-        double v = cabs1(LU[r * N + k]);
-        if (v > best || (v != v && best == -1.0)) { best = v; bi = r; }
-      }
-      // reduce (max value, then smallest row)
-      for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(best, off);
-        int oi = __shfl_xor(bi, off);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
-      if (t == 0) { red_i[0] = bi; perm[k] = bi; }
+    double v = -1.0;
+    if (l >= k && l < N) {
+      v = cabs1(A[k * LD + l]);
+      if (v != v) v = -0.5;  // a NaN row is taken only if nothing else is left
     }
-    __syncthreads();
-    const int piv = red_i[0];
-    if (piv != k) {
-      for (int j = t; j < N; j += NT) {
-        cplx a = LU[k * N + j];
-        LU[k * N + j] = LU[piv * N + j];
-        LU[piv * N + j] = a;
-      }
+    int bi = l;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov2 = __shfl_xor(v, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov2 > v || (ov2 == v && oi < bi)) { v = ov2; bi = oi; }
     }
-    __syncthreads();
-    const cplx p = LU[k * N + k];
-    const bool nz = (p.x != 0.0 || p.y != 0.0);
-    if (nz) {
-      const cplx rp = cdiv(mkc(1.0, 0.0), p);
-      for (int i = k + 1 + t; i < N; i += NT) LU[i * N + k] = cmul(LU[i * N + k], rp);
+    const int piv = bi;
+    if (l == 0) perm[k] = piv;
+    if (piv != k && l < N) {
+      const cplx a = A[l * LD + k];
+      A[l * LD + k] = A[l * LD + piv];
+      A[l * LD + piv] = a;
     }
-    __syncthreads();
-    const int rem = N - k - 1;
-    for (int e = t; e < rem * rem; e += NT) {
-      int i = k + 1 + e / rem, j = k + 1 + e % rem;
-      cfms(LU[i * N + j], LU[i * N + k], LU[k * N + j]);
+    wave_sync();
+    const cplx p = A[k * LD + k];
+    if (l > k && l < N && (p.x != 0.0 || p.y != 0.0))
+      A[k * LD + l] = cmul(A[k * LD + l], cdiv(mkc(1.0, 0.0), p));
+    wave_sync();
+    if (l > k && l < N) {
+      const cplx li = A[k * LD + l];
+      for (int j = k + 1; j < N; ++j) cfms(A[j * LD + l], li, A[j * LD + k]);
     }
-    __syncthreads();
+    wave_sync();
   }
-  if (t == 0) {
+  if (l == 0) {
     double logabs = 0.0;
     cplx ph = mkc(1.0, 0.0);
     bool sing = false;
     for (int k = 0; k < N; ++k) {
-      cplx d = LU[k * N + k];
-      double a = hypot(d.x, d.y);
+      const cplx d = A[k * LD + k];
+      const double a = hypot(d.x, d.y);
       if (a == 0.0) sing = true;
       else {
         logabs += log(a);
@@ -82,30 +90,71 @@ __device__ void lu_slogdet(cplx* LU, int* perm, int N, cplx* ov_out, double* red
       }
       if (perm[k] != k) ph = mkc(-ph.x, -ph.y);
     }
-    *ov_out = sing ? mkc(0.0, 0.0) : cscale(ph, exp(logabs));
+    *ov = sing ? mkc(0.0, 0.0) : cscale(ph, exp(logabs));
   }
-  (void)red_v;
 }
 
-// Column j of S^-1 from the packed LU (threads j < N), into Inv (row-major).
-__device__ void lu_inverse(const cplx* LU, const int* perm, int N, cplx* Inv) {
-  const int j = threadIdx.x;
-  if (j < N) {
-    int pos = j;
-    for (int k = 0; k < N; ++k) {
-      if (pos == k) pos = perm[k];
-      else if (pos == perm[k]) pos = k;
+// Inv (row-major, Inv[i*LD + j]) = S^-1 from the LU, one wavefront, lane = column.
+__device__ void wave_inverse(const cplx* A, int LD, const int* perm, int N, cplx* Inv) {
+  const int j = threadIdx.x & 63;
+  if (j >= N) return;
+  int pos = j;
+  for (int k = 0; k < N; ++k) {
+    if (pos == k) pos = perm[k];
+    else if (pos == perm[k]) pos = k;
+  }
+  for (int i = 0; i < N; ++i) Inv[i * LD + j] = mkc(i == pos ? 1.0 : 0.0, 0.0);
+  for (int i = pos + 1; i < N; ++i) {
+    cplx x = Inv[i * LD + j];
+    for (int k = pos; k < i; ++k) cfms(x, A[k * LD + i], Inv[k * LD + j]);
+    Inv[i * LD + j] = x;
+  }
+  for (int i = N - 1; i >= 0; --i) {
+    cplx x = Inv[i * LD + j];
+    for (int k = i + 1; k < N; ++k) cfms(x, A[k * LD + i], Inv[k * LD + j]);
+    Inv[i * LD + j] = cdiv(x, A[i * LD + i]);
+  }
+}
+
+// Ghalf (N x M) = Inv (N x N) . phi^T on MFMA; optional FP64 and split-bf16 outputs.
+__device__ void ghalf_mfma(const cplx* Inv, int LD, int N, int M, const cplx* __restrict__ phi,
+                           cplx* __restrict__ Ghalf, bf16_t* __restrict__ gb, long Ks,
+                           long gplane) {
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = l & 15, lq = l >> 4;
+  const int RTn = (N + 15) >> 4, RTm = (M + 15) >> 4;
+  const int kN = (N + 3) & ~3;
+  for (int pi = wv; pi < RTn * RTm; pi += NT / 64) {
+    const int ti = pi / RTm, tj = pi % RTm;
+    d4v cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    const int ai = 16 * ti + li;  // row of Inv
+    const int bj = 16 * tj + li;  // row of phi (column of phi^T)
+    for (int k0 = 0; k0 < kN; k0 += 4) {
+      const int k = k0 + lq;
+      const cplx a = (ai < N && k < N) ? Inv[ai * LD + k] : mkc(0.0, 0.0);
+      const cplx b = (bj < M && k < N) ? phi[(long)bj * N + k] : mkc(0.0, 0.0);
+      cr = mfma64(a.x, b.x, cr);
+      ci = mfma64(a.x, b.y, ci);
+      cr = mfma64(-a.y, b.y, cr);
+      ci = mfma64(a.y, b.x, ci);
     }
-    for (int i = 0; i < N; ++i) Inv[i * N + j] = mkc(i == pos ? 1.0 : 0.0, 0.0);
-    for (int i = pos + 1; i < N; ++i) {
-      cplx x = Inv[i * N + j];
-      for (int k = pos; k < i; ++k) cfms(x, LU[i * N + k], Inv[k * N + j]);
-      Inv[i * N + j] = x;
-    }
-    for (int i = N - 1; i >= 0; --i) {
-      cplx x = Inv[i * N + j];
-      for (int k = i + 1; k < N; ++k) cfms(x, LU[i * N + k], Inv[k * N + j]);
-      Inv[i * N + j] = cdiv(x, LU[i * N + i]);
+    const int q = 16 * tj + li;
+    if (q >= M) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * ti + lq + 4 * r;
+      if (i >= N) continue;
+      const long k = (long)i * M + q;
+      if (Ghalf) Ghalf[k] = mkc(cr[r], ci[r]);
+      if (gb) {
+        bf16_t h, lo;
+        split2(cr[r], h, lo);
+        gb[k] = h;
+        gb[gplane + k] = lo;
+        split2(ci[r], h, lo);
+        gb[Ks + k] = h;
+        gb[gplane + Ks + k] = lo;
+      }
     }
   }
 }
@@ -113,25 +162,31 @@ __device__ void lu_inverse(const cplx* LU, const int* perm, int N, cplx* Inv) {
 // One spin of one walker.  mode 0: overlap only; 1: + Ghalf.  eye != 0
 // asserts trial == eye(M, N) exactly, so S[i][j] = phi[j][i] (the products
 // with the zero entries of the trial contribute exact zeros).
+// gb (optional): split-bf16 planes of Ghalf at this walker's real-part row;
+// the imaginary row follows at +Ks and the lo plane at +gplane.
+template <bool GENERAL>
 __device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
                             const cplx* __restrict__ phi, const cplx* __restrict__ trial,
-                            cplx* __restrict__ Ghalf, cplx* ov_out, char* smem) {
+                            cplx* __restrict__ Ghalf, cplx* ov_out, char* smem, bf16_t* gb,
+                            long Ks, long gplane) {
   const int t = threadIdx.x;
   if (N == 0) {
     if (t == 0) *ov_out = mkc(1.0, 0.0);
     __syncthreads();
     return;
   }
-  cplx* LU = (cplx*)smem;
-  cplx* Inv = LU + N * N;
-  cplx* slab = Inv + N * N;
-  cplx* slab_t = slab + QCH * N;
+  const int LD = N | 1;
+  cplx* LU = (cplx*)smem;          // column-major [N][LD]
+  cplx* Inv = LU + N * LD;         // row-major [N][LD]
+  cplx* slab = Inv + N * LD;       // [QCH][N]
+  cplx* slab_t = slab + QCH * N;   // [QCH][N]
   int* perm = (int*)(slab_t + QCH * N);
-  int* red_i = perm + 64;
-  double* red_v = (double*)(red_i + 4);
 
-  if (eye) {
-    for (int e = t; e < N * N; e += NT) LU[e] = phi[(long)(e % N) * N + e / N];
+  if (!GENERAL || eye) {
+    for (int e = t; e < N * N; e += NT) {
+      const int j = e / N, i = e % N;
+      LU[j * LD + i] = phi[(long)j * N + i];
+    }
   } else {
     // S[i][j] = sum_p phi[p][i] conj(trial[p][j]), p staged through LDS
     constexpr int RMAX = (64 * 64 + NT - 1) / NT;
@@ -158,45 +213,42 @@ __device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
       const int e = t + NT * r;
-      if (e < N * N) LU[e] = acc[r];
+      if (e < N * N) LU[(e % N) * LD + e / N] = acc[r];
     }
   }
   __syncthreads();
-  lu_slogdet(LU, perm, N, ov_out, red_v, red_i);
+  if (t < 64) wave_lu(LU, LD, N, perm, ov_out);
   __syncthreads();
   if (mode == 0) return;
-  lu_inverse(LU, perm, N, Inv);
+  if (t < 64) wave_inverse(LU, LD, perm, N, Inv);
   __syncthreads();
-  // Ghalf[i][q] = sum_j Inv[i][j] phi[q][j], q in slabs of QCH rows of phi
-  for (int q0 = 0; q0 < M; q0 += QCH) {
-    int nq = min(QCH, M - q0);
-    for (int e = t; e < nq * N; e += NT) slab[e] = phi[(long)q0 * N + e];
-    __syncthreads();
-    for (int e = t; e < nq * N; e += NT) {
-      int i = e / nq, ql = e % nq;
-      cplx acc = mkc(0.0, 0.0);
-      for (int j = 0; j < N; ++j) cfma(acc, Inv[i * N + j], slab[ql * N + j]);
-      Ghalf[(long)i * M + q0 + ql] = acc;
-    }
-    __syncthreads();
-  }
+  ghalf_mfma(Inv, LD, N, M, phi, Ghalf, gb, Ks, gplane);
+  __syncthreads();
 }
 
+// GENERAL = false: both trials are identity matrices (S is a gather).
+template <bool GENERAL>
 __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, int qch,
                                                const cplx* __restrict__ phia,
                                                const cplx* __restrict__ phib,
                                                const cplx* __restrict__ tu,
                                                const cplx* __restrict__ td,
                                                cplx* __restrict__ Gha, cplx* __restrict__ Ghb,
-                                               cplx* __restrict__ ovlp) {
+                                               cplx* __restrict__ ovlp, GhalfBf16 gbf) {
   extern __shared__ __align__(16) char smem[];
   __shared__ cplx ov[2];
   const long w = blockIdx.x;
-  greens_spin(mode & 1, (mode >> 1) & 1, M, Na, qch, phia + w * M * Na, tu,
-              (mode & 1) ? Gha + w * Na * M : nullptr, &ov[0], smem);
+  const long W = gridDim.x;
+  const bool gh = (mode & 1) != 0;
+  bf16_t* ga = (gh && gbf.a) ? gbf.a + 2 * w * gbf.Ka : nullptr;
+  bf16_t* gbb = (gh && gbf.b) ? gbf.b + 2 * w * gbf.Kb : nullptr;
+  greens_spin<GENERAL>(mode & 1, (mode >> 1) & 1, M, Na, qch, phia + w * M * Na, tu,
+              (gh && Gha) ? Gha + w * Na * M : nullptr, &ov[0], smem, ga, gbf.Ka,
+              2 * W * gbf.Ka);
   __syncthreads();
-  greens_spin(mode & 1, (mode >> 2) & 1, M, Nb, qch, phib + w * M * Nb, td,
-              (mode & 1) ? Ghb + w * Nb * M : nullptr, &ov[1], smem);
+  greens_spin<GENERAL>(mode & 1, (mode >> 2) & 1, M, Nb, qch, phib + w * M * Nb, td,
+              (gh && Ghb) ? Ghb + w * Nb * M : nullptr, &ov[1], smem, gbb, gbf.Kb,
+              2 * W * gbf.Kb);
   __syncthreads();
   if (threadIdx.x == 0) ovlp[w] = cmul(ov[0], ov[1]);
 }
@@ -206,24 +258,31 @@ __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, 
 static int greens_qch(int Na, int Nb) { return (Na > 48 || Nb > 48) ? 8 : 16; }
 
 size_t greens_smem_bytes(int Na, int Nb) {
-  int N = Na > Nb ? Na : Nb;
-  return (size_t)(2 * N * N + 2 * greens_qch(Na, Nb) * N) * sizeof(cplx) + 64 * sizeof(int) + 4 * sizeof(int) +
-         64 * sizeof(double) + 64;
+  const int N = Na > Nb ? Na : Nb;
+  const int LD = N | 1;
+  return (size_t)(2 * N * LD + 2 * greens_qch(Na, Nb) * N) * sizeof(cplx) + 64 * sizeof(int) + 64;
 }
 
 int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,
                   const double* phib, const double* tu, const double* td, double* Gha,
-                  double* Ghb, double* ovlp) {
+                  double* Ghb, double* ovlp, const GhalfBf16* gbf) {
   if (Na > 64 || Nb > 64) return fail(AFQ_EUNSUPPORTED, "nocc > 64 is not supported (got %d, %d)", Na, Nb);
   if (W <= 0) return AFQ_OK;
-  size_t sm = greens_smem_bytes(Na, Nb);
+  const size_t sm = greens_smem_bytes(Na, Nb);
   if (sm + 64 > max_lds_bytes())
     return fail(AFQ_EUNSUPPORTED, "greens: %zu bytes of LDS needed, device allows %zu", sm,
                 max_lds_bytes());
-  hipLaunchKernelGGL(k_greens, dim3(W), dim3(NT), sm, s, mode, M, Na, Nb, greens_qch(Na, Nb),
-                     (const cplx*)phia,
-                     (const cplx*)phib, (const cplx*)tu, (const cplx*)td, (cplx*)Gha, (cplx*)Ghb,
-                     (cplx*)ovlp);
+  GhalfBf16 g{};
+  if (gbf) g = *gbf;
+  const bool eye_both = ((mode >> 1) & 1 || Na == 0) && ((mode >> 2) & 1 || Nb == 0);
+  if (eye_both)
+    hipLaunchKernelGGL((k_greens<false>), dim3(W), dim3(NT), sm, s, mode, M, Na, Nb,
+                       greens_qch(Na, Nb), (const cplx*)phia, (const cplx*)phib, (const cplx*)tu,
+                       (const cplx*)td, (cplx*)Gha, (cplx*)Ghb, (cplx*)ovlp, g);
+  else
+    hipLaunchKernelGGL((k_greens<true>), dim3(W), dim3(NT), sm, s, mode, M, Na, Nb,
+                       greens_qch(Na, Nb), (const cplx*)phia, (const cplx*)phib, (const cplx*)tu,
+                       (const cplx*)td, (cplx*)Gha, (cplx*)Ghb, (cplx*)ovlp, g);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

@@ -9,7 +9,7 @@ int copy_async(hipStream_t s, const double* src, double* dst, long n);
 // greens (afq_greens.hip)
 int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,
                   const double* phib, const double* tu, const double* td, double* Gha,
-                  double* Ghb, double* ovlp);
+                  double* Ghb, double* ovlp, const GhalfBf16* gbf = nullptr);
 int full_g_launch(hipStream_t s, int W, int M, int N, const double* trial, const double* Ghalf,
                   double* G);
 
@@ -18,7 +18,8 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
                          const double* rb, const double* Gha, const double* Ghb, double* vbiasT);
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
-                 double* cmf, int nsplit);
+                 double* cmf, int nsplit, const float* vpart = nullptr, bf16_t* Xp = nullptr,
+                 int Gp = 0);
 int build_vhs_fp64(hipStream_t s, int W, int M, int G, const double* xs, const double* chol,
                    double dt, double* VHS);
 int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, const double* phi,
@@ -34,11 +35,15 @@ int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, 
 
 // stabilisation / population control (afq_ortho.hip, afq_comb.hip)
 int ortho_launch(hipStream_t s, int W, int M, int Na, int Nb, double* phia, double* phib,
-                 double* log_detR, double* sign_detR);
+                 double* log_detR, double* sign_detR, double* scratch2w);
+size_t ortho_scratch_bytes(int W);
 int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
                 const double* phia, const double* phib, double r, const double* eh,
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
                 double* cum_scratch);
+int gather_launch(hipStream_t s, int W, int M, int Na, int Nb, const int32_t* idx,
+                  const double* phia, const double* phib, const double* eh, double* w_new,
+                  double* phia_new, double* phib_new, double* eh_new);
 int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp);
 int block_stats_launch(hipStream_t s, int W, const double* weights, const double* e, double* out);
 int accumulate_launch(hipStream_t s, int W, const double* weights, const double* eh, double* acc);

@@ -14,19 +14,33 @@ namespace {
 constexpr int NT = 256;
 
 // afqmc_gto.py:687-706 per walker (one workgroup): cap, shift, cfb, cmf.
+// vbias comes as nsplit FP64 partials (vbias) or FP32 partials (vpart), summed
+// in FP64 in split order.  Optional outputs: xbar, xshifted (FP64) and the
+// split-bf16 planes of xshifted, Xp[2][2W][Gp] with rows (w, re/im).
 __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
                                               const cplx* __restrict__ vbias,  // nsplit x (W,G)
+                                              const float2* __restrict__ vpart,
                                               const cplx* __restrict__ mf,            // (G,)
                                               const double* __restrict__ xi,          // (W,G)
                                               double sqdt, double fb, cplx* __restrict__ xbar_out,
                                               cplx* __restrict__ xs_out, cplx* __restrict__ cfb,
-                                              cplx* __restrict__ cmf) {
+                                              cplx* __restrict__ cmf, bf16_t* __restrict__ Xp,
+                                              int Gp, long Xplane) {
   __shared__ double red[32];
   const long w = blockIdx.x;
   cplx s_mf = mkc(0, 0), s_xixb = mkc(0, 0), s_xb2 = mkc(0, 0);
   for (int g = threadIdx.x; g < G; g += NT) {
-    cplx v = vbias[w * G + g];
-    for (int sp = 1; sp < nsplit; ++sp) v = cadd(v, vbias[sp * sstride + w * G + g]);
+    cplx v;
+    if (vpart) {
+      v = mkc(0.0, 0.0);
+      for (int sp = 0; sp < nsplit; ++sp) {
+        const float2 pv = vpart[sp * sstride + w * G + g];
+        v = cadd(v, mkc((double)pv.x, (double)pv.y));
+      }
+    } else {
+      v = vbias[w * G + g];
+      for (int sp = 1; sp < nsplit; ++sp) v = cadd(v, vbias[sp * sstride + w * G + g]);
+    }
     const cplx m = mf[g];
     cplx xb = mkc(sqdt * (v.y + m.x), -sqdt * (v.x - m.y));
     const double a = hypot(xb.x, xb.y);
@@ -34,7 +48,16 @@ __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
     const double x = xi[w * G + g];
     const cplx xs = mkc(x - xb.x, -xb.y);
     if (xbar_out) xbar_out[w * G + g] = xb;
-    xs_out[w * G + g] = xs;
+    if (xs_out) xs_out[w * G + g] = xs;
+    if (Xp) {
+      bf16_t h, l;
+      split2(xs.x, h, l);
+      Xp[2 * w * Gp + g] = h;
+      Xp[Xplane + 2 * w * Gp + g] = l;
+      split2(xs.y, h, l);
+      Xp[(2 * w + 1) * Gp + g] = h;
+      Xp[Xplane + (2 * w + 1) * Gp + g] = l;
+    }
     s_mf = cadd(s_mf, cmul(xs, m));
     s_xixb = cadd(s_xixb, cscale(xb, x));
     s_xb2 = cadd(s_xb2, cmul(xb, xb));
@@ -116,10 +139,11 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
 
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
-                 double* cmf, int nsplit) {
+                 double* cmf, int nsplit, const float* vpart, bf16_t* Xp, int Gp) {
   if (W <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, nsplit, (long)W * G, (const cplx*)vbiasT, (const cplx*)mf, xi,
-                     sqrt(dt), fb, (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf);
+  hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, nsplit, (long)W * G,
+                     (const cplx*)vbiasT, (const float2*)vpart, (const cplx*)mf, xi, sqrt(dt), fb,
+                     (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf, Xp, Gp, 2L * W * Gp);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

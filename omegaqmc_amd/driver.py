@@ -78,8 +78,14 @@ class AFQMCDriver:
         }
         return st
 
-    def step(self, st, step_count, seed, eshift, stabilize_freq, pop_control_freq):
-        """One driver step (afqmc_gto.py:1387-1451) on preallocated buffers."""
+    def step(self, st, step_count, seed, eshift, stabilize_freq, pop_control_freq,
+             comb_idx=None):
+        """One driver step (afqmc_gto.py:1387-1451) on preallocated buffers.
+
+        ``comb_idx`` (device int32 (W,)) replays given survivor indices at a
+        population-control step instead of drawing the comb -- used to follow
+        another run's trajectory exactly (e.g. the FP64 run of the same fields).
+        """
         L = _lib.load()
         ctx = _lib.context().handle
         W = st['phia'].shape[0]
@@ -93,12 +99,21 @@ class AFQMCDriver:
         if step_count > 1:
             check(L.afq_clip_weights(ctx, W, ptr(st['weights']), WEIGHT_CLIP_FRACTION), "clip")
         if step_count % pop_control_freq == 0:
-            r = float(L.afq_comb_offset(seed & (2 ** 64 - 1), step_count))
-            check(L.afq_population_control_comb(ctx, W, M, Na, Nb, ptr(st['weights']),
-                                                ptr(st['phia']), ptr(st['phib']), r,
-                                                ptr(st['e_hybrid']), ptr(st['weights2']),
-                                                ptr(st['phia2']), ptr(st['phib2']),
-                                                ptr(st['e_hybrid2']), ptr(st['idx'])), "comb")
+            if comb_idx is not None:
+                st['idx'].copy_(comb_idx)
+                check(L.afq_population_control_apply(ctx, W, M, Na, Nb, ptr(st['idx']),
+                                                      ptr(st['phia']), ptr(st['phib']),
+                                                      ptr(st['e_hybrid']), ptr(st['weights2']),
+                                                      ptr(st['phia2']), ptr(st['phib2']),
+                                                      ptr(st['e_hybrid2'])), "comb apply")
+            else:
+                r = float(L.afq_comb_offset(seed & (2 ** 64 - 1), step_count))
+                check(L.afq_population_control_comb(ctx, W, M, Na, Nb, ptr(st['weights']),
+                                                    ptr(st['phia']), ptr(st['phib']), r,
+                                                    ptr(st['e_hybrid']), ptr(st['weights2']),
+                                                    ptr(st['phia2']), ptr(st['phib2']),
+                                                    ptr(st['e_hybrid2']), ptr(st['idx'])),
+                      "comb")
             for a, b in (('phia', 'phia2'), ('phib', 'phib2'), ('weights', 'weights2'),
                          ('e_hybrid', 'e_hybrid2')):
                 st[a], st[b] = st[b], st[a]

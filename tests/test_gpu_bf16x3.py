@@ -1,0 +1,84 @@
+"""Split-BF16 (BF16x3, FP32 accumulate) step against the FP64 step.
+
+Both precisions are fed identical auxiliary fields and comb offsets.  The
+per-product error of BF16x3 is ~2^-16 relative, so one step must agree with
+the FP64 oracle to ~1e-5 relative in the walkers and the block energy of a
+short trajectory must stay within the 0.5 mHa budget of the FP64 run.
+"""
+
+import numpy as np
+import pytest
+
+from afq_cases import rel_err
+from oracle import afqmc as O
+from oracle import philox as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def afq(afq_lib):
+    import torch
+    import omegaqmc_amd as A
+    torch.cuda.set_device(0)
+    return A
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _n(t):
+    return t.detach().cpu().numpy()
+
+
+def _drivers(M, Na, Nb, G, normalise=True):
+    from omegaqmc_amd.synth import make_system
+    from omegaqmc_amd.driver import AFQMCDriver
+    s = make_system(M, Na, Nb, G, seed=42, normalise=normalise)
+    integ = {'h1e': _t(s.h1e), 'h1e_mod': _t(s.h1e_mod), 'chol': _t(s.chol), 'enuc': s.enuc,
+             'nbasis': M, 'nup': Na, 'ndown': Nb}
+    return (s, AFQMCDriver(integ, verbose=False, precision="fp64"),
+            AFQMCDriver(integ, verbose=False, precision="bf16x3"))
+
+
+@pytest.mark.parametrize("M,Na,Nb,G,W", [(40, 5, 4, 120, 64), (64, 8, 7, 200, 130),
+                                         (100, 13, 13, 300, 32)])
+def test_one_step_close_to_oracle(afq, M, Na, Nb, G, W):
+    s, d64, dbf = _drivers(M, Na, Nb, G)
+    od = O.Driver(s.h1e, s.h1e_mod, s.chol, s.enuc, Na, Nb)
+    rng = np.random.default_rng(3)
+    pa = od.trial_up + 0.1 * (rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na)))
+    pb = od.trial_dn + 0.1 * (rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb)))
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    xi = P.aux_fields(5, 1, W, G)
+    eshift = od.ehf - od.e_const
+    ref = O.propagate_walkers(pa, pb, w, None, eh, od.propagator, od.chol, od.rchol_a, od.rchol_b,
+                              od.trial_up, od.trial_dn, eshift, xi)
+    ta, tb, tw, te = _t(pa), _t(pb), _t(w), _t(eh)
+    ov = dbf.system.step(ta, tb, tw, te, eshift, xi=_t(xi))
+    for x, y, tol in ((ta, ref[0], 1e-4), (tb, ref[1], 1e-4), (ov, ref[3], 1e-3),
+                      (tw, ref[2], 1e-4)):
+        assert rel_err(_n(x), y) < tol
+
+
+def test_trajectory_energy_within_budget(afq):
+    """20 steps (QR every 5, comb every 10) with identical fields; at each comb
+    the BF16x3 run replays the FP64 run's survivor indices (same trajectory).
+    The mixed estimator of the BF16x3 walkers stays within 0.5 mHa of FP64."""
+    import torch
+    s, d64, dbf = _drivers(60, 8, 8, 240)
+    W = 128
+    s64, sbf = d64.init_walkers(W), dbf.init_walkers(W)
+    eshift = d64.ehf - d64.e_const
+    for step in range(1, 21):
+        d64.step(s64, step, 42, eshift, 5, 10)
+        dbf.step(sbf, step, 42, eshift, 5, 10, comb_idx=s64['idx'] if step % 10 == 0 else None)
+    d64.block_energy(s64)
+    dbf.block_energy(sbf)
+    torch.cuda.synchronize()
+    e = {"fp64": float(s64['stats'][0].item()), "bf16x3": float(sbf['stats'][0].item())}
+    assert np.isfinite(e["fp64"])
+    assert abs(e["bf16x3"] - e["fp64"]) < 0.5e-3, e

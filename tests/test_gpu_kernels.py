@@ -203,6 +203,43 @@ def test_comb_indices_bit_exact(afq, W):
         assert np.all(_n(wn) == 1.0)
 
 
+def test_population_control_apply_replays_indices(afq):
+    """afq_population_control_apply with given survivor indices gathers the
+    same walkers as the comb that chose them (weights reset to 1)."""
+    import torch
+    from omegaqmc_amd import _lib
+    rng = np.random.default_rng(5)
+    W, M, Na, Nb = 37, 6, 2, 1
+    pa = rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na))
+    pb = rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb))
+    eh = rng.normal(size=W) + 1j * rng.normal(size=W)
+    idx = rng.integers(0, W, size=W).astype(np.int32)
+    idx[:3] = W - 1
+    out = [torch.empty_like(_t(x)) for x in (pa, pb, eh)]
+    wn = torch.empty(W, dtype=torch.float64, device="cuda")
+    ti, tpa, tpb, teh = _t(idx), _t(pa), _t(pb), _t(eh)
+    L = _lib.load()
+    _lib.check(L.afq_population_control_apply(
+        _lib.context().handle, W, M, Na, Nb, _lib.ptr(ti), _lib.ptr(tpa), _lib.ptr(tpb),
+        _lib.ptr(teh), _lib.ptr(wn), _lib.ptr(out[0]), _lib.ptr(out[1]), _lib.ptr(out[2])),
+        "apply")
+    torch.cuda.synchronize()
+    assert np.array_equal(_n(out[0]), pa[idx]) and np.array_equal(_n(out[1]), pb[idx])
+    assert np.array_equal(_n(out[2]), eh[idx]) and np.all(_n(wn) == 1.0)
+    # an out-of-range index is never read: its output row is poisoned with NaN
+    bad_idx = idx.copy()
+    bad_idx[4], bad_idx[7] = W, -1
+    _lib.check(L.afq_population_control_apply(
+        _lib.context().handle, W, M, Na, Nb, _lib.ptr(_t(bad_idx)), _lib.ptr(tpa), _lib.ptr(tpb),
+        _lib.ptr(teh), _lib.ptr(wn), _lib.ptr(out[0]), _lib.ptr(out[1]), _lib.ptr(out[2])),
+        "apply")
+    torch.cuda.synchronize()
+    got = _n(out[0])
+    assert np.all(np.isnan(got[[4, 7]])) and np.all(np.isnan(_n(out[2])[[4, 7]]))
+    keep = [i for i in range(W) if i not in (4, 7)]
+    assert np.array_equal(got[keep], pa[idx[keep]])
+
+
 def test_aux_fields_match_oracle_stream(afq):
     import omegaqmc_amd as A
     for (W, G, step) in ((4, 7, 1), (33, 50, 12345)):
