@@ -13,7 +13,7 @@ beside it, with the energy error of the reduced-precision path against the
 FP64 run of the same trajectory.  ``roofline`` is computed from HIP events
 recorded around each phase of the fused step during the timed region.
 ``cpu_baseline`` times the NumPy FP64 oracle on this host on a bounded
-sample (a subset of walkers of the same system, one step).
+sample (a subset of walkers of the same system, a few steps).
 """
 
 from __future__ import annotations
@@ -52,8 +52,44 @@ def phase_flops(M, Na, Nb, G, W, T=6):
         "one_body": W * sum(8.0 * M * M * N for N in Ns),       # both half steps
         "force_bias": W * sum(4.0 * G * N * M for N in Ns),
         "vhs": W * 4.0 * G * M * M,
-        "taylor": W * sum(8.0 * T * M * M * N for N in Ns),
+        # one fused kernel: expH1 half step, T Taylor terms, expH1 half step
+        "taylor": W * sum((8.0 * T + 8.0) * M * M * N for N in Ns),
     }
+
+
+KERNEL_OF_PHASE = {
+    ("taylor", "fp64"): "k_prop_f64 (expH1 + 6-term Taylor + expH1, FP64 MFMA)",
+    ("taylor", "bf16x3"): "k_prop_bf3 (expH1 + 6-term Taylor + expH1, BF16x3 MFMA)",
+    ("vhs", "fp64"): "k_vhs_f64 (packed VHS build, FP64 MFMA)",
+    ("vhs", "bf16x3"): "k_bf3_gemm<0> (VHS build, BF16x3 MFMA)",
+    ("force_bias", "fp64"): "k_fb_f64 (split-K force bias, FP64 MFMA)",
+    ("force_bias", "bf16x3"): "k_bf3_gemm<1> (split-K force bias, BF16x3 MFMA)",
+    ("greens", "fp64"): "k_greens (wave LU + MFMA Ghalf)",
+    ("greens", "bf16x3"): "k_greens (wave LU + MFMA Ghalf)",
+}
+
+
+# rocprofv3 symbol of the dominant kernel at the M=300 workload (MT = 16-row
+# tiles per wave for M=300)
+KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop_f64<3>", ("taylor", "bf16x3"): "k_prop_bf3<5>",
+                 ("vhs", "fp64"): "k_vhs_f64<true>", ("vhs", "bf16x3"): "k_bf3_gemm<0>",
+                 ("force_bias", "fp64"): "k_fb_f64", ("force_bias", "bf16x3"): "k_bf3_gemm<1>",
+                 ("greens", "fp64"): "k_greens<false>", ("greens", "bf16x3"): "k_greens<false>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phaseless_step_M300_N40_G1800_W1024_hbm_traffic.md")
+
+
+def measured_traffic(symbol):
+    """HBM bytes per launch of ``symbol`` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain-copy calibration); None if absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic" and " ".join(parts[1:-1]) == symbol:
+                    return float(parts[-1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_info():
@@ -159,6 +195,19 @@ def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10):
     return worst, trace
 
 
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 3}))
+
+
 def cpu_baseline(sys_np, sample_walkers=8, nsteps=1):
     """Oracle (NumPy FP64 restatement) on this host for a bounded sample."""
     from oracle import afqmc as O
@@ -190,7 +239,10 @@ def main():
     ap.add_argument("--walkers", type=int, default=1024)
     ap.add_argument("--precision", default="auto", help="fp64 | fp32 | bf16x3 | auto")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-walkers", type=int, default=8)
+    ap.add_argument("--cpu-walkers", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--acc-steps", type=int, default=30,
                     help="steps of the same-trajectory energy comparison")
     args = ap.parse_args()
@@ -203,6 +255,9 @@ def main():
     from omegaqmc_amd import _lib
     from omegaqmc_amd.synth import make_system
     torch.cuda.set_device(0)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
     M, Na, Nb, G, W = 300, 40, 40, 1800, args.walkers
     print(f"[bench] generating system M={M} N={Na}+{Nb} G={G}", file=sys.stderr, flush=True)
     # size-normalised L_g variance: with the literal sigma the walker overlap
@@ -231,6 +286,9 @@ def main():
             torch.cuda.empty_cache()
             print(f"[bench] {prec} vs FP64 energy, same trajectory ({args.acc_steps} steps)",
                   file=sys.stderr, flush=True)
+            if args.acc_steps < 10:
+                extra[f"{prec}_energy_err_mHa"] = None  # unchecked: never reported as value
+                continue
             dE, trace = lockstep_energy_error(sys_np, W, args.acc_steps, prec)
             extra[f"{prec}_energy_err_mHa"] = round(dE * 1e3, 5)
             extra[f"{prec}_energy_err_trace_mHa"] = [round(x, 5) for x in trace]
@@ -249,24 +307,27 @@ def main():
     if dom is not None:
         per_launch_ms = dom_ms / best["phases"][dom]["intervals"]
         flops = pf[dom]
-        if dom == "one_body":
-            flops /= 2  # two timed intervals per step, one half step each
         ach = flops / (per_launch_ms * 1e-3) / 1e12
         peak = PEAK["fp64"] if best_prec == "fp64" else (PEAK["fp32"] if best_prec == "fp32"
                                                          else PEAK["bf16"] / 3)
-        roof = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
-                "ms_per_launch": round(per_launch_ms, 4)}
+        sym = KERNEL_SYMBOL.get((dom, best_prec))
+        traffic = measured_traffic(sym) if (sym and M == 300 and W == 1024) else None
+        roof = {"bound": "mfma", "kernel": KERNEL_OF_PHASE.get((dom, best_prec), dom),
+                "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": traffic,
+                "ms_per_launch": round(per_launch_ms, 4), "flops_per_launch": flops,
+                "symbol": sym}
 
     cpu = None
     if not args.no_cpu:
         model, ncpu = cpu_info()
         os.environ.setdefault("OMP_NUM_THREADS", str(ncpu))
         print(f"[bench] CPU baseline ({args.cpu_walkers} walkers, oracle)", file=sys.stderr, flush=True)
-        rate, secs = cpu_baseline(sys_np, args.cpu_walkers)
+        rate, secs = cpu_baseline(sys_np, args.cpu_walkers, args.cpu_steps)
         cpu = {"value": round(rate, 4), "unit": "walker-steps/s", "cores": ncpu, "kind": "port",
-               "sample": f"NumPy FP64 oracle, {args.cpu_walkers} walkers x 1 step of the same "
-                         f"M=300 system ({secs:.1f} s), BLAS threads={ncpu}, {model}"}
+               "sample": f"NumPy FP64 oracle, {args.cpu_walkers} walkers x {args.cpu_steps} steps "
+                         f"of the same M=300 system ({secs:.1f} s timed after one warm-up step), "
+                         f"BLAS threads={ncpu}, {model}"}
 
     flop_ws = step_flops(M, Na, Nb, G)
     out = {
