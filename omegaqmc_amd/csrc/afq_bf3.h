@@ -12,12 +12,13 @@ struct Bf3System {
   int Gp;        // G padded to a multiple of 32
   int nblk;      // upper-triangle 16x16 blocks of an M x M matrix (RT(RT+1)/2)
   int Kp;        // 16*RT padded to a multiple of 32 (K of the propagator products)
-  int Ka, Kb;    // N_s * M padded to multiples of 32 (K of the force bias)
+  int ldq;       // M padded to a multiple of 8: row stride of (i, q) in Lhalf / Ghalf planes
+  int Ka, Kb;    // N_s * ldq padded to multiples of 32 (K of the force bias)
   // L_g block-packed: column c = blk*256 + i*16 + j  <->  (16P+i, 16Q+j), P <= Q;
   // stored column-major in g: Lb[plane][c][Gp]
   bf16_t* Lb;
   int2* blk_pq;  // (P, Q) of every block
-  // Lhalf_s[g][i*M+q] as [plane][G][K_s]
+  // Lhalf_s[g][i*ldq+q] as [plane][G][K_s], zero in the padding
   bf16_t* Lha;
   bf16_t* Lhb;
   // expH1 as [plane][16*RT][Kp]
@@ -37,6 +38,14 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
 // phi <- expH1 . Taylor(VHS) . expH1 . phi in place, FP64 walkers.
 int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, double* phia,
                   double* phib, int nmax);
+// Exchange energy e_x[w] = 1/2 sum_s sum_g sum_ij T_ij T_ji, T = Lhalf_s[g] Ghalf_s^T,
+// from the split-bf16 Ghalf planes (N_s <= 40); part: bf3_exchange_part_bytes.
+bool bf3_exchange_supported(const Bf3System& b);
+size_t bf3_exchange_part_bytes(const Bf3System& b, int W);
+int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, const bf16_t* ghb,
+                 double* part, double* ex);
+// Coulomb energy 1/2 sum_g X_g^2 from the force-bias partials part[nsplit][W][G].
+int bf3_coulomb(hipStream_t s, int W, int G, int nsplit, const float* part, double* ec);
 size_t bf3_vhs_bytes(const Bf3System& b, int W);
 size_t bf3_x_bytes(const Bf3System& b, int W);
 size_t bf3_gh_bytes(const Bf3System& b, int W);

@@ -86,6 +86,21 @@ __global__ void k_prep_mat(long n, int rows, int cols, int cols_p, long plane,
   }
 }
 
+// Lhalf (G, N, M) -> [plane][G][K] with element (g, i, q) at g K + i ldq + q
+__global__ void k_prep_lh(long n, int N, int M, int ldq, long K, long plane,
+                          const double* __restrict__ R, bf16_t* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long g = e / K;
+    const int c = (int)(e % K);
+    const int i = c / ldq, q = c % ldq;
+    const double v = (i < N && q < M) ? R[(g * N + i) * M + q] : 0.0;
+    bf16_t h, l;
+    split2(v, h, l);
+    out[e] = h;
+    out[plane + e] = l;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // the GEMM core: 256 x 256 tile, K step 32, 8 waves of 64 x 128
 // ----------------------------------------------------------------------------
@@ -458,8 +473,9 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
   b->Gp = (G + 31) / 32 * 32;
   b->nblk = b->RT * (b->RT + 1) / 2;
   b->Kp = (16 * b->RT + 31) / 32 * 32;
-  b->Ka = (s->Na * M + 31) / 32 * 32;
-  b->Kb = (s->Nb * M + 31) / 32 * 32;
+  b->ldq = (M + 7) / 8 * 8;
+  b->Ka = (s->Na * b->ldq + 31) / 32 * 32;
+  b->Kb = (s->Nb * b->ldq + 31) / 32 * 32;
   b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = nullptr; b->Lhb = nullptr; b->E = nullptr;
   if (b->RT > 32) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: M=%d > 512", M);
   const long NC = (long)b->nblk * 256;
@@ -490,14 +506,14 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
   }
   if (G > 0 && s->Na > 0) {
     const long n = (long)G * b->Ka;
-    hipLaunchKernelGGL(k_prep_mat, dim3(4096), dim3(256), 0, st, n, G, s->Na * M, b->Ka, n,
-                       s->rchol_a, b->Lha);
+    hipLaunchKernelGGL(k_prep_lh, dim3(4096), dim3(256), 0, st, n, s->Na, M, b->ldq, (long)b->Ka,
+                       n, s->rchol_a, b->Lha);
     AFQ_CHECK_LAUNCH();
   }
   if (G > 0 && s->Nb > 0) {
     const long n = (long)G * b->Kb;
-    hipLaunchKernelGGL(k_prep_mat, dim3(4096), dim3(256), 0, st, n, G, s->Nb * M, b->Kb, n,
-                       s->rchol_b, b->Lhb);
+    hipLaunchKernelGGL(k_prep_lh, dim3(4096), dim3(256), 0, st, n, s->Nb, M, b->ldq, (long)b->Kb,
+                       n, s->rchol_b, b->Lhb);
     AFQ_CHECK_LAUNCH();
   }
   AFQ_CHECK_HIP(hipStreamSynchronize(st));
@@ -597,6 +613,359 @@ int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, do
   else if (b.RT <= 5 * PW) AFQ_PROP_BF3(5);
   else AFQ_PROP_BF3(8);
 #undef AFQ_PROP_BF3
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+
+// ----------------------------------------------------------------------------
+// exchange energy (energy.py:51-91): for every (g, walker) of one spin
+//   T = Lhalf_g . Ghalf^T  (N x N complex),   E_x += sum_ij T_ij T_ji
+// as one GEMM with rows (g, i) of Lhalf and columns (walker, re/im, j) of
+// Ghalf, K = q.  A wave owns an 80 x 80 tile holding ng = 80/N whole g blocks
+// and nw = 80/(2N) whole walkers, so every N x N block is complete in its
+// accumulators; the contraction T_ij T_ji goes through a wave-private LDS
+// block.  Sums per (row tile, walker) are written out and reduced in a fixed
+// order by k_exch_reduce (no atomics: the result is deterministic).
+// ----------------------------------------------------------------------------
+namespace {
+
+constexpr int XNW = 4, XNT = 64 * XNW, XWT = 80, XRT = 2 * XWT, XCT = 2 * XWT, XGR = 8;
+constexpr int XMAXN = XWT / 2;
+constexpr int XSLOT = (2 * XRT) * 4 / XNT;  // 16-byte staging slots per thread and operand
+constexpr int XEPI = XWT * (XMAXN + 1);     // floats of one wave's epilogue block
+
+struct Bf3Exch {
+  const bf16_t* a;  // Lhalf hi plane (lo plane at +aplane): row (g, i) at g ka + i ldq
+  const bf16_t* b;  // Ghalf hi plane (lo plane at +bplane): row (w, c, j) at (2 w + c) kb + j ldq
+  int aplane, bplane;
+  int ka, kb;
+  int N, G, W, ldq, Mp;
+  int ng, nw, ntr, ntc;
+  double* part;  // [ntr][W] complex
+};
+
+__device__ __forceinline__ void xwave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup tile: 160 rows (g, i) x 160 columns (walker, re/im, j), 4 waves of
+// 80 x 80 in a 2 x 2 grid, K step 32; two workgroups share a CU so that one
+// computes while the other waits on its loads.
+__global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
+  extern __shared__ __align__(16) bf16_t lds[];
+  const int t = threadIdx.x, l = t & 63, wv = t >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+  const int li = l & 15, lq = l >> 4;
+  // each XCD walks a contiguous range of tiles; inside it XGR row tiles share
+  // every column tile (row tile fastest), so the L2 holds both operands
+  const int v = xcd_remap(blockIdx.x, gridDim.x);
+  const int grp = v / (XGR * a.ntc), in = v % (XGR * a.ntc);
+  const int tr = grp * XGR + in % XGR, tc = in / XGR;
+  if (tr >= a.ntr) return;  // whole workgroup: before any barrier
+  const int N = a.N, NR = a.ng * N, NC = a.nw * 2 * N;
+  const int kq = t & 3;
+
+  // staging slots: operand rows 0..2*XRT-1 = (plane, row); slot i covers
+  // row (t >> 2) + 64 i, K chunk kq
+  int oa[XSLOT], ob[XSLOT];
+  bool va[XSLOT], vb[XSLOT];
+#pragma unroll
+  for (int i = 0; i < XSLOT; ++i) {
+    const int rp = (t >> 2) + 64 * i, p = rp >= XRT, r = rp - p * XRT;
+    {
+      const int wrr = r / XWT, rr = r % XWT;
+      const int g = (tr * 2 + wrr) * a.ng + rr / N, ii = rr % N;
+      va[i] = rr < NR && g < a.G;
+      oa[i] = va[i] ? p * a.aplane + g * a.ka + ii * a.ldq + 8 * kq : 0;
+    }
+    {
+      const int wcc = r / XWT, cc = r % XWT;
+      const int w = (tc * 2 + wcc) * a.nw + cc / (2 * N);
+      const int rem = cc % (2 * N), ri = rem / N, j = rem % N;
+      vb[i] = cc < NC && w < a.W;
+      ob[i] = vb[i] ? p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * kq : 0;
+    }
+  }
+  uint4 ra[XSLOT], rb[XSLOT];
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) const v4u gv4u;
+  const gv4u* ga = (const gv4u*)(a.a);
+  const gv4u* gb = (const gv4u*)(a.b);
+  auto load = [&](int k0) {
+    const bool kin = k0 + 8 * kq < a.ldq;
+#pragma unroll
+    for (int i = 0; i < XSLOT; ++i) {
+      // always load (at offset 0 when the slot is empty), then mask: with no
+      // pointer select the loads stay global and nothing goes to scratch
+      const bool ua = va[i] && kin, ub = vb[i] && kin;
+      const v4u x = ga[(ua ? oa[i] + k0 : 0) >> 3];
+      const v4u y = gb[(ub ? ob[i] + k0 : 0) >> 3];
+      const unsigned ma = ua ? ~0u : 0u, mb = ub ? ~0u : 0u;
+      ra[i] = make_uint4(x.x & ma, x.y & ma, x.z & ma, x.w & ma);
+      rb[i] = make_uint4(y.x & mb, y.y & mb, y.z & mb, y.w & mb);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < XSLOT; ++i) {
+      const int rp = (t >> 2) + 64 * i;
+      *(uint4*)(lds + rp * LROW + 8 * kq) = ra[i];
+      *(uint4*)(lds + (2 * XRT + rp) * LROW + 8 * kq) = rb[i];
+    }
+  };
+
+  f4v acc[5][5];
+#pragma unroll
+  for (int u = 0; u < 5; ++u)
+#pragma unroll
+    for (int w = 0; w < 5; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = a.Mp / GK;
+  load(0);
+  store();
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + 1 < nk) load((kt + 1) * GK);
+    uint4 ah[5], al[5];
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int r = XWT * wr + 16 * u + li;
+      ah[u] = *(const uint4*)(lds + r * LROW + 8 * lq);
+      al[u] = *(const uint4*)(lds + (XRT + r) * LROW + 8 * lq);
+    }
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+      const int c = XWT * wc + 16 * w + li;
+      const uint4 bh = *(const uint4*)(lds + (2 * XRT + c) * LROW + 8 * lq);
+      const uint4 bl = *(const uint4*)(lds + (3 * XRT + c) * LROW + 8 * lq);
+#pragma unroll
+      for (int u = 0; u < 5; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+
+  // Epilogue.  The lane holding T_c[i][j] (c = re/im) of walker wi writes it
+  // transposed, Rt[wi][c][j][i]; after a wave barrier it reads T_re[j][i] and
+  // T_im[j][i] at the same place and accumulates its share of sum_ij T_ij T_ji
+  // per column (products in FP64: the terms cancel strongly).  Column sums
+  // are then reduced per walker.
+  const int NP = N + 1;
+  float* Rt = (float*)lds + wv * XEPI;
+  int wb[5], rb0[5], rb1[5];
+  bool cv[5], cim_[5];
+#pragma unroll
+  for (int w = 0; w < 5; ++w) {
+    const int col = 16 * w + li;
+    cv[w] = col < NC;
+    const int wi = cv[w] ? col / (2 * N) : 0;
+    const int rem = col - wi * 2 * N;
+    cim_[w] = rem >= N;
+    const int j = rem - (cim_[w] ? N : 0);
+    wb[w] = ((wi * 2 + (cim_[w] ? 1 : 0)) * N + j) * NP;
+    rb0[w] = (wi * 2) * N * NP + j;
+    rb1[w] = (wi * 2 + 1) * N * NP + j;
+  }
+  double cre[5], cimg[5];
+#pragma unroll
+  for (int w = 0; w < 5; ++w) cre[w] = cimg[w] = 0.0;
+  const int gw0 = (tr * 2 + wr) * a.ng;
+  for (int gi = 0; gi < a.ng; ++gi) {
+    if (gw0 + gi >= a.G) break;  // wave-uniform
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * u + 4 * lq + r - gi * N;
+        if (i >= 0 && i < N) {
+#pragma unroll
+          for (int w = 0; w < 5; ++w)
+            if (cv[w]) Rt[wb[w] + i] = acc[u][w][r];
+        }
+      }
+    xwave_sync();
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * u + 4 * lq + r - gi * N;
+        if (i >= 0 && i < N) {
+          const int iNP = i * NP;
+#pragma unroll
+          for (int w = 0; w < 5; ++w) {
+            if (!cv[w]) continue;
+            const double x = acc[u][w][r];
+            const double tre = Rt[rb0[w] + iNP];  // T_re[j][i]
+            const double tim = Rt[rb1[w] + iNP];  // T_im[j][i]
+            const double p = cim_[w] ? -tim : tre;
+            const double q = cim_[w] ? tre : tim;
+            cre[w] += x * p;
+            cimg[w] += x * q;
+          }
+        }
+      }
+    xwave_sync();
+  }
+  // column sums over the four row groups of the lane layout, then per walker
+  double* cs = (double*)((float*)lds + XNW * XEPI) + wv * 2 * XWT;  // [80][2]
+#pragma unroll
+  for (int w = 0; w < 5; ++w) {
+    double x = cre[w], y = cimg[w];
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 32);
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    if (lq == 0) {
+      cs[2 * (16 * w + li)] = x;
+      cs[2 * (16 * w + li) + 1] = y;
+    }
+  }
+  xwave_sync();
+  double* wsum = (double*)((float*)lds + XNW * XEPI) + XNW * 2 * XWT;  // [XNW][nw][2]
+  if (l < a.nw) {
+    double x = 0.0, y = 0.0;
+    for (int c = 2 * N * l; c < 2 * N * (l + 1); ++c) {
+      x += cs[2 * c];
+      y += cs[2 * c + 1];
+    }
+    wsum[(wv * a.nw + l) * 2] = x;
+    wsum[(wv * a.nw + l) * 2 + 1] = y;
+  }
+  __syncthreads();
+  // the two row waves of a column hold disjoint g ranges of the same walkers
+  if (t < 2 * a.nw) {
+    const int wcc = t / a.nw, wi = t % a.nw;
+    const int w = (tc * 2 + wcc) * a.nw + wi;
+    if (w < a.W) {
+      const double re = wsum[((wcc * a.nw) + wi) * 2] + wsum[(((2 + wcc) * a.nw) + wi) * 2];
+      const double im = wsum[((wcc * a.nw) + wi) * 2 + 1] + wsum[(((2 + wcc) * a.nw) + wi) * 2 + 1];
+      a.part[((long)tr * a.W + w) * 2] = re;
+      a.part[((long)tr * a.W + w) * 2 + 1] = im;
+    }
+  }
+}
+
+// e_x[w] = 1/2 sum_s sum_tr part_s[tr][w], in a fixed order
+__global__ void k_exch_reduce(int W, int ntra, const double* __restrict__ pa, int ntrb,
+                              const double* __restrict__ pb, double* __restrict__ ex) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double re = 0.0, im = 0.0;
+  for (int r = 0; r < ntra; ++r) {
+    re += pa[((long)r * W + w) * 2];
+    im += pa[((long)r * W + w) * 2 + 1];
+  }
+  for (int r = 0; r < ntrb; ++r) {
+    re += pb[((long)r * W + w) * 2];
+    im += pb[((long)r * W + w) * 2 + 1];
+  }
+  ex[2 * w] = 0.5 * re;
+  ex[2 * w + 1] = 0.5 * im;
+}
+
+// e_c[w] = 1/2 sum_g X_g^2 with X_g = sum of the split-K force-bias partials
+__global__ __launch_bounds__(256) void k_coul_parts(int W, int G, int nsplit,
+                                                    const float2* __restrict__ part,
+                                                    double* __restrict__ ec) {
+  __shared__ double red[32];
+  const int w = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int g = threadIdx.x; g < G; g += 256) {
+    double xr = 0.0, xi = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) {
+      const float2 p = part[((long)sp * W + w) * G + g];
+      xr += p.x;
+      xi += p.y;
+    }
+    re += xr * xr - xi * xi;
+    im += 2.0 * xr * xi;
+  }
+  const cplx c = block_sum_c(mkc(re, im), red);
+  if (threadIdx.x == 0) {
+    ec[2 * w] = 0.5 * c.x;
+    ec[2 * w + 1] = 0.5 * c.y;
+  }
+}
+
+void exch_geom(int N, int G, int W, int* ng, int* nw, int* ntr, int* ntc) {
+  *ng = N > 0 ? XWT / N : 1;
+  *nw = N > 0 ? std::max(1, XWT / (2 * N)) : 1;
+  *ntr = cdiv_int(G, 2 * *ng);
+  *ntc = cdiv_int(W, 2 * *nw);
+}
+
+size_t exch_lds_bytes() {
+  const size_t stage = (size_t)(4 * XRT) * LROW * sizeof(bf16_t);
+  const size_t epi = (size_t)XNW * XEPI * sizeof(float) + (size_t)XNW * 2 * XWT * sizeof(double) +
+                     (size_t)XNW * XWT * 2 * sizeof(double);
+  return std::max(stage, epi);
+}
+
+}  // namespace
+
+bool bf3_exchange_supported(const Bf3System& b) { return b.Na <= XMAXN && b.Nb <= XMAXN; }
+
+size_t bf3_exchange_part_bytes(const Bf3System& b, int W) {
+  int ng, nw, ntra = 0, ntrb = 0, ntc;
+  if (b.Na) exch_geom(b.Na, b.G, W, &ng, &nw, &ntra, &ntc);
+  if (b.Nb) exch_geom(b.Nb, b.G, W, &ng, &nw, &ntrb, &ntc);
+  return (size_t)(ntra + ntrb) * W * 2 * sizeof(double);
+}
+
+int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, const bf16_t* ghb,
+                 double* part, double* ex) {
+  if (W <= 0) return AFQ_OK;
+  if (!bf3_exchange_supported(b))
+    return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: nocc %d/%d > %d", b.Na, b.Nb, XMAXN);
+  const size_t sm = exch_lds_bytes();
+  if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: LDS %zu", sm);
+  const int Ns[2] = {b.Na, b.Nb};
+  const long Ks[2] = {b.Ka, b.Kb};
+  const bf16_t* gh[2] = {gha, ghb};
+  const bf16_t* lh[2] = {b.Lha, b.Lhb};
+  int ntrs[2] = {0, 0};
+  double* parts[2] = {part, part};
+  for (int sp = 0; sp < 2; ++sp) {
+    if (Ns[sp] == 0 || b.G == 0) continue;
+    Bf3Exch a{};
+    const long aplane = (long)b.G * Ks[sp], bplane = 2L * W * Ks[sp];
+    if (2 * aplane >= (1L << 31) || 2 * bplane >= (1L << 31))
+      return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: operand planes exceed 2^31 elements");
+    a.a = lh[sp];
+    a.b = gh[sp];
+    a.aplane = (int)aplane;
+    a.bplane = (int)bplane;
+    a.ka = (int)Ks[sp];
+    a.kb = (int)Ks[sp];
+    a.N = Ns[sp];
+    a.G = b.G;
+    a.W = W;
+    a.ldq = b.ldq;
+    a.Mp = (b.ldq + GK - 1) / GK * GK;
+    exch_geom(a.N, b.G, W, &a.ng, &a.nw, &a.ntr, &a.ntc);
+    a.part = part + (sp == 1 ? (size_t)ntrs[0] * W * 2 : 0);
+    parts[sp] = a.part;
+    ntrs[sp] = a.ntr;
+    const long nb = (long)cdiv_int(a.ntr, XGR) * XGR * a.ntc;
+    if (nb > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: grid too large");
+    hipLaunchKernelGGL(k_exch_bf3, dim3((unsigned)nb), dim3(XNT), sm, s, a);
+    AFQ_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, ntrs[0], parts[0],
+                     ntrs[1], parts[1], ex);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int bf3_coulomb(hipStream_t s, int W, int G, int nsplit, const float* part, double* ec) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_coul_parts, dim3(W), dim3(256), 0, s, W, G, nsplit, (const float2*)part, ec);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

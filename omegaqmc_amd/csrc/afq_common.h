@@ -57,12 +57,14 @@ __device__ __forceinline__ void split2(double x, bf16_t& h, bf16_t& l) {
   l = bf_bits((float)(x - (double)bf_val(h)));
 }
 
-// Optional split-bf16 output of the half-rotated Green's function:
-// per spin, planes [2][2W][K_s] with rows (walker, re/im), k = i*M + q.
+// Split-bf16 copies of Ghalf written by the Green's-function kernel: plane
+// hi at a, lo at a + 2 W Ka; row (w, re/im) at (2 w + c) Ka; element (i, q)
+// at i * ld + q (ld >= M; the padding q in [M, ld) is never written).
 struct GhalfBf16 {
   bf16_t* a;
   bf16_t* b;
   long Ka, Kb;
+  int ld;
 };
 
 // NaN-propagating max/min (jnp.maximum / jnp.minimum semantics).

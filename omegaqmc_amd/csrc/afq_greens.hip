@@ -119,7 +119,7 @@ __device__ void wave_inverse(const cplx* A, int LD, const int* perm, int N, cplx
 // Ghalf (N x M) = Inv (N x N) . phi^T on MFMA; optional FP64 and split-bf16 outputs.
 __device__ void ghalf_mfma(const cplx* Inv, int LD, int N, int M, const cplx* __restrict__ phi,
                            cplx* __restrict__ Ghalf, bf16_t* __restrict__ gb, long Ks,
-                           long gplane) {
+                           long gplane, int ldq) {
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = l & 15, lq = l >> 4;
   const int RTn = (N + 15) >> 4, RTm = (M + 15) >> 4;
@@ -147,6 +147,7 @@ __device__ void ghalf_mfma(const cplx* Inv, int LD, int N, int M, const cplx* __
       const long k = (long)i * M + q;
       if (Ghalf) Ghalf[k] = mkc(cr[r], ci[r]);
       if (gb) {
+        const long k = (long)i * ldq + q;
         bf16_t h, lo;
         split2(cr[r], h, lo);
         gb[k] = h;
@@ -168,7 +169,7 @@ template <bool GENERAL>
 __device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
                             const cplx* __restrict__ phi, const cplx* __restrict__ trial,
                             cplx* __restrict__ Ghalf, cplx* ov_out, char* smem, bf16_t* gb,
-                            long Ks, long gplane) {
+                            long Ks, long gplane, int ldq) {
   const int t = threadIdx.x;
   if (N == 0) {
     if (t == 0) *ov_out = mkc(1.0, 0.0);
@@ -222,7 +223,7 @@ __device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
   if (mode == 0) return;
   if (t < 64) wave_inverse(LU, LD, perm, N, Inv);
   __syncthreads();
-  ghalf_mfma(Inv, LD, N, M, phi, Ghalf, gb, Ks, gplane);
+  ghalf_mfma(Inv, LD, N, M, phi, Ghalf, gb, Ks, gplane, ldq);
   __syncthreads();
 }
 
@@ -244,11 +245,11 @@ __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, 
   bf16_t* gbb = (gh && gbf.b) ? gbf.b + 2 * w * gbf.Kb : nullptr;
   greens_spin<GENERAL>(mode & 1, (mode >> 1) & 1, M, Na, qch, phia + w * M * Na, tu,
               (gh && Gha) ? Gha + w * Na * M : nullptr, &ov[0], smem, ga, gbf.Ka,
-              2 * W * gbf.Ka);
+              2 * W * gbf.Ka, gbf.ld);
   __syncthreads();
   greens_spin<GENERAL>(mode & 1, (mode >> 2) & 1, M, Nb, qch, phib + w * M * Nb, td,
               (gh && Ghb) ? Ghb + w * Nb * M : nullptr, &ov[1], smem, gbb, gbf.Kb,
-              2 * W * gbf.Kb);
+              2 * W * gbf.Kb, gbf.ld);
   __syncthreads();
   if (threadIdx.x == 0) ovlp[w] = cmul(ov[0], ov[1]);
 }
@@ -273,7 +274,12 @@ int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const d
     return fail(AFQ_EUNSUPPORTED, "greens: %zu bytes of LDS needed, device allows %zu", sm,
                 max_lds_bytes());
   GhalfBf16 g{};
-  if (gbf) g = *gbf;
+  if (gbf) {
+    g = *gbf;
+    if (g.ld < M) return fail(AFQ_EINVAL, "greens: bf16 row stride %d < M=%d", g.ld, M);
+    if ((g.a && g.Ka < (long)Na * g.ld) || (g.b && g.Kb < (long)Nb * g.ld))
+      return fail(AFQ_EINVAL, "greens: bf16 plane row too short");
+  }
   const bool eye_both = ((mode >> 1) & 1 || Na == 0) && ((mode >> 2) & 1 || Nb == 0);
   if (eye_both)
     hipLaunchKernelGGL((k_greens<false>), dim3(W), dim3(NT), sm, s, mode, M, Na, Nb,

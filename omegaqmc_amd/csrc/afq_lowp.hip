@@ -125,7 +125,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
     xi = xg;
   }
   // 1: Ghalf (split bf16 planes only) and ovlp_old, FP64 LU
-  GhalfBf16 gbf{Na ? gha : nullptr, Nb ? ghb : nullptr, b.Ka, b.Kb};
+  GhalfBf16 gbf{Na ? gha : nullptr, Nb ? ghb : nullptr, b.Ka, b.Kb, b.ldq};
   phase_begin(sys, PH_GREENS);
   rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
                      nullptr, nullptr, oo, &gbf);
@@ -170,10 +170,33 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   return rc;
 }
 
+// Block-end energy (afqmc_gto.py:1466-1473): Ghalf, overlap and E_1b in FP64;
+// the Coulomb X_g (force-bias GEMM) and the exchange T_g products in split
+// BF16 with FP32 accumulation, their squares and all sums over g in FP64.
 int system_energy_lowp(afq_system* sys, int W, const double* phia, const double* phib,
                        const double* h1e, double enuc, double* e_tot, double* e_1b, double* e_2b) {
-  // the block-end energy is evaluated in FP64
-  return system_energy_fp64(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
+  const Bf3System& b = *(const Bf3System*)sys->chol_lp;
+  if (!bf3_exchange_supported(b))
+    return system_energy_fp64(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
+  hipStream_t s = sys->ctx->stream;
+  LowpWork* L = nullptr;
+  int rc = ensure(sys, W, &L);
+  if (rc) return rc;
+  char* base = (char*)L->buf;
+  bf16_t* gha = (bf16_t*)(base + L->gh_a);
+  bf16_t* ghb = (bf16_t*)(base + L->gh_b);
+  float* part = (float*)(base + L->part);
+  GhalfBf16 gbf{sys->Na ? gha : nullptr, sys->Nb ? ghb : nullptr, b.Ka, b.Kb, b.ldq};
+  EnergySlots es;
+  if ((rc = energy_prologue(sys, W, phia, phib, h1e, enuc, &gbf, e_1b, &es))) return rc;
+  if (bf3_exchange_part_bytes(b, W) > es.part_bytes)
+    return fail(AFQ_EUNSUPPORTED, "exchange partials exceed the workspace");
+  int nsplit = 0;
+  if ((rc = bf3_force_bias(s, b, W, gha, ghb, part, &nsplit))) return rc;
+  if (nsplit > FB_SPLITS_MAX) return fail(AFQ_EUNSUPPORTED, "force-bias splits %d", nsplit);
+  if ((rc = bf3_coulomb(s, W, sys->G, nsplit, part, es.ec))) return rc;
+  if ((rc = bf3_exchange(s, b, W, gha, ghb, es.part, es.ex))) return rc;
+  return combine_launch(s, W, e_1b, es.ec, es.ex, e_tot, e_2b);
 }
 
 }  // namespace afq

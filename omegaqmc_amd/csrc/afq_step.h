@@ -10,6 +10,22 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
 int system_energy_fp64(afq_system* sys, int W, const double* phia, const double* phib,
                        const double* h1e, double enuc, double* e_tot, double* e_1b, double* e_2b);
 
+// FP64 workspace pieces of the block-end energy (inside the step workspace).
+struct EnergySlots {
+  double* gha;    // Ghalf_a (W, Na, M) complex
+  double* ghb;    // Ghalf_b (W, Nb, M) complex
+  double* X;      // (W, G) complex
+  double* ec;     // (W,) complex
+  double* ex;     // (W,) complex
+  double* part;   // W M M complex of scratch
+  size_t part_bytes;
+};
+// Ghalf (FP64, plus split-bf16 planes when gbf != NULL), the overlap, and
+// E_1b = Tr(h G) + E_nuc through h1e Psi* (energy.py:193-215).
+int energy_prologue(afq_system* sys, int W, const double* phia, const double* phib,
+                    const double* h1e, double enuc, const GhalfBf16* gbf, double* e_1b,
+                    EnergySlots* es);
+
 // reduced-precision path (afq_lowp.hip)
 int system_prepare_lowp(afq_system* sys);
 void system_release_lowp(afq_system* sys);

@@ -100,7 +100,7 @@ Layout layout(const afq_system* s, int W) {
   sz[GHB] = 16L * W * Nb * M;
   sz[OVLP_OLD] = 16L * W;
   sz[VBIAS] = 16L * W * G * std::max(1, s->fb_nsplit);
-  sz[XS] = 16L * W * G;
+  sz[XS] = 16L * std::max((long)W * G, M * (Na + Nb));  // also h1e Psi* of the energy
   sz[CFB] = 16L * W;
   sz[CMF] = 16L * W;
   sz[XI] = 8L * W * G;
@@ -292,23 +292,24 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   return rc;
 }
 
-int system_energy_fp64(afq_system* sys, int W, const double* phia, const double* phib,
-                       const double* h1e, double enuc, double* e_tot, double* e_1b, double* e_2b) {
+int energy_prologue(afq_system* sys, int W, const double* phia, const double* phib,
+                    const double* h1e, double enuc, const GhalfBf16* gbf, double* e_1b,
+                    EnergySlots* es) {
   hipStream_t s = sys->ctx->stream;
   const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G;
   int rc = ensure_work(sys, W);
   if (rc) return rc;
-  double* gha = work_slot(sys, W, GHA);
-  double* ghb = work_slot(sys, W, GHB);
+  es->gha = work_slot(sys, W, GHA);
+  es->ghb = work_slot(sys, W, GHB);
   double* oo = work_slot(sys, W, OVLP_OLD);
-  double* X = work_slot(sys, W, VBIAS);
-  double* hT = work_slot(sys, W, XS);  // (M, Na) + (M, Nb) complex: fits in W*G complex? see below
-  double* ec = work_slot(sys, W, CFB);
-  double* ex = work_slot(sys, W, CMF);
-  double* part = work_slot(sys, W, VHS);
-  if ((long)M * (Na + Nb) > (long)W * G) return fail(AFQ_EUNSUPPORTED, "energy workspace too small");
+  es->X = work_slot(sys, W, VBIAS);
+  double* hT = work_slot(sys, W, XS);  // (M, Na) + (M, Nb) complex
+  es->ec = work_slot(sys, W, CFB);
+  es->ex = work_slot(sys, W, CMF);
+  es->part = work_slot(sys, W, VHS);
+  es->part_bytes = (size_t)W * M * M * 16;
   rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
-                     gha, ghb, oo);
+                     es->gha, es->ghb, oo, gbf);
   if (rc) return rc;
   // hT_s = h1e @ conj(trial_s)  (M x N_s)
   double* hTa = hT;
@@ -325,16 +326,28 @@ int system_energy_fp64(afq_system* sys, int W, const double* phia, const double*
     g.C = (cplx*)hts[sp]; g.ldc = Ns[sp];
     if ((rc = gemm_fp64(s, g))) return rc;
   }
-  if ((rc = e1b_half_launch(s, W, M, Na, Nb, hTa, hTb, gha, ghb, enuc, e_1b))) return rc;
-  if ((rc = coulomb_launch(s, W, M, Na, Nb, G, sys->rchol_a, sys->rchol_b, false, gha, ghb, X, ec)))
+  return e1b_half_launch(s, W, M, Na, Nb, hTa, hTb, es->gha, es->ghb, enuc, e_1b);
+}
+
+int system_energy_fp64(afq_system* sys, int W, const double* phia, const double* phib,
+                       const double* h1e, double enuc, double* e_tot, double* e_1b, double* e_2b) {
+  hipStream_t s = sys->ctx->stream;
+  const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G;
+  EnergySlots es;
+  int rc = energy_prologue(sys, W, phia, phib, h1e, enuc, nullptr, e_1b, &es);
+  if (rc) return rc;
+  if ((rc = coulomb_launch(s, W, M, Na, Nb, G, sys->rchol_a, sys->rchol_b, false, es.gha, es.ghb,
+                           es.X, es.ec)))
     return rc;
   // T slab in the VHS region (W*M*M complex): gchunk with gchunk*N*W*N <= W*M*M
   const int N = std::max(Na, Nb);
   int gc = N > 0 ? std::max(1, (M * M) / (N * N)) : G;
   gc = std::min(gc, std::max(G, 1));
-  if ((rc = exchange_fp64(s, W, M, Na, G, gha, sys->rchol_a, false, gc, part, ex, false))) return rc;
-  if ((rc = exchange_fp64(s, W, M, Nb, G, ghb, sys->rchol_b, false, gc, part, ex, true))) return rc;
-  return combine_launch(s, W, e_1b, ec, ex, e_tot, e_2b);
+  if ((rc = exchange_fp64(s, W, M, Na, G, es.gha, sys->rchol_a, false, gc, es.part, es.ex, false)))
+    return rc;
+  if ((rc = exchange_fp64(s, W, M, Nb, G, es.ghb, sys->rchol_b, false, gc, es.part, es.ex, true)))
+    return rc;
+  return combine_launch(s, W, e_1b, es.ec, es.ex, e_tot, e_2b);
 }
 
 }  // namespace afq

@@ -37,3 +37,36 @@ def rel_err(a, b):
     b = np.asarray(b)
     den = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / den)
+
+
+def bf16_round(x):
+    """Round float64 -> float32 -> bfloat16 (nearest even), as the GPU's
+    split does (v_cvt to f32, then to bf16); returned as float64."""
+    x32 = np.asarray(x, dtype=np.float32)
+    b = x32.view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def bf16x3_split(x):
+    """hi = bf16(x), lo = bf16(x - hi)."""
+    hi = bf16_round(x)
+    return hi, bf16_round(np.asarray(x) - hi)
+
+
+def bf16x3_two_body(Ghalfa, Ghalfb, rchol_a, rchol_b):
+    """(e_coul, e_exch) of energy.py:51-91 with every product a*b replaced by
+    ah*bh + ah*bl + al*bh of the split operands, summed in float64: the error
+    the split-BF16 kernels are designed to have (without FP32 accumulation)."""
+    def prod(sub, L, G):
+        Lh, Ll = bf16x3_split(L)
+        Gh, Gl = bf16x3_split(G)
+        f = lambda a, b: np.einsum(sub, a, b)  # noqa: E731
+        return f(Lh, Gh) + f(Lh, Gl) + f(Ll, Gh)
+    e_exch = 0.0
+    X = 0.0
+    for L, Gh in ((rchol_a.real, Ghalfa), (rchol_b.real, Ghalfb)):
+        T = prod('giq,wjq->gwij', L, Gh.real) + 1j * prod('giq,wjq->gwij', L, Gh.imag)
+        e_exch = e_exch + 0.5 * np.sum(T * T.transpose(0, 1, 3, 2), axis=(0, 2, 3))
+        X = X + prod('giq,wiq->gw', L, Gh.real) + 1j * prod('giq,wiq->gw', L, Gh.imag)
+    return 0.5 * np.sum(X ** 2, axis=0), e_exch

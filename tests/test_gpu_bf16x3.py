@@ -9,7 +9,7 @@ short trajectory must stay within the 0.5 mHa budget of the FP64 run.
 import numpy as np
 import pytest
 
-from afq_cases import rel_err
+from afq_cases import bf16x3_two_body, rel_err
 from oracle import afqmc as O
 from oracle import philox as P
 
@@ -82,3 +82,36 @@ def test_trajectory_energy_within_budget(afq):
     e = {"fp64": float(s64['stats'][0].item()), "bf16x3": float(sbf['stats'][0].item())}
     assert np.isfinite(e["fp64"])
     assert abs(e["bf16x3"] - e["fp64"]) < 0.5e-3, e
+
+
+@pytest.mark.parametrize("M,Na,Nb,G,W", [(40, 5, 4, 120, 64), (100, 40, 38, 130, 37),
+                                         (60, 16, 1, 90, 21), (64, 40, 40, 200, 130)])
+def test_block_energy_close_to_oracle(afq, M, Na, Nb, G, W):
+    """Mixed-estimator local energy with the split-BF16 Coulomb and exchange
+    contractions (ragged row/walker tiles, N = 1 .. 40) against the FP64
+    oracle.  E_2b = E_coul - E_exch cancels two large terms, so the bound is
+    per walker relative to |E_coul| + |E_exch|: 3e-5 (BF16x3 products carry
+    ~2^-17 relative error, FP32 partial sums over K = N M add ~1e-6; these
+    walkers are far from the trial, |Ghalf| up to 35).  The
+    error itself must be the split's own: a float64 emulation of the same
+    hi/lo products lands within a third of it (plus 2e-7 of the scale for
+    the FP32 accumulation)."""
+    s, d64, dbf = _drivers(M, Na, Nb, G)
+    od = O.Driver(s.h1e, s.h1e_mod, s.chol, s.enuc, Na, Nb)
+    rng = np.random.default_rng(M + W)
+    pa = od.trial_up + 0.2 * (rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na)))
+    pb = od.trial_dn + 0.2 * (rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb)))
+    Ga, Gb, Gha, Ghb, _ = O.greens_function(pa, pb, od.trial_up.astype(complex),
+                                            od.trial_dn.astype(complex))
+    rt, r1, r2 = O.local_energy(s.h1e, Ga, Gb, Gha, Ghb, od.rchol_a, od.rchol_b, s.enuc)
+    et, e1, e2 = dbf.system.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc)
+    ec, ex = O.local_energy_2body(Gha, Ghb, od.rchol_a, od.rchol_b)
+    scale = np.abs(ec) + np.abs(ex)
+    assert rel_err(_n(e1), r1) < 1e-12
+    assert np.all(np.abs(_n(e2) - r2) < 3e-5 * scale)
+    assert np.all(np.abs(_n(et) - rt) < 3e-5 * scale + 1e-12 * np.abs(r1))
+    mc, mx = bf16x3_two_body(Gha, Ghb, od.rchol_a, od.rchol_b)
+    emu = mc - mx
+    assert np.all(np.abs(_n(e2) - emu) < 0.34 * np.abs(emu - r2) + 2e-7 * scale)
+    ft, _, f2 = d64.system.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc)
+    assert rel_err(_n(f2), r2) < 1e-11
