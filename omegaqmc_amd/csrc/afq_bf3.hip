@@ -139,25 +139,42 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   const int row0 = tm * GB, col0 = tn * GB;
   const Bf3Split& S = a.sp[blockIdx.y];
 
-  auto Arow = [&](int p, int r) { return lds + (p * GB + r) * LROW; };
-  auto Brow = [&](int p, int c) { return lds + ((2 + p) * GB + c) * LROW; };
+  // two stage buffers: the next K step is stored while this one is consumed
+  constexpr int GSTAGE = 4 * GB * LROW;
+  auto Arow = [&](int buf, int p, int r) { return lds + buf * GSTAGE + (p * GB + r) * LROW; };
+  auto Brow = [&](int buf, int p, int c) { return lds + buf * GSTAGE + ((2 + p) * GB + c) * LROW; };
 
-  uint4 ra[4], rb[4];
+  // staging slots (fixed over K): slot i covers plane i / 2, row
+  // ((t + GT i) & 1023) / 4, K chunk t % 4.  Row pointers are formed once so
+  // that the K loop issues plain global loads back to back (no pointer loads
+  // or selects in between, which would serialise them).  A row or column
+  // outside the matrix reads row 0 instead: it only feeds output rows or
+  // columns that the epilogue discards.
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) const v4u gv4u;
+  const gv4u* pa[4];
+  const gv4u* pb[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int c = t + GT * i, r = (c & 1023) >> 2, kq = c & 3;
+    const int gr = row0 + r, gc = col0 + r;
+    pa[i] = (const gv4u*)(S.a[i >> 1] + (long)(gr < a.rows ? gr : 0) * S.lda + 8 * kq);
+    pb[i] = (const gv4u*)(S.b[i >> 1] + (long)(gc < a.cols ? gc : 0) * S.ldb + 8 * kq);
+  }
+  v4u ra[4], rb[4];
   auto load = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int c = t + GT * i, p = c >> 10, r = (c & 1023) >> 2, kq = c & 3;
-      const int gr = row0 + r, gc = col0 + r;
-      ra[i] = gr < a.rows ? *(const uint4*)(S.a[p] + gr * S.lda + k0 + 8 * kq) : make_uint4(0, 0, 0, 0);
-      rb[i] = gc < a.cols ? *(const uint4*)(S.b[p] + gc * S.ldb + k0 + 8 * kq) : make_uint4(0, 0, 0, 0);
+      ra[i] = pa[i][k0 >> 3];
+      rb[i] = pb[i][k0 >> 3];
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int c = t + GT * i, p = c >> 10, r = (c & 1023) >> 2, kq = c & 3;
-      *(uint4*)(Arow(p, r) + 8 * kq) = ra[i];
-      *(uint4*)(Brow(p, r) + 8 * kq) = rb[i];
+      const int c = t + GT * i, r = (c & 1023) >> 2, kq = c & 3;
+      *(v4u*)(Arow(buf, i >> 1, r) + 8 * kq) = ra[i];
+      *(v4u*)(Brow(buf, i >> 1, r) + 8 * kq) = rb[i];
     }
   };
 
@@ -170,31 +187,30 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   const int nk = (S.kend - S.kbeg) / GK;
   if (nk > 0) {
     load(S.kbeg);
-    store();
+    store(0);
   }
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     if (kt + 1 < nk) load(S.kbeg + (kt + 1) * GK);
+    const int buf = kt & 1;
     uint4 ah[4], al[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int r = 64 * wr + 16 * u + li;
-      ah[u] = *(const uint4*)(Arow(0, r) + 8 * lq);
-      al[u] = *(const uint4*)(Arow(1, r) + 8 * lq);
+      ah[u] = *(const uint4*)(Arow(buf, 0, r) + 8 * lq);
+      al[u] = *(const uint4*)(Arow(buf, 1, r) + 8 * lq);
     }
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
       const int c = 128 * wc + 16 * w + li;
-      const uint4 bh = *(const uint4*)(Brow(0, c) + 8 * lq);
-      const uint4 bl = *(const uint4*)(Brow(1, c) + 8 * lq);
+      const uint4 bh = *(const uint4*)(Brow(buf, 0, c) + 8 * lq);
+      const uint4 bl = *(const uint4*)(Brow(buf, 1, c) + 8 * lq);
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
     }
+    // the other buffer was last read before the previous barrier
+    if (kt + 1 < nk) store(buf ^ 1);
     __syncthreads();
-    if (kt + 1 < nk) {
-      store();
-      __syncthreads();
-    }
   }
 
   if (EPI == 0) {
@@ -312,18 +328,23 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 #pragma unroll
     for (int p = 0; p < 4; ++p) b[p] = *(const uint4*)(T(p, 16 * x + li) + 32 * ks + 8 * lq);
   };
-  // acc = E . T (E real, planes [2][16 RT][Kp])
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) const v4u gv4u;
+  auto gld = [](const bf16_t* p) {
+    const v4u x = *(const gv4u*)p;
+    return make_uint4(x.x, x.y, x.z, x.w);
+  };
+  // acc = E . T (E real, planes [2][16 RT][Kp]); row tiles this wave does not
+  // own read a valid row and are not used
   auto onebody = [&]() {
     zero();
     for (int ks = 0; ks < Kp / 32; ++ks) {
       uint4 eh[MT], el[MT];
 #pragma unroll
       for (int u = 0; u < MT; ++u) {
-        const long r = 16 * (wv + PW * u) + li;
-        if (owns(u)) {
-          eh[u] = *(const uint4*)(E + r * Kp + 32 * ks + 8 * lq);
-          el[u] = *(const uint4*)(E + Eplane + r * Kp + 32 * ks + 8 * lq);
-        }
+        const long r = 16 * min(wv + PW * u, RT - 1) + li;
+        eh[u] = gld(E + r * Kp + 32 * ks + 8 * lq);
+        el[u] = gld(E + Eplane + r * Kp + 32 * ks + 8 * lq);
       }
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
@@ -338,47 +359,61 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       }
     }
   };
-  // acc = V . T (V complex, block-major planes)
-  // VHS fragments of one row tile u at k-step ks (zeros outside the matrix)
+  // acc = V . T (V complex, block-major planes).  VHS fragments of row tile u
+  // at k-step ks; indices are clamped into the matrix: a clamped row tile is
+  // not owned, and a clamped column block Q = RT meets zero rows of T.
   auto ldv = [&](uint4* v, int u, int ks) {
-    const int Q = 2 * ks + (lq >> 1), j0 = 8 * (lq & 1);
-    const int P = wv + PW * u;
-    if (P < RT && Q < RT) {
-      const bf16_t* blk = Vw + ((long)P * RT + Q) * 1024 + li * 16 + j0;
+    const int Q = min(2 * ks + (lq >> 1), RT - 1), j0 = 8 * (lq & 1);
+    const int P = min(wv + PW * u, RT - 1);
+    const bf16_t* blk = Vw + ((long)P * RT + Q) * 1024 + li * 16 + j0;
 #pragma unroll
-      for (int p = 0; p < 4; ++p) v[p] = *(const uint4*)(blk + p * 256);
-    } else {
+    for (int p = 0; p < 4; ++p) v[p] = gld(blk + p * 256);
+  };
+  auto vstep = [&](const uint4* vc, uint4 (*b)[4], int u) {
+    if (owns(u)) {
+      const uint4 ni = neg8(vc[2]), nl = neg8(vc[3]);
 #pragma unroll
-      for (int p = 0; p < 4; ++p) v[p] = make_uint4(0, 0, 0, 0);
+      for (int x = 0; x < 3; ++x) {
+        are[u][x] = mfma3(vc[0], vc[1], b[x][0], b[x][1], are[u][x]);
+        are[u][x] = mfma3(ni, nl, b[x][2], b[x][3], are[u][x]);
+        aim[u][x] = mfma3(vc[0], vc[1], b[x][2], b[x][3], aim[u][x]);
+        aim[u][x] = mfma3(vc[2], vc[3], b[x][0], b[x][1], aim[u][x]);
+      }
     }
   };
-  // Row tiles are walked one at a time with the next tile's fragments (also
-  // across k-steps) in flight while the current one feeds the MFMAs.
+  // Row tiles are walked one at a time with the next tile's fragments in
+  // flight.  Two fragment buffers alternate by the parity of the sub-step
+  // (row tile, k-step); the k-steps go in pairs so that, with the row-tile
+  // loop unrolled, every buffer choice is static and no loaded register is
+  // copied (a copy would wait for the load).
   auto term = [&]() {
     zero();
     const int nks = Kp / 32;
-    uint4 vc[4], vn[4];
-    ldv(vc, 0, 0);
-    for (int ks = 0; ks < nks; ++ks) {
-      uint4 b[3][4];
+    uint4 v0[4], v1[4];
+    ldv(v0, 0, 0);
+    for (int ks = 0; ks < nks; ks += 2) {
 #pragma unroll
-      for (int x = 0; x < 3; ++x) bfrag(x, ks, b[x]);
+      for (int h = 0; h < 2; ++h) {
+        const int kk = ks + h;
+        const bool live = kk < nks;  // uniform
+        uint4 b[3][4];
+        if (live) {
 #pragma unroll
-      for (int u = 0; u < MT; ++u) {
-        if (u + 1 < MT) ldv(vn, u + 1, ks);
-        else if (ks + 1 < nks) ldv(vn, 0, ks + 1);
-        if (owns(u)) {
-          const uint4 ni = neg8(vc[2]), nl = neg8(vc[3]);
-#pragma unroll
-          for (int x = 0; x < 3; ++x) {
-            are[u][x] = mfma3(vc[0], vc[1], b[x][0], b[x][1], are[u][x]);
-            are[u][x] = mfma3(ni, nl, b[x][2], b[x][3], are[u][x]);
-            aim[u][x] = mfma3(vc[0], vc[1], b[x][2], b[x][3], aim[u][x]);
-            aim[u][x] = mfma3(vc[2], vc[3], b[x][0], b[x][1], aim[u][x]);
-          }
+          for (int x = 0; x < 3; ++x) bfrag(x, kk, b[x]);
         }
 #pragma unroll
-        for (int p = 0; p < 4; ++p) vc[p] = vn[p];
+        for (int u = 0; u < MT; ++u) {
+          const int sidx = h * MT + u;
+          const int un = (u + 1 < MT) ? u + 1 : 0;
+          const int kn = (u + 1 < MT) ? kk : kk + 1;
+          if ((sidx & 1) == 0) {
+            ldv(v1, un, kn < nks ? kn : nks - 1);
+            if (live) vstep(v0, b, u);
+          } else {
+            ldv(v0, un, kn < nks ? kn : nks - 1);
+            if (live) vstep(v1, b, u);
+          }
+        }
       }
     }
   };
@@ -529,10 +564,11 @@ void bf3_release(Bf3System* b) {
   b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = b->Lhb = nullptr; b->E = nullptr;
 }
 
-static size_t gemm_lds() { return (size_t)4 * GB * LROW * sizeof(bf16_t); }
+static size_t gemm_lds() { return (size_t)2 * 4 * GB * LROW * sizeof(bf16_t); }
 
 int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt, bf16_t* Vb) {
   if (W <= 0) return AFQ_OK;
+  if (gemm_lds() > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 GEMM: LDS %zu", gemm_lds());
   Bf3Gemm a{};
   a.nsplit = 1;
   a.sp[0].a[0] = X;
@@ -585,6 +621,7 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
   }
   *nsplit_out = n;
   if (W <= 0 || n == 0) return AFQ_OK;
+  if (gemm_lds() > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 GEMM: LDS %zu", gemm_lds());
   a.nsplit = n;
   a.rows = 2 * W;
   a.cols = b.G;
@@ -689,31 +726,31 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       ob[i] = vb[i] ? p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * kq : 0;
     }
   }
-  uint4 ra[XSLOT], rb[XSLOT];
   typedef unsigned v4u __attribute__((ext_vector_type(4)));
   typedef __attribute__((address_space(1))) const v4u gv4u;
   const gv4u* ga = (const gv4u*)(a.a);
   const gv4u* gb = (const gv4u*)(a.b);
+  // Plain global loads back to back; an empty slot (row outside the tile's
+  // g blocks or walkers) reads offset 0: it only feeds outputs the epilogue
+  // ignores.  K chunks at or beyond ldq do feed valid outputs and are zeroed
+  // when stored (mask applied at the store, so the loads stay in flight).
+  v4u ra[XSLOT], rb[XSLOT];
+  unsigned kmask = ~0u;
   auto load = [&](int k0) {
     const bool kin = k0 + 8 * kq < a.ldq;
+    kmask = kin ? ~0u : 0u;
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
-      // always load (at offset 0 when the slot is empty), then mask: with no
-      // pointer select the loads stay global and nothing goes to scratch
-      const bool ua = va[i] && kin, ub = vb[i] && kin;
-      const v4u x = ga[(ua ? oa[i] + k0 : 0) >> 3];
-      const v4u y = gb[(ub ? ob[i] + k0 : 0) >> 3];
-      const unsigned ma = ua ? ~0u : 0u, mb = ub ? ~0u : 0u;
-      ra[i] = make_uint4(x.x & ma, x.y & ma, x.z & ma, x.w & ma);
-      rb[i] = make_uint4(y.x & mb, y.y & mb, y.z & mb, y.w & mb);
+      ra[i] = ga[(va[i] && kin ? oa[i] + k0 : 0) >> 3];
+      rb[i] = gb[(vb[i] && kin ? ob[i] + k0 : 0) >> 3];
     }
   };
   auto store = [&]() {
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
       const int rp = (t >> 2) + 64 * i;
-      *(uint4*)(lds + rp * LROW + 8 * kq) = ra[i];
-      *(uint4*)(lds + (2 * XRT + rp) * LROW + 8 * kq) = rb[i];
+      *(v4u*)(lds + rp * LROW + 8 * kq) = ra[i] & kmask;
+      *(v4u*)(lds + (2 * XRT + rp) * LROW + 8 * kq) = rb[i] & kmask;
     }
   };
 

@@ -166,7 +166,7 @@ __device__ void ghalf_mfma(const cplx* Inv, int LD, int N, int M, const cplx* __
 // gb (optional): split-bf16 planes of Ghalf at this walker's real-part row;
 // the imaginary row follows at +Ks and the lo plane at +gplane.
 template <bool GENERAL>
-__device__ void greens_spin(int mode, int eye, int M, int N, int QCH,
+__device__ __forceinline__ void greens_spin(int mode, int eye, int M, int N, int QCH,
                             const cplx* __restrict__ phi, const cplx* __restrict__ trial,
                             cplx* __restrict__ Ghalf, cplx* ov_out, char* smem, bf16_t* gb,
                             long Ks, long gplane, int ldq) {
