@@ -200,6 +200,50 @@ int afq_aux_fields(afq_ctx* ctx, uint64_t seed, int64_t step, int nwalkers, int 
 /* Comb offset r in [0,1) for a step (host computation, no device work). */
 double afq_comb_offset(uint64_t seed, int64_t step);
 
+/* ---- multi-determinant trials ------------------------------------------ */
+/* The determinants are occupation strings, trial_s[I] = eye(M)[:, occ_s[I]]
+ * (the driver's construction, afqmc_gto.py:1163-1174): occ_up (ndet, nup) and
+ * occ_dn (ndet, ndown) are device int32 arrays of distinct orbital indices in
+ * [0, nbasis) (checked on the host: these calls synchronise), ci (ndet,)
+ * complex.  Dets whose |o_Ia o_Ib| is below max(1e-14 max_J |o_J|, 1e-300)
+ * are masked as in greens_th4.py:34-46; det chunks of det_chunk_size are
+ * summed in the order of the reference's scan.  N_s <= 40. */
+
+/* greens_th4.py:75-148 greens_function_multidet -> Ga, Gb (W,M,M), overlap (W,) complex */
+int afq_greens_function_multidet(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                 int ndet, const double* phia, const double* phib,
+                                 const int32_t* occ_up, const int32_t* occ_dn,
+                                 const double* ci_coeffs, int det_chunk_size,
+                                 double* Ga, double* Gb, double* overlap);
+
+/* greens_th4.py:186-228 greens_function_multidet_overlap_only -> overlap (W,) */
+int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int nwalkers, int nbasis, int nup,
+                                              int ndown, int ndet, const double* phia,
+                                              const double* phib, const int32_t* occ_up,
+                                              const int32_t* occ_dn, const double* ci_coeffs,
+                                              int det_chunk_size, double* overlap);
+
+/* energy_th4.py:26-119 local_energy_multidet: h1e (M,M) real, chol (G,M,M)
+ * real -> e_tot, e_1b, e_2b (W,) complex (e_2b may be NULL); FP64. */
+int afq_local_energy_multidet(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                              int naux, int ndet, const double* h1e, const double* chol,
+                              const double* phia, const double* phib, const int32_t* occ_up,
+                              const int32_t* occ_dn, const double* ci_coeffs, double enuc,
+                              int det_chunk_size, double* e_tot, double* e_1b, double* e_2b);
+
+/* System with a multi-determinant trial (afqmc_gto.py:1163-1225): the
+ * propagator (expH1, mf_shift from the |c|^2-weighted charge density) is the
+ * caller's; occ/ci are copied.  afq_propagate_walkers then runs
+ * propagate_walkers_multidet (afqmc_gto.py:728-839: force bias from the
+ * multi-det G, overlap_new from the multi-det overlap) and
+ * afq_system_local_energy runs local_energy_multidet (FP64).  precision
+ * AFQ_BF16X3 applies to the VHS build and the propagator only. */
+int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int nbasis, int nup, int ndown,
+                               int naux, const double* chol, const double* expH1,
+                               const double* mf_shift, int ndet, const int32_t* occ_up,
+                               const int32_t* occ_dn, const double* ci_coeffs, double dt,
+                               double fbbound, int exp_nmax, int det_chunk_size, int precision);
+
 /* ---- prepared system and the fused step -------------------------------- */
 
 /* Setup counterpart of build_propagator (afqmc_gto.py:574-627) + the half

@@ -11,7 +11,10 @@ from .afqmc_gto import (  # noqa: F401
     force_bias, greens_function, greens_function_force_bias, greens_function_overlap,
     half_rotate_cholesky, local_energy, local_energy_2body, local_energy_streamed,
     one_body_chol_correction, orthogonalize_walkers, population_control_comb,
-    prepare_afqmc_integrals_from_arrays, propagate_walkers)
+    prepare_afqmc_integrals_from_arrays, propagate_walkers,
+    MultiDetSystem, greens_function_multidet, greens_function_multidet_overlap_only,
+    local_energy_multidet, multidet_charge_density, occupations_from_trials,
+    propagate_walkers_multidet)
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
 
 __version__ = "0.1.0"

@@ -60,6 +60,13 @@ SIGNATURES = {
     "afq_system_create": [_P, ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P,
                           _D, _D, _I, _I],
     "afq_system_destroy": [_P],
+    "afq_greens_function_multidet": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
+    "afq_greens_function_multidet_overlap_only": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P,
+                                                  _I, _P],
+    "afq_local_energy_multidet": [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D,
+                                  _I, _P, _P, _P],
+    "afq_system_create_multidet": [_P, ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P, _I, _P,
+                                   _P, _P, _D, _D, _I, _I, _I],
     "afq_propagate_walkers": [_P, _I, _P, _P, _P, _P, _D, _P, _U64, _I64, _P],
     "afq_system_local_energy": [_P, _I, _P, _P, _P, _D, _P, _P, _P],
     "afq_system_timing_enable": [_P, _I],

@@ -510,6 +510,148 @@ def propagate_walkers(phia, phib, weights, overlap, e_hybrid, propagator, chol, 
 
 
 # ---------------------------------------------------------------------------
+# Multi-determinant trials (greens_th4.py, energy_th4.py, afqmc_gto.py:728-839)
+# ---------------------------------------------------------------------------
+
+def occupations_from_trials(trials):
+    """(ndet, M, N) dets built as eye(M)[:, occ] (afqmc_gto.py:1172-1174) ->
+    occ (ndet, N) int32 on the device.  Raises ValueError for a det that is
+    not a selection of unit columns (only occupation-string dets are built)."""
+    t = trials.detach().cpu() if isinstance(trials, torch.Tensor) else torch.from_numpy(
+        np.asarray(trials))
+    t = t.to(torch.complex128) if not t.is_complex() else t
+    nd, M, N = t.shape
+    occ = torch.argmax(torch.abs(t), dim=1).to(torch.int32)  # (nd, N)
+    ref = torch.zeros_like(t)
+    if N:
+        ref.scatter_(1, occ.long().unsqueeze(1), 1.0)
+    if not torch.equal(t, ref):
+        raise ValueError("multi-determinant trials must be columns of the identity "
+                         "(occupation strings)")
+    return occ.to(_dev()).contiguous()
+
+
+def _occ(x):
+    """Occupation strings given directly (int) or as dense trials."""
+    if isinstance(x, (np.ndarray, torch.Tensor)) and not (
+            (isinstance(x, np.ndarray) and np.issubdtype(x.dtype, np.integer))
+            or (isinstance(x, torch.Tensor) and not x.is_floating_point() and not x.is_complex())):
+        return occupations_from_trials(x)
+    t = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x)
+    return t.to(device=_dev(), dtype=torch.int32).contiguous()
+
+
+def greens_function_multidet(phia, phib, trials_up, trials_dn, ci_coeffs, det_chunk_size=5):
+    """greens_th4.py:75-148 -> (Ga, Gb, overlap)."""
+    pa, pb, ci = _c(phia), _c(phib), _c(ci_coeffs)
+    oa, ob = _occ(trials_up), _occ(trials_dn)
+    W, M, Na = pa.shape
+    Nb, nd = pb.shape[2], ci.shape[0]
+    Ga = torch.empty((W, M, M), dtype=C128, device=pa.device)
+    Gb = torch.empty_like(Ga)
+    ov = torch.empty((W,), dtype=C128, device=pa.device)
+    check(_L().afq_greens_function_multidet(_ctx(), W, M, Na, Nb, nd, ptr(pa), ptr(pb), ptr(oa),
+                                            ptr(ob), ptr(ci), int(det_chunk_size), ptr(Ga),
+                                            ptr(Gb), ptr(ov)), "afq_greens_function_multidet")
+    return Ga, Gb, ov
+
+
+def greens_function_multidet_overlap_only(phia, phib, trials_up, trials_dn, ci_coeffs,
+                                          det_chunk_size=5):
+    """greens_th4.py:186-228 -> overlap."""
+    pa, pb, ci = _c(phia), _c(phib), _c(ci_coeffs)
+    oa, ob = _occ(trials_up), _occ(trials_dn)
+    W, M, Na = pa.shape
+    ov = torch.empty((W,), dtype=C128, device=pa.device)
+    check(_L().afq_greens_function_multidet_overlap_only(
+        _ctx(), W, M, Na, pb.shape[2], ci.shape[0], ptr(pa), ptr(pb), ptr(oa), ptr(ob), ptr(ci),
+        int(det_chunk_size), ptr(ov)), "afq_greens_function_multidet_overlap_only")
+    return ov
+
+
+def local_energy_multidet(h1e, chol, phia, phib, trials_up, trials_dn, ci_coeffs, enuc,
+                          det_chunk_size=5):
+    """energy_th4.py:26-119 -> (e_tot, e_1b, e_2b)."""
+    pa, pb, ci = _c(phia), _c(phib), _c(ci_coeffs)
+    h, L = _r(h1e), _r(chol)
+    oa, ob = _occ(trials_up), _occ(trials_dn)
+    W, M, Na = pa.shape
+    out = [torch.empty((W,), dtype=C128, device=pa.device) for _ in range(3)]
+    check(_L().afq_local_energy_multidet(_ctx(), W, M, Na, pb.shape[2], L.shape[0], ci.shape[0],
+                                         ptr(h), ptr(L), ptr(pa), ptr(pb), ptr(oa), ptr(ob),
+                                         ptr(ci), float(enuc), int(det_chunk_size), ptr(out[0]),
+                                         ptr(out[1]), ptr(out[2])), "afq_local_energy_multidet")
+    return tuple(out)
+
+
+def multidet_charge_density(trials_up, trials_dn, ci_coeffs):
+    """afqmc_gto.py:1209-1218: sum_d |c_d|^2 (T_d T_d^H)_up + (...)_dn / sum |c|^2."""
+    tu, td, ci = _c(trials_up), _c(trials_dn), _c(ci_coeffs)
+    w = torch.abs(ci) ** 2
+    w = (w / torch.sum(w)).to(C128)
+    G = torch.einsum('d,dpi,dqi->pq', w, tu, tu.conj())
+    return G + torch.einsum('d,dpi,dqi->pq', w, td, td.conj())
+
+
+class MultiDetSystem(System):
+    """An ``afq_system`` with a multi-determinant trial of occupation strings."""
+
+    def __init__(self, propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound=None,
+                 exp_nmax=6, det_chunk_size=5, precision="fp64"):
+        self.chol = _r(chol)
+        E = propagator['expH1']
+        if isinstance(E, torch.Tensor) and E.is_complex():
+            if float(torch.max(torch.abs(E.imag))) != 0.0:
+                raise ValueError("expH1 must be real (mf_shift purely imaginary)")
+            E = E.real
+        self.expH1 = _r(E)
+        self.mf_shift = _c(propagator['mf_shift'])
+        self.occ_up, self.occ_dn = _occ(trials_up), _occ(trials_dn)
+        self.ci = _c(ci_coeffs)
+        self.dt = float(propagator['dt'])
+        self.fbbound = FBBOUND_DEFAULT if fbbound is None else float(fbbound)
+        self.nmax = int(exp_nmax)
+        self.precision = precision
+        G, M, _ = self.chol.shape
+        self.M, self.G = M, G
+        self.Na, self.Nb = self.occ_up.shape[1], self.occ_dn.shape[1]
+        self.ndet = self.ci.shape[0]
+        eye = torch.eye(M, dtype=C128, device=self.chol.device)
+        self.trial_up = eye[:, self.occ_up[0].long()].contiguous()   # dominant det
+        self.trial_dn = eye[:, self.occ_dn[0].long()].contiguous()
+        h = _lib._P()
+        check(_L().afq_system_create_multidet(
+            _ctx(), _lib.ctypes.byref(h), M, self.Na, self.Nb, G, ptr(self.chol),
+            ptr(self.expH1), ptr(self.mf_shift), self.ndet, ptr(self.occ_up), ptr(self.occ_dn),
+            ptr(self.ci), self.dt, self.fbbound, self.nmax, int(det_chunk_size),
+            _lib.PRECISIONS[precision]), "afq_system_create_multidet")
+        self.handle = h
+
+
+def propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagator, chol,
+                               trials_up, trials_dn, ci_coeffs, eshift, rng_key, fbbound=None,
+                               exp_nmax=6, walker_chunk_size=None, chol_chunk_g=None,
+                               det_chunk_size=5, xi=None, precision="fp64"):
+    """afqmc_gto.py:728-839 -> (phia, phib, weights_new, overlap_new, e_hybrid_new, rng_key)."""
+    key = (id(propagator), id(chol), id(trials_up), id(trials_dn), id(ci_coeffs), fbbound,
+           exp_nmax, det_chunk_size, precision, "md")
+    sysobj = propagator.get('_systems', {}).get(key) if isinstance(propagator, dict) else None
+    if sysobj is None:
+        sysobj = MultiDetSystem(propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound,
+                                exp_nmax, det_chunk_size, precision)
+        if isinstance(propagator, dict):
+            propagator.setdefault('_systems', {})[key] = sysobj
+    pa, pb = _c(phia).clone(), _c(phib).clone()
+    w = _r(weights).clone()
+    eh = _c(e_hybrid).clone()
+    if rng_key is None:
+        rng_key = PhiloxKey()
+    next_key, k = rng_key.split() if isinstance(rng_key, PhiloxKey) else (rng_key, None)
+    ov = sysobj.step(pa, pb, w, eh, eshift, xi=xi, key=k)
+    return pa, pb, w, ov, eh, next_key
+
+
+# ---------------------------------------------------------------------------
 # Statistics (utils.py:86-149)
 # ---------------------------------------------------------------------------
 

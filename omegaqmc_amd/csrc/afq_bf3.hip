@@ -539,13 +539,13 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
     hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, st, n, M, M, b->Kp, n, s->expH1, b->E);
     AFQ_CHECK_LAUNCH();
   }
-  if (G > 0 && s->Na > 0) {
+  if (G > 0 && s->Na > 0 && s->rchol_a) {
     const long n = (long)G * b->Ka;
     hipLaunchKernelGGL(k_prep_lh, dim3(4096), dim3(256), 0, st, n, s->Na, M, b->ldq, (long)b->Ka,
                        n, s->rchol_a, b->Lha);
     AFQ_CHECK_LAUNCH();
   }
-  if (G > 0 && s->Nb > 0) {
+  if (G > 0 && s->Nb > 0 && s->rchol_b) {
     const long n = (long)G * b->Kb;
     hipLaunchKernelGGL(k_prep_lh, dim3(4096), dim3(256), 0, st, n, s->Nb, M, b->ldq, (long)b->Kb,
                        n, s->rchol_b, b->Lhb);

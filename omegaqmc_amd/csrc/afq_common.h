@@ -114,6 +114,16 @@ struct afq_system {
   size_t work_bytes = 0;
   // optional per-phase HIP-event timing of the fused step
   struct Timing* timing = nullptr;
+  // multi-determinant trial (ndet > 0): occupation strings and CI
+  // coefficients (owned device copies) and the workspace of its step
+  int ndet = 0;
+  int det_chunk = 5;
+  int* occ_a = nullptr;
+  int* occ_b = nullptr;
+  double* ci = nullptr;
+  int md_cap = 0;
+  void* md_work = nullptr;
+  size_t md_bytes = 0;
 };
 
 // Phases of the fused step timed by afq_system_timing_*.

@@ -224,6 +224,15 @@ int e1b_half_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* h
   return AFQ_OK;
 }
 
+int e1b_fullg_launch(hipStream_t s, int W, int M, const double* h1e, const double* Ga,
+                     const double* Gb, double enuc, double* e1b) {
+  if (W <= 0) return AFQ_OK;
+  hipLaunchKernelGGL((k_e1b_fullg<false>), dim3(W), dim3(NT), 0, s, M, h1e, (const cplx*)Ga,
+                     (const cplx*)Gb, enuc, (cplx*)e1b);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
 int combine_launch(hipStream_t s, int W, const double* e1b, const double* ec, const double* ex,
                    double* etot, double* e2b) {
   hipLaunchKernelGGL(k_combine, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, (const cplx*)e1b,

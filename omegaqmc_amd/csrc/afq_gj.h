@@ -1,0 +1,207 @@
+// Wavefront-level dense kernels shared by the single- and multi-determinant
+// Green's functions: in-register Gauss-Jordan inversion with implicit row
+// pivoting, and Ghalf = S^-1 phi^T on FP64 MFMA by one wavefront.
+#pragma once
+#include "afq_common.h"
+
+namespace afq {
+namespace gj {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double cabs1(cplx a) { return fabs(a.x) + fabs(a.y); }
+
+// Orders the LDS accesses of one wavefront (lanes exchange data through LDS).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int GJ_NMAX = 40;
+constexpr int GJ_LD = GJ_NMAX + 1;
+// LDS of one wavefront: S^-1 (GJ_NMAX x GJ_LD complex), the pivot row, the pivot order
+constexpr int GJ_WAVE_LDS = (GJ_NMAX * GJ_LD + GJ_NMAX) * 16 + 64 * 4;
+
+// ----------------------------------------------------------------------------
+// Register Gauss-Jordan path (identity trials, N <= 40): one wavefront per
+// spin, lane i holds row i of S in registers.  Step k picks the pivot row p_k
+// of column k among the rows not used yet (first maximal |re| + |im|, as
+// getrf) and eliminates column k from every other row in place (in-place
+// Gauss-Jordan inversion with the row permutation kept implicit): afterwards
+// row p_k holds row k of S^-1, whose column j sits in the storage column of
+// column p_j.  det S = sign(p) prod_k d_k with d_k the pivots.
+//
+// The row is kept rotated so that the active column is always register 0:
+// the update of step k writes column j+1 into register j, and the new column
+// k (the pivot row's 1/d, the others' -f/d) goes to the last register.  Every
+// register index is static, so the step is straight-line VALU code.  With
+// g = f/d for the other rows and g = 1 - 1/d for the pivot row, the single
+// update a_j <- a_j - g * (unscaled pivot row)_j covers both.
+// ----------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* order,
+                                        int& mystep, cplx* ov) {
+  const int l = threadIdx.x & 63;
+  bool active = l < N;
+  mystep = 0;
+  double logabs = 0.0;
+  cplx ph = mkc(1.0, 0.0);
+  int parity = 0;
+  bool sing = false;
+  for (int k = 0; k < N; ++k) {
+    const cplx f = a[0];
+    double v = -1.0;
+    if (active) {
+      v = cabs1(f);
+      if (v != v) v = -0.5;  // a NaN row is taken only if nothing else is left
+    }
+    int bi = l;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov2 = __shfl_xor(v, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov2 > v || (ov2 == v && oi < bi)) { v = ov2; bi = oi; }
+    }
+    const int p = __builtin_amdgcn_readfirstlane(bi);
+    const bool me = l == p;
+    if (me) {
+#pragma unroll
+      for (int j = 1; j < NB; ++j) bro[j] = a[j];
+      bro[0] = f;
+      active = false;
+      mystep = k;
+    }
+    if (l == 0) order[k] = p;
+    // sign of the pivot permutation from its Lehmer code
+    const unsigned long long am = __ballot(active || me);
+    parity ^= __popcll(am & ((1ull << p) - 1ull)) & 1;
+    wave_sync();
+    const cplx d = bro[0];
+    const double ad = hypot(d.x, d.y);
+    if (ad == 0.0) {
+      sing = true;
+    } else {
+      logabs += log(ad);
+      ph = cmul(ph, mkc(d.x / ad, d.y / ad));
+    }
+    const cplx id = cdiv(mkc(1.0, 0.0), d);
+    const cplx g = me ? mkc(1.0 - id.x, -id.y) : cmul(f, id);
+    const cplx col = me ? id : mkc(-g.x, -g.y);
+#pragma unroll
+    for (int j = 0; j + 1 < NB; ++j) {
+      cplx x = a[j + 1];
+      cfms(x, g, bro[j + 1]);
+      a[j] = x;
+    }
+    a[NB - 1] = col;
+    wave_sync();
+  }
+  if (l == 0) {
+    if (sing) *ov = mkc(0.0, 0.0);
+    else {
+      const cplx r = cscale(ph, exp(logabs));
+      *ov = parity ? mkc(-r.x, -r.y) : r;
+    }
+  }
+}
+
+// Ghalf (N x M) = Inv (N x N) . phi^T on FP64 MFMA by one wavefront, N <= 40.
+// Column tiles (16 orbitals) go in pairs with two static fragment buffers: the
+// phi fragments of the next tile are loaded while the current one runs its
+// N/16 row tiles (each phi fragment feeds every row tile).
+template <int KQ>
+__device__ __forceinline__ void ghalf_tile(const cplx* Inv, int LD, int N, int M, int tj,
+                                           const cplx (&b)[KQ], cplx* __restrict__ Ghalf,
+                                           bf16_t* __restrict__ gb, long Ks, long gplane,
+                                           int ldq) {
+  const int l = threadIdx.x & 63;
+  const int li = l & 15, lq = l >> 4;
+  const int RTn = (N + 15) >> 4;
+  const int q = 16 * tj + li;
+  for (int ti = 0; ti < RTn; ++ti) {
+    d4v cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+    const int ai = 16 * ti + li;
+#pragma unroll
+    for (int kk = 0; kk < KQ; ++kk) {
+      const int k = 4 * kk + lq;
+      const cplx a = (ai < N && k < N) ? Inv[ai * LD + k] : mkc(0.0, 0.0);
+      cr = mfma64(a.x, b[kk].x, cr);
+      ci = mfma64(a.x, b[kk].y, ci);
+      cr = mfma64(-a.y, b[kk].y, cr);
+      ci = mfma64(a.y, b[kk].x, ci);
+    }
+    if (q >= M) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * ti + lq + 4 * r;
+      if (i >= N) continue;
+      if (Ghalf) Ghalf[(long)i * M + q] = mkc(cr[r], ci[r]);
+      if (gb) {
+        const long k = (long)i * ldq + q;
+        bf16_t h, lo;
+        split2(cr[r], h, lo);
+        gb[k] = h;
+        gb[gplane + k] = lo;
+        split2(ci[r], h, lo);
+        gb[Ks + k] = h;
+        gb[gplane + Ks + k] = lo;
+      }
+    }
+  }
+}
+
+template <int KQ>
+__device__ __forceinline__ void ghalf_frag(const cplx* __restrict__ phi, int N, int M, int tj,
+                                           cplx (&b)[KQ]) {
+  const int l = threadIdx.x & 63;
+  const int li = l & 15, lq = l >> 4;
+  const int bj = min(16 * tj + li, M - 1);  // rows past M only feed unused outputs
+#pragma unroll
+  for (int kk = 0; kk < KQ; ++kk) {
+    const int k = 4 * kk + lq;
+    b[kk] = k < N ? phi[(long)bj * N + k] : mkc(0.0, 0.0);
+  }
+}
+
+template <int KQ>
+__device__ __forceinline__ void ghalf_mfma_wave(const cplx* Inv, int LD, int N, int M,
+                                                const cplx* __restrict__ phi,
+                                                cplx* __restrict__ Ghalf, bf16_t* __restrict__ gb,
+                                                long Ks, long gplane, int ldq) {
+  const int RTm = (M + 15) >> 4;
+  cplx b0[KQ], b1[KQ];
+  ghalf_frag<KQ>(phi, N, M, 0, b0);
+  for (int tj = 0; tj < RTm; tj += 2) {
+    if (tj + 1 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 1, b1);
+    ghalf_tile<KQ>(Inv, LD, N, M, tj, b0, Ghalf, gb, Ks, gplane, ldq);
+    if (tj + 1 >= RTm) break;
+    if (tj + 2 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 2, b0);
+    ghalf_tile<KQ>(Inv, LD, N, M, tj + 1, b1, Ghalf, gb, Ks, gplane, ldq);
+  }
+}
+
+// Scatter the Gauss-Jordan result (registers, rotated N times) into S^-1 in
+// LDS: row p_k of the result is row k of S^-1; storage column c (now in
+// register (c - N) mod NB) is column p_c.
+template <int NB>
+__device__ __forceinline__ void gj_store_inverse(const cplx (&a)[NB], int N, int mystep,
+                                                 const int* order, cplx* Inv) {
+  const int l = threadIdx.x & 63;
+  if (l < N) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      int c = q + N;
+      if (c >= NB) c -= NB;
+      if (c < N) Inv[mystep * GJ_LD + order[c]] = a[q];
+    }
+  }
+  wave_sync();
+}
+
+}  // namespace gj
+}  // namespace afq

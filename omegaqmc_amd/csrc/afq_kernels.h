@@ -57,6 +57,9 @@ int e1b_half_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* h
                     double* e1b);
 int combine_launch(hipStream_t s, int W, const double* e1b, const double* ec, const double* ex,
                    double* etot, double* e2b);
+// e1b[w] = sum_pq h[p][q] (Ga + Gb)[w][q][p] + enuc, h real
+int e1b_fullg_launch(hipStream_t s, int W, int M, const double* h1e, const double* Ga,
+                     const double* Gb, double enuc, double* e1b);
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                    const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
                    double* X, double* e_coul);

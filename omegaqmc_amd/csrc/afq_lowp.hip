@@ -75,6 +75,17 @@ int ensure(afq_system* s, int W, LowpWork** out) {
 
 }  // namespace
 
+// The split-bf16 xshifted planes and VHS of the reduced-precision workspace
+// (used by the multi-determinant step, whose force bias is FP64).
+int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs) {
+  LowpWork* L = nullptr;
+  int rc = ensure(sys, W, &L);
+  if (rc) return rc;
+  *X = (bf16_t*)((char*)L->buf + L->X);
+  *vhs = (bf16_t*)((char*)L->buf + L->vhs);
+  return AFQ_OK;
+}
+
 int system_prepare_lowp(afq_system* sys) {
   if (sys->precision == AFQ_FP64) return AFQ_OK;
   if (sys->precision != AFQ_BF16X3)

@@ -1,0 +1,666 @@
+// Multi-determinant trials whose determinants are occupation strings,
+// trial_s[I] = eye(M)[:, occ_s[I]] (the driver's construction,
+// OmegaQMC/afqmc_gto.py:1163-1174).
+//
+//   greens_th4.py:75-148   Ghalf_I, o_I per det; dets with
+//                           |o_Ia o_Ib| < max(1e-14 max_J |o_J|, 1e-300) are
+//                           masked (their Ghalf reads as zero);
+//                           w_I = c_I^* o_Ia o_Ib, overlap = sum_I w_I (det
+//                           chunks summed in the reference's scan order);
+//                           G_s = sum_I w_I T_I^* Ghalf_I / overlap.
+//   greens_th4.py:186-228  overlap only.
+//   energy_th4.py:26-119   E_1b from G, E_2b = sum_I w_I E_2b,I / overlap with
+//                           the rows occ_I of L as the half-rotated vectors.
+//   afqmc_gto.py:728-839   the step: force bias from G, then the single-det
+//                           propagation kernels.
+//
+// With occupation-string dets S_I = phi^T T_I^* is the row gather
+// S_I[i][j] = phi[occ_I[j]][i] and T_I^* Ghalf_I scatters the rows of Ghalf_I
+// to rows occ_I of G, so no dense trial product is formed.  Each (walker,
+// det) pair of spins is one workgroup of two wavefronts running the register
+// Gauss-Jordan kernel of the single-det Green's function.
+#include <algorithm>
+#include <vector>
+
+#include "afq_bf3.h"
+#include "afq_gemm.h"
+#include "afq_gj.h"
+#include "afq_kernels.h"
+#include "afq_mfma.h"
+#include "afq_multidet.h"
+#include "afq_step.h"
+
+namespace afq {
+
+void phase_begin(afq_system* s, int ph);
+void phase_end(afq_system* s, int ph);
+int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs);
+
+namespace {
+
+using gj::GJ_LD;
+using gj::GJ_NMAX;
+using gj::GJ_WAVE_LDS;
+
+constexpr double MASK_RTOL = 1e-14;   // greens_th4.py:24
+constexpr double MASK_ATOL = 1e-300;  // greens_th4.py:25
+
+// Grid W * nd (det fastest), two wavefronts: wave s = spin s.
+template <int NB>
+__global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, int Nb, int nd,
+                                               const cplx* __restrict__ phia,
+                                               const cplx* __restrict__ phib,
+                                               const int* __restrict__ occa,
+                                               const int* __restrict__ occb, cplx* __restrict__ oda,
+                                               cplx* __restrict__ odb, cplx* __restrict__ gha,
+                                               cplx* __restrict__ ghb) {
+  extern __shared__ __align__(16) char smem[];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int I = blockIdx.x % nd;
+  const long w = blockIdx.x / nd;
+  const int N = wv ? Nb : Na;
+  cplx* od = (wv ? odb : oda) + w * nd + I;
+  if (N == 0) {
+    if (l == 0) *od = mkc(1.0, 0.0);
+    return;  // no workgroup barrier in this kernel
+  }
+  char* base = smem + wv * GJ_WAVE_LDS;
+  cplx* Inv = (cplx*)base;
+  cplx* bro = Inv + GJ_NMAX * GJ_LD;
+  int* order = (int*)(bro + GJ_NMAX);
+  const cplx* phi = wv ? phib + w * M * Nb : phia + w * M * Na;
+  const int* occ = (wv ? occb : occa) + (long)I * N;
+  cplx a[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    a[j] = (l < N && j < N) ? phi[(long)occ[j] * N + l] : mkc(0.0, 0.0);
+  int mystep = 0;
+  gj::wave_gj<NB>(a, N, bro, order, mystep, od);
+  if (mode & 1) {
+    gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
+    cplx* G = (wv ? ghb : gha) + ((long)I * W + w) * N * M;
+    gj::ghalf_mfma_wave<(NB + 3) / 4>(Inv, GJ_LD, N, M, phi, G, nullptr, 0, 0, 0);
+  }
+}
+
+// One thread per walker: masks, w_I = conj(c_I) o_Ia o_Ib and the overlap,
+// summed per det chunk then across chunks as the reference's scan.
+__global__ void k_md_weights(int W, int nd, int chunk, const cplx* __restrict__ oda,
+                             const cplx* __restrict__ odb, const cplx* __restrict__ ci,
+                             cplx* __restrict__ z, int* __restrict__ keep,
+                             cplx* __restrict__ ovlp) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double omax = 0.0;
+  for (int I = 0; I < nd; ++I) {
+    const cplx o = cmul(oda[(long)w * nd + I], odb[(long)w * nd + I]);
+    omax = jmax(omax, hypot(o.x, o.y));
+  }
+  const double tol = jmax(MASK_RTOL * omax, MASK_ATOL);
+  cplx acc = mkc(0.0, 0.0);
+  for (int c0 = 0; c0 < nd; c0 += chunk) {
+    cplx cs = mkc(0.0, 0.0);
+    const int c1 = min(nd, c0 + chunk);
+    for (int I = c0; I < c1; ++I) {
+      const cplx oa = oda[(long)w * nd + I], ob = odb[(long)w * nd + I];
+      const cplx o = cmul(oa, ob);
+      const cplx wi = cmul(cmul(cconj(ci[I]), oa), ob);
+      cs = cadd(cs, wi);
+      const bool k = hypot(o.x, o.y) >= tol;
+      keep[(long)w * nd + I] = k ? 1 : 0;
+      z[(long)w * nd + I] = k ? wi : mkc(0.0, 0.0);
+    }
+    acc = cadd(acc, cs);
+  }
+  ovlp[w] = acc;
+}
+
+// G_s[w][p][q] = sum_I z_I Ghalf_I[i][q] (p = occ_I[i]) / overlap; the
+// Ghalf_I of a masked det is zeroed in place (it may hold inf/NaN).
+// Grid (W, 2), one thread per column q.
+__global__ __launch_bounds__(256) void k_md_gfull(int W, int M, int Na, int Nb, int nd,
+                                                  const int* __restrict__ occa,
+                                                  const int* __restrict__ occb,
+                                                  const cplx* __restrict__ z,
+                                                  const int* __restrict__ keep,
+                                                  const cplx* __restrict__ ovlp,
+                                                  cplx* __restrict__ gha, cplx* __restrict__ ghb,
+                                                  cplx* __restrict__ Ga, cplx* __restrict__ Gb,
+                                                  int want_g) {
+  const long w = blockIdx.x;
+  const int sp = blockIdx.y;
+  const int N = sp ? Nb : Na;
+  const int* occ = sp ? occb : occa;
+  cplx* gh = sp ? ghb : gha;
+  cplx* G = (sp ? Gb : Ga) + w * M * M;
+  const long MM = (long)M * M;
+  (void)MM;
+  const cplx ov = ovlp[w];
+  for (int q = threadIdx.x; q < M; q += blockDim.x) {
+    if (want_g)
+      for (int p = 0; p < M; ++p) G[(long)p * M + q] = mkc(0.0, 0.0);
+    for (int I = 0; I < nd; ++I) {
+      cplx* g = gh + ((long)I * W + w) * N * M;
+      if (!keep[w * nd + I]) {
+        for (int i = 0; i < N; ++i) g[(long)i * M + q] = mkc(0.0, 0.0);
+        continue;
+      }
+      if (!want_g) continue;
+      const cplx zi = z[w * nd + I];
+      for (int i = 0; i < N; ++i) {
+        const int p = occ[(long)I * N + i];
+        cfma(G[(long)p * M + q], zi, g[(long)i * M + q]);
+      }
+    }
+    if (want_g)
+      for (int p = 0; p < M; ++p) G[(long)p * M + q] = cdiv(G[(long)p * M + q], ov);
+  }
+}
+
+// Gt[w][q][p] = Ga[w][p][q] + Gb[w][p][q]  (the force bias contracts L_pq G_qp)
+__global__ void k_md_gt(long n, int M, const cplx* __restrict__ Ga, const cplx* __restrict__ Gb,
+                        cplx* __restrict__ Gt) {
+  const long MM = (long)M * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long w = e / MM, r = e % MM;
+    const int q = (int)(r / M), p = (int)(r % M);
+    const long src = w * MM + (long)p * M + q;
+    Gt[e] = cadd(Ga[src], Gb[src]);
+  }
+}
+
+// rchol_I[g][i][q] = L[g][occ_I[i]][q]
+__global__ void k_md_rchol(long n, int M, int N, const int* __restrict__ occ,
+                           const double* __restrict__ L, double* __restrict__ out) {
+  const long NM = (long)N * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long g = e / NM, r = e % NM;
+    const int i = (int)(r / M), q = (int)(r % M);
+    out[e] = L[(g * M + occ[i]) * M + q];
+  }
+}
+
+// E2[I][w] = e_coul - e_exch of det I
+__global__ void k_md_e2b_store(int W, const cplx* __restrict__ ec, const cplx* __restrict__ ex,
+                               cplx* __restrict__ e2) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < W) e2[w] = csub(ec[w], ex[w]);
+}
+
+// e2b = (sum over chunks of sum_I z_I E2_I) / overlap; e_tot = e1b + e2b
+__global__ void k_md_e2b_final(int W, int nd, int chunk, const cplx* __restrict__ z,
+                               const cplx* __restrict__ e2, const cplx* __restrict__ ovlp,
+                               const cplx* __restrict__ e1b, cplx* __restrict__ etot,
+                               cplx* __restrict__ e2b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  cplx acc = mkc(0.0, 0.0);
+  for (int c0 = 0; c0 < nd; c0 += chunk) {
+    cplx cs = mkc(0.0, 0.0);
+    for (int I = c0; I < min(nd, c0 + chunk); ++I)
+      cs = cadd(cs, cmul(z[(long)w * nd + I], e2[(long)I * W + w]));
+    acc = cadd(acc, cs);
+  }
+  const cplx v = cdiv(acc, ovlp[w]);
+  if (e2b) e2b[w] = v;
+  etot[w] = cadd(e1b[w], v);
+}
+
+size_t a256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t md_bytes(int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull) {
+  size_t o = 0;
+  if (ghalf || gfull) {
+    o += a256(16UL * nd * W * Na * M);
+    o += a256(16UL * nd * W * Nb * M);
+  }
+  o += 2 * a256(16UL * W * nd);        // oda, odb
+  o += a256(16UL * W * nd);            // z
+  o += a256(4UL * W * nd);             // keep
+  o += a256(16UL * W);                 // ovlp
+  if (gfull) o += 2 * a256(16UL * W * M * M);
+  return o;
+}
+
+MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull) {
+  MdBufs b;
+  char* p = (char*)base;
+  auto take = [&](size_t n) {
+    char* r = p;
+    p += a256(n);
+    return r;
+  };
+  if (ghalf || gfull) {
+    b.gha = (double*)take(16UL * nd * W * Na * M);
+    b.ghb = (double*)take(16UL * nd * W * Nb * M);
+  }
+  b.oda = (double*)take(16UL * W * nd);
+  b.odb = (double*)take(16UL * W * nd);
+  b.z = (double*)take(16UL * W * nd);
+  b.keep = (int*)take(4UL * W * nd);
+  b.ovlp = (double*)take(16UL * W);
+  if (gfull) {
+    b.Ga = (double*)take(16UL * W * M * M);
+    b.Gb = (double*)take(16UL * W * M * M);
+  }
+  return b;
+}
+
+int md_check_occ(hipStream_t s, int nd, int N, int M, const int32_t* occ) {
+  if (nd == 0 || N == 0) return AFQ_OK;
+  if (!occ) return fail(AFQ_EINVAL, "occupation strings are NULL");
+  std::vector<int32_t> h((size_t)nd * N);
+  AFQ_CHECK_HIP(hipMemcpyAsync(h.data(), occ, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  AFQ_CHECK_HIP(hipStreamSynchronize(s));
+  for (int I = 0; I < nd; ++I) {
+    std::vector<int32_t> r(h.begin() + (size_t)I * N, h.begin() + (size_t)(I + 1) * N);
+    for (int32_t v : r)
+      if (v < 0 || v >= M) return fail(AFQ_EINVAL, "det %d: orbital index %d outside [0, %d)", I, v, M);
+    std::sort(r.begin(), r.end());
+    if (std::adjacent_find(r.begin(), r.end()) != r.end())
+      return fail(AFQ_EINVAL, "det %d repeats an orbital", I);
+  }
+  return AFQ_OK;
+}
+
+int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, const double* phia,
+              const double* phib, const int* occa, const int* occb, const double* ci, bool ghalf,
+              bool gfull, const MdBufs& b) {
+  if (W <= 0) return AFQ_OK;
+  const int Nmax = std::max(Na, Nb);
+  if (Nmax > GJ_NMAX) return fail(AFQ_EUNSUPPORTED, "multi-det: nocc %d > %d", Nmax, GJ_NMAX);
+  const size_t sm = 2 * (size_t)GJ_WAVE_LDS;
+  if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "multi-det: LDS %zu", sm);
+  if ((long)W * nd > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "multi-det: grid too large");
+  const int mode = (ghalf || gfull) ? 1 : 0;
+  const dim3 grid((unsigned)((long)W * nd));
+#define AFQ_MDGJ(NB)                                                                        \
+  hipLaunchKernelGGL((k_md_gj<NB>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd,         \
+                     (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda,          \
+                     (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb)
+  if (Nmax <= 8) AFQ_MDGJ(8);
+  else if (Nmax <= 16) AFQ_MDGJ(16);
+  else if (Nmax <= 24) AFQ_MDGJ(24);
+  else if (Nmax <= 32) AFQ_MDGJ(32);
+  else AFQ_MDGJ(40);
+#undef AFQ_MDGJ
+  AFQ_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_md_weights, dim3(cdiv_int(W, 128)), dim3(128), 0, s, W, nd,
+                     std::max(1, chunk), (const cplx*)b.oda, (const cplx*)b.odb, (const cplx*)ci,
+                     (cplx*)b.z, b.keep, (cplx*)b.ovlp);
+  AFQ_CHECK_LAUNCH();
+  if (ghalf || gfull) {
+    hipLaunchKernelGGL(k_md_gfull, dim3(W, 2), dim3(256), 0, s, W, M, Na, Nb, nd, occa, occb,
+                       (const cplx*)b.z, b.keep, (const cplx*)b.ovlp, (cplx*)b.gha, (cplx*)b.ghb,
+                       (cplx*)b.Ga, (cplx*)b.Gb, gfull ? 1 : 0);
+    AFQ_CHECK_LAUNCH();
+  }
+  return AFQ_OK;
+}
+
+// FP64 multi-det local energy given the buffers of md_greens(ghalf, gfull).
+// scratch: E2 (nd W complex) | rchol_a | rchol_b | X (W G complex) | ec | ex
+//          | exchange slab (gc N W N complex)
+static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int* gc_out) {
+  const long N = std::max(Na, Nb);
+  long gc = N > 0 ? std::max(1L, std::min((long)G, (256L << 20) / (16L * N * W * N))) : 1;
+  if (gc_out) *gc_out = (int)gc;
+  return a256(16UL * nd * W) + a256(8UL * G * Na * M) + a256(8UL * G * Nb * M) +
+         a256(16UL * W * G) + 2 * a256(16UL * W) + a256(16UL * gc * N * W * N);
+}
+
+static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd, int chunk,
+                     const double* h1e, const double* chol, const int* occa, const int* occb,
+                     double enuc, const MdBufs& b, void* scr, double* e_tot, double* e_1b,
+                     double* e_2b) {
+  int gc = 1;
+  (void)md_energy_scratch(W, M, Na, Nb, G, nd, &gc);
+  char* p = (char*)scr;
+  auto take = [&](size_t n) {
+    char* r = p;
+    p += a256(n);
+    return r;
+  };
+  double* E2 = (double*)take(16UL * nd * W);
+  double* ra = (double*)take(8UL * G * Na * M);
+  double* rb = (double*)take(8UL * G * Nb * M);
+  double* X = (double*)take(16UL * W * G);
+  double* ec = (double*)take(16UL * W);
+  double* ex = (double*)take(16UL * W);
+  double* slab = (double*)take(16UL * gc * std::max(Na, Nb) * W * std::max(Na, Nb));
+  int rc = e1b_fullg_launch(s, W, M, h1e, b.Ga, b.Gb, enuc, e_1b);
+  if (rc) return rc;
+  for (int I = 0; I < nd; ++I) {
+    if (Na && G) {
+      const long n = (long)G * Na * M;
+      hipLaunchKernelGGL(k_md_rchol, dim3(2048), dim3(256), 0, s, n, M, Na, occa + (long)I * Na,
+                         chol, ra);
+      AFQ_CHECK_LAUNCH();
+    }
+    if (Nb && G) {
+      const long n = (long)G * Nb * M;
+      hipLaunchKernelGGL(k_md_rchol, dim3(2048), dim3(256), 0, s, n, M, Nb, occb + (long)I * Nb,
+                         chol, rb);
+      AFQ_CHECK_LAUNCH();
+    }
+    const double* ga = b.gha + 2L * I * W * Na * M;
+    const double* gb = b.ghb + 2L * I * W * Nb * M;
+    if ((rc = coulomb_launch(s, W, M, Na, Nb, G, ra, rb, false, ga, gb, X, ec))) return rc;
+    if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
+    if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
+    hipLaunchKernelGGL(k_md_e2b_store, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W,
+                       (const cplx*)ec, (const cplx*)ex, (cplx*)E2 + (long)I * W);
+    AFQ_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_md_e2b_final, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, nd,
+                     std::max(1, chunk), (const cplx*)b.z, (const cplx*)E2, (const cplx*)b.ovlp,
+                     (const cplx*)e_1b, (cplx*)e_tot, (cplx*)e_2b);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the prepared multi-det system
+// ---------------------------------------------------------------------------
+namespace {
+
+struct MdStep {
+  MdBufs b;
+  double *oo, *vbias, *xs, *cfb, *cmf, *xi, *Gt, *vhs, *ovn;
+};
+
+size_t md_step_bytes(const afq_system* s, int W) {
+  const long M = s->M, G = s->G;
+  size_t n = md_bytes(W, s->M, s->Na, s->Nb, s->ndet, true, true);
+  n += a256(16UL * W);                   // oo
+  n += a256(16UL * W * G);               // vbias
+  n += a256(16UL * W * G);               // xs
+  n += 2 * a256(16UL * W);               // cfb, cmf
+  n += a256(8UL * W * G);                // xi
+  n += a256(16UL * W * M * M);           // Gt
+  n += a256(16UL * W * M * M);           // FP64 VHS
+  n += md_bytes(W, s->M, s->Na, s->Nb, s->ndet, false, false);  // overlap-only pass
+  return n;
+}
+
+int md_ensure(afq_system* s, int W, MdStep* st, MdBufs* ovb) {
+  if (!s->md_work || W > s->md_cap) {
+    if (s->md_work) {
+      AFQ_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+      AFQ_CHECK_HIP(hipFree(s->md_work));
+      s->md_work = nullptr;
+    }
+    const size_t n = md_step_bytes(s, W);
+    if (hipMalloc(&s->md_work, n) != hipSuccess) {
+      s->md_work = nullptr;
+      s->md_cap = 0;
+      return fail(AFQ_ENOMEM, "multi-det workspace of %zu bytes", n);
+    }
+    s->md_bytes = n;
+    s->md_cap = W;
+  }
+  const int C = s->md_cap;
+  const long M = s->M, G = s->G;
+  char* p = (char*)s->md_work;
+  st->b = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, true, true);
+  p += md_bytes(C, s->M, s->Na, s->Nb, s->ndet, true, true);
+  auto take = [&](size_t n) {
+    char* r = p;
+    p += a256(n);
+    return (double*)r;
+  };
+  st->oo = take(16UL * C);
+  st->vbias = take(16UL * C * G);
+  st->xs = take(16UL * C * G);
+  st->cfb = take(16UL * C);
+  st->cmf = take(16UL * C);
+  st->xi = take(8UL * C * G);
+  st->Gt = take(16UL * C * M * M);
+  st->vhs = take(16UL * C * M * M);
+  *ovb = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, false, false);
+  // the buffers are laid out for C walkers; re-carve for W keeps strides W
+  st->b = md_carve(s->md_work, W, s->M, s->Na, s->Nb, s->ndet, true, true);
+  return AFQ_OK;
+}
+
+}  // namespace
+
+int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, double* weights,
+                       double* e_hybrid, double eshift, const double* xi_in, uint64_t seed,
+                       int64_t step, double* ovlp_new) {
+  hipStream_t s = sys->ctx->stream;
+  const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G, nd = sys->ndet;
+  MdStep st;
+  MdBufs ovb;
+  int rc = md_ensure(sys, W, &st, &ovb);
+  if (rc) return rc;
+  const double* xi = xi_in;
+  if (!xi) {
+    phase_begin(sys, PH_RNG);
+    rc = aux_fields_launch(s, seed, step, W, G, st.xi);
+    phase_end(sys, PH_RNG);
+    if (rc) return rc;
+    xi = st.xi;
+  }
+  // 1: multi-det G and overlap
+  phase_begin(sys, PH_GREENS);
+  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
+                 true, true, st.b);
+  phase_end(sys, PH_GREENS);
+  if (rc) return rc;
+  // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp as one GEMM over (p, q)
+  phase_begin(sys, PH_FORCEBIAS);
+  {
+    const long n = (long)W * M * M;
+    hipLaunchKernelGGL(k_md_gt, dim3(4096), dim3(256), 0, s, n, M, (const cplx*)st.b.Ga,
+                       (const cplx*)st.b.Gb, (cplx*)st.Gt);
+    AFQ_CHECK_LAUNCH();
+    GemmArgs g;
+    g.m = W; g.n = G; g.k = M * M;
+    g.A = st.Gt; g.lda_m = (long)M * M; g.lda_k = 1; g.a_cplx = true;
+    g.B = sys->chol; g.ldb_k = 1; g.ldb_n = (long)M * M; g.b_cplx = false;
+    g.C = (cplx*)st.vbias; g.ldc = G;
+    if ((rc = gemm_fp64(s, g))) return rc;
+  }
+  phase_end(sys, PH_FORCEBIAS);
+  // 4-6: shift, VHS, expH1 . exp(VHS) . expH1 with the single-det kernels
+  if (sys->precision == AFQ_BF16X3) {
+    bf16_t* X = nullptr;
+    bf16_t* vb = nullptr;
+    if ((rc = lowp_step_buffers(sys, W, &X, &vb))) return rc;
+    const Bf3System& b = *(const Bf3System*)sys->chol_lp;
+    phase_begin(sys, PH_SHIFT);
+    rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr,
+                      nullptr, st.cfb, st.cmf, 1, nullptr, X, b.Gp);
+    phase_end(sys, PH_SHIFT);
+    if (rc) return rc;
+    phase_begin(sys, PH_VHS);
+    rc = bf3_vhs(s, b, W, X, sys->dt, vb);
+    phase_end(sys, PH_VHS);
+    if (rc) return rc;
+    phase_begin(sys, PH_TAYLOR);
+    rc = bf3_propagate(s, b, W, vb, phia, phib, sys->nmax);
+    phase_end(sys, PH_TAYLOR);
+    if (rc) return rc;
+  } else {
+    phase_begin(sys, PH_SHIFT);
+    rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr, st.xs,
+                      st.cfb, st.cmf, 1);
+    phase_end(sys, PH_SHIFT);
+    if (rc) return rc;
+    phase_begin(sys, PH_VHS);
+    if (sys->chol_packed)
+      rc = vhs_mfma_f64(s, W, M, G, st.xs, sys->chol_packed, sys->packed_P, sys->colmap, sys->dt,
+                        st.vhs);
+    else
+      rc = vhs_mfma_f64(s, W, M, G, st.xs, sys->chol, M * M, nullptr, sys->dt, st.vhs);
+    phase_end(sys, PH_VHS);
+    if (rc) return rc;
+    phase_begin(sys, PH_TAYLOR);
+    rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, st.vhs, phia, phib, phia, phib, sys->nmax);
+    phase_end(sys, PH_TAYLOR);
+    if (rc) return rc;
+  }
+  // 7: multi-det overlap of the propagated walkers
+  phase_begin(sys, PH_OVERLAP);
+  MdBufs ob = md_carve(ovb.oda, W, M, Na, Nb, nd, false, false);
+  ob.ovlp = ovlp_new;
+  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
+                 false, false, ob);
+  phase_end(sys, PH_OVERLAP);
+  if (rc) return rc;
+  // 8
+  phase_begin(sys, PH_WEIGHTS);
+  rc = weights_launch(s, W, weights, st.b.ovlp, ovlp_new, st.cfb, st.cmf, e_hybrid, eshift,
+                      sys->dt, weights, e_hybrid);
+  phase_end(sys, PH_WEIGHTS);
+  return rc;
+}
+
+int system_energy_multidet(afq_system* sys, int W, const double* phia, const double* phib,
+                           const double* h1e, double enuc, double* e_tot, double* e_1b,
+                           double* e_2b) {
+  hipStream_t s = sys->ctx->stream;
+  const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G, nd = sys->ndet;
+  MdStep st;
+  MdBufs ovb;
+  int rc = md_ensure(sys, W, &st, &ovb);
+  if (rc) return rc;
+  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
+                 true, true, st.b);
+  if (rc) return rc;
+  const size_t need = md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
+  void* scr = scratch(sys->ctx, need);
+  if (!scr) return AFQ_ENOMEM;
+  return md_energy(s, W, M, Na, Nb, G, nd, sys->det_chunk, h1e, sys->chol, sys->occ_a,
+                   sys->occ_b, enuc, st.b, scr, e_tot, e_1b, e_2b);
+}
+
+}  // namespace afq
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int afq_greens_function_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int nd,
+                                 const double* phia, const double* phib, const int32_t* occ_up,
+                                 const int32_t* occ_dn, const double* ci, int det_chunk_size,
+                                 double* Ga, double* Gb, double* overlap) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
+  AFQ_REQUIRE(phia && phib && ci && Ga && Gb && overlap, "NULL argument");
+  hipStream_t s = ctx->stream;
+  int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
+  if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
+  if (rc || W == 0) return rc;
+  const size_t n = afq::md_bytes(W, M, Na, Nb, nd, true, false);
+  void* scr = afq::scratch(ctx, n);
+  if (!scr) return AFQ_ENOMEM;
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, true, false);
+  b.Ga = Ga;
+  b.Gb = Gb;
+  b.ovlp = overlap;
+  return afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
+                        true, true, b);
+}
+
+int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int W, int M, int Na, int Nb, int nd,
+                                              const double* phia, const double* phib,
+                                              const int32_t* occ_up, const int32_t* occ_dn,
+                                              const double* ci, int det_chunk_size,
+                                              double* overlap) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
+  AFQ_REQUIRE(phia && phib && ci && overlap, "NULL argument");
+  hipStream_t s = ctx->stream;
+  int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
+  if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
+  if (rc || W == 0) return rc;
+  void* scr = afq::scratch(ctx, afq::md_bytes(W, M, Na, Nb, nd, false, false));
+  if (!scr) return AFQ_ENOMEM;
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, false, false);
+  b.ovlp = overlap;
+  return afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
+                        false, false, b);
+}
+
+int afq_local_energy_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, int nd,
+                              const double* h1e, const double* chol, const double* phia,
+                              const double* phib, const int32_t* occ_up, const int32_t* occ_dn,
+                              const double* ci, double enuc, int det_chunk_size, double* e_tot,
+                              double* e_1b, double* e_2b) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0 && G >= 0,
+              "bad shape");
+  AFQ_REQUIRE(h1e && chol && phia && phib && ci && e_tot && e_1b, "NULL argument");
+  hipStream_t s = ctx->stream;
+  int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
+  if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
+  if (rc || W == 0) return rc;
+  const size_t nb = afq::md_bytes(W, M, Na, Nb, nd, true, true);
+  const size_t ne = afq::md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
+  char* scr = (char*)afq::scratch(ctx, nb + ne);
+  if (!scr) return AFQ_ENOMEM;
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, true, true);
+  if ((rc = afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
+                           true, true, b)))
+    return rc;
+  return afq::md_energy(s, W, M, Na, Nb, G, nd, det_chunk_size, h1e, chol, occ_up, occ_dn, enuc,
+                        b, scr + nb, e_tot, e_1b, e_2b);
+}
+
+int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int M, int Na, int Nb, int G,
+                               const double* chol, const double* expH1, const double* mf,
+                               int nd, const int32_t* occ_up, const int32_t* occ_dn,
+                               const double* ci, double dt, double fbbound, int nmax,
+                               int det_chunk_size, int precision) {
+  AFQ_REQUIRE(ctx && out, "ctx/out is NULL");
+  AFQ_REQUIRE(M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && G >= 0 && nd > 0, "bad shape");
+  AFQ_REQUIRE(std::max(Na, Nb) <= afq::gj::GJ_NMAX, "multi-det: nocc > %d is not supported",
+              afq::gj::GJ_NMAX);
+  AFQ_REQUIRE(dt > 0 && nmax >= 0 && det_chunk_size > 0, "bad dt / nmax / det_chunk_size");
+  AFQ_REQUIRE(precision == AFQ_FP64 || precision == AFQ_BF16X3,
+              "multi-det: precision %d is not built (use FP64 or BF16X3)", precision);
+  AFQ_REQUIRE(chol && expH1 && mf && ci, "NULL system array");
+  hipStream_t s = ctx->stream;
+  int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
+  if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
+  if (rc) return rc;
+  afq_system* sy = new afq_system();
+  sy->ctx = ctx;
+  sy->M = M; sy->Na = Na; sy->Nb = Nb; sy->G = G;
+  sy->chol = chol; sy->expH1 = expH1; sy->mf_shift = mf;
+  sy->dt = dt; sy->fbbound = fbbound; sy->nmax = nmax; sy->precision = precision;
+  sy->ndet = nd;
+  sy->det_chunk = det_chunk_size;
+  if (hipMalloc(&sy->occ_a, sizeof(int) * (size_t)nd * std::max(Na, 1)) != hipSuccess ||
+      hipMalloc(&sy->occ_b, sizeof(int) * (size_t)nd * std::max(Nb, 1)) != hipSuccess ||
+      hipMalloc(&sy->ci, 16UL * nd) != hipSuccess) {
+    afq_system_destroy(sy);
+    return afq::fail(AFQ_ENOMEM, "multi-det trial arrays");
+  }
+  hipError_t e = hipSuccess;
+  if (Na) e = hipMemcpyAsync(sy->occ_a, occ_up, sizeof(int) * (size_t)nd * Na, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && Nb)
+    e = hipMemcpyAsync(sy->occ_b, occ_dn, sizeof(int) * (size_t)nd * Nb, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(sy->ci, ci, 16UL * nd, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) {
+    afq_system_destroy(sy);
+    return afq::hip_fail(e, "multi-det trial copy");
+  }
+  rc = afq::prepare_fp64(sy);
+  if (!rc) rc = afq::system_prepare_lowp(sy);
+  if (rc) {
+    afq_system_destroy(sy);
+    return rc;
+  }
+  *out = sy;
+  return AFQ_OK;
+}
+
+}  // extern "C"

@@ -4,6 +4,8 @@
 
 namespace afq {
 
+// FP64 operands of a system (packed L, identity-trial detection)
+int prepare_fp64(afq_system* s);
 int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* weights,
                    double* e_hybrid, double eshift, const double* xi, uint64_t seed, int64_t step,
                    double* ovlp_new);

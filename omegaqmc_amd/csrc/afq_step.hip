@@ -17,6 +17,7 @@
 #include "afq_gemm.h"
 #include "afq_kernels.h"
 #include "afq_mfma.h"
+#include "afq_multidet.h"
 #include "afq_step.h"
 
 struct Timing {
@@ -155,7 +156,7 @@ int prepare_fp64(afq_system* s) {
   s->eye_mask = 0;
   for (int sp = 0; sp < 2; ++sp) {
     const size_t n = 2 * (size_t)M * Ns[sp];
-    if (n == 0) continue;
+    if (n == 0 || !tr[sp]) continue;  // no single-det trial (multi-det system)
     double* h = new double[n];
     hipError_t e = hipMemcpyAsync(h, tr[sp], n * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -417,6 +418,10 @@ int afq_system_destroy(afq_system* s) {
     delete s->timing;
   }
   if (s->work) (void)hipFree(s->work);
+  if (s->md_work) (void)hipFree(s->md_work);
+  if (s->occ_a) (void)hipFree(s->occ_a);
+  if (s->occ_b) (void)hipFree(s->occ_b);
+  if (s->ci) (void)hipFree(s->ci);
   if (s->chol_packed) (void)hipFree(s->chol_packed);
   if (s->colmap) (void)hipFree(s->colmap);
   afq::system_release_lowp(s);
@@ -431,6 +436,9 @@ int afq_propagate_walkers(afq_system* sys, int W, double* phia, double* phib, do
   AFQ_REQUIRE(W >= 0, "bad walker count");
   if (W == 0) return AFQ_OK;
   AFQ_REQUIRE(phia && phib && weights && e_hybrid && ovlp_new, "NULL walker array");
+  if (sys->ndet > 0)
+    return afq::propagate_multidet(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
+                                   ovlp_new);
   if (sys->precision == AFQ_FP64)
     return afq::propagate_fp64(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
                                ovlp_new);
@@ -445,6 +453,8 @@ int afq_system_local_energy(afq_system* sys, int W, const double* phia, const do
   AFQ_REQUIRE(W >= 0, "bad walker count");
   AFQ_REQUIRE(e_tot && e_1b, "NULL output");
   if (W == 0) return AFQ_OK;
+  if (sys->ndet > 0)
+    return afq::system_energy_multidet(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
   if (sys->precision == AFQ_FP64)
     return afq::system_energy_fp64(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
   return afq::system_energy_lowp(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);

@@ -1,0 +1,173 @@
+"""Multi-determinant trial on the GPU against the NumPy oracle (oracle/multidet.py).
+
+Dets are occupation strings (the driver's construction).  FP64 parity
+1e-10 relative; BF16x3 step 1e-4.  The last test runs the multi-det step at
+the M=150, N=20+18, Nchol=800, 32-det size and checks finiteness and the
+walker-independent invariants.
+"""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from afq_cases import rel_err, walkers_near_trial
+from oracle import afqmc as O
+from oracle import multidet as MD
+from oracle import philox as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def afq(afq_lib):
+    import torch
+    import omegaqmc_amd as A
+    torch.cuda.set_device(0)
+    return A
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _n(t):
+    return t.detach().cpu().numpy()
+
+
+def _case(M, Na, Nb, G, ndet, W, seed, scale=0.3):
+    from omegaqmc_amd.synth import make_system
+    s = make_system(M, Na, Nb, G, seed=seed)
+    rng = np.random.default_rng(seed)
+    ups = list(itertools.combinations(range(min(M, Na + 3)), Na))
+    dns = list(itertools.combinations(range(min(M, Nb + 3)), Nb))
+    pairs = [(tuple(range(Na)), tuple(range(Nb)))]
+    while len(pairs) < ndet:
+        p = (ups[rng.integers(len(ups))], dns[rng.integers(len(dns))])
+        if p not in pairs:
+            pairs.append(p)
+    occ_up = np.array([p[0] for p in pairs], dtype=np.int32).reshape(ndet, Na)
+    occ_dn = np.array([p[1] for p in pairs], dtype=np.int32).reshape(ndet, Nb)
+    ci = np.exp(-np.arange(ndet) / 4.0) * (rng.normal(size=ndet) + 0.2j * rng.normal(size=ndet))
+    tu, td = MD.trials_from_occ(M, occ_up), MD.trials_from_occ(M, occ_dn)
+    pa = walkers_near_trial(np.eye(M, Na).astype(complex), W, rng, scale)
+    pb = walkers_near_trial(np.eye(M, Nb).astype(complex), W, rng, scale)
+    return s, occ_up, occ_dn, ci, tu, td, pa, pb
+
+
+CASES = [(12, 3, 2, 20, 7, 9, 1), (16, 5, 4, 30, 11, 6, 2), (10, 1, 1, 12, 4, 5, 3),
+         (20, 8, 6, 25, 6, 3, 4)]
+
+
+@pytest.mark.parametrize("M,Na,Nb,G,nd,W,seed", CASES)
+def test_greens_function_multidet(afq, M, Na, Nb, G, nd, W, seed):
+    s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
+    Ga, Gb, ov = afq.greens_function_multidet(_t(pa), _t(pb), _t(ou), _t(od), _t(ci), 3)
+    rGa, rGb, rov = MD.greens_function_multidet(pa, pb, tu, td, ci, 3)
+    assert rel_err(_n(Ga), rGa) < 1e-10 and rel_err(_n(Gb), rGb) < 1e-10
+    assert rel_err(_n(ov), rov) < 1e-10
+    ov2 = afq.greens_function_multidet_overlap_only(_t(pa), _t(pb), _t(tu), _t(td), _t(ci), 3)
+    assert rel_err(_n(ov2), MD.greens_function_multidet_overlap_only(pa, pb, tu, td, ci, 3)) < 1e-10
+
+
+def test_masked_dets_copies_of_det0(afq):
+    s, ou, od, ci, tu, td, _, _ = _case(14, 4, 3, 20, 6, 4, 9)
+    pa = np.repeat(tu[:1].astype(complex), 4, axis=0)
+    pb = np.repeat(td[:1].astype(complex), 4, axis=0)
+    Ga, Gb, ov = afq.greens_function_multidet(_t(pa), _t(pb), _t(ou), _t(od), _t(ci))
+    rGa, rGb, rov = MD.greens_function_multidet(pa, pb, tu, td, ci)
+    assert np.all(np.isfinite(_n(Ga))) and np.all(np.isfinite(_n(Gb)))
+    assert rel_err(_n(Ga), rGa) < 1e-12 and rel_err(_n(ov), rov) < 1e-12
+    e, _, _ = afq.local_energy_multidet(_t(s.h1e), _t(s.chol), _t(pa), _t(pb), _t(ou), _t(od),
+                                        _t(ci), s.enuc)
+    re, _, _ = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc)
+    assert np.all(np.isfinite(_n(e))) and rel_err(_n(e), re) < 1e-10
+
+
+@pytest.mark.parametrize("M,Na,Nb,G,nd,W,seed", CASES[:3])
+def test_local_energy_multidet(afq, M, Na, Nb, G, nd, W, seed):
+    s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
+    et, e1, e2 = afq.local_energy_multidet(_t(s.h1e), _t(s.chol), _t(pa), _t(pb), _t(ou), _t(od),
+                                           _t(ci), s.enuc, 4)
+    rt, r1, r2 = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc, 4)
+    assert rel_err(_n(e1), r1) < 1e-10 and rel_err(_n(e2), r2) < 1e-10
+    assert rel_err(_n(et), rt) < 1e-10
+
+
+def _propagator(afq, s, tu, td, ci, dt=0.01):
+    prop = MD.build_propagator_multidet(s.h1e_mod, s.chol, tu, td, ci, dt)
+    return prop, {'expH1': _t(prop['expH1'].real.copy()), 'mf_shift': _t(prop['mf_shift']),
+                  'dt': dt}
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("M,Na,Nb,G,nd,W,seed", CASES[:2])
+def test_propagate_walkers_multidet(afq, precision, tol, M, Na, Nb, G, nd, W, seed):
+    s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    rng = np.random.default_rng(seed + 100)
+    w = rng.uniform(0.5, 1.5, W)
+    eh = rng.normal(size=W) + 0.1j * rng.normal(size=W)
+    xi = P.aux_fields(7, 1, W, G)
+    ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -3.0, xi)
+    out = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
+                                         _t(ou), _t(od), _t(ci), -3.0, None, xi=_t(xi),
+                                         precision=precision)
+    # e_hybrid = -(log ratio + cfb + cmf) / dt amplifies the walkers' error by 1/dt
+    for k in (0, 1, 2, 3, 4):
+        assert rel_err(_n(out[k]), ref[k]) < (tol if k < 4 else 10 * tol), k
+
+
+def test_multidet_system_energy(afq):
+    import omegaqmc_amd as A
+    s, ou, od, ci, tu, td, pa, pb = _case(16, 5, 4, 30, 11, 6, 2)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    sysobj = A.MultiDetSystem(propd, _t(s.chol), _t(ou), _t(od), _t(ci), det_chunk_size=5)
+    et, e1, e2 = sysobj.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc)
+    rt, r1, r2 = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc)
+    assert rel_err(_n(et), rt) < 1e-10 and rel_err(_n(e2), r2) < 1e-10
+
+
+def test_multidet_step_at_full_size(afq):
+    """M=150, N=20+18, Nchol=800, 32 dets, 128 walkers: 3 steps from the
+    dominant det; overlaps and weights finite, the multi-det overlap equals
+    the overlap-only entry point, and the propagated walkers' local energy is
+    finite."""
+    import torch
+    import omegaqmc_amd as A
+    from omegaqmc_amd.synth import make_system
+    M, Na, Nb, G, nd, W = 150, 20, 18, 800, 32, 128
+    s = make_system(M, Na, Nb, G, seed=5, normalise=True, rotate_device="cuda")
+    # aufbau det + singles/doubles near the Fermi level (deterministic list)
+    dets = [(tuple(range(Na)), tuple(range(Nb)))]
+    for i in range(Na - 4, Na):
+        for a in range(Na, Na + 4):
+            dets.append((tuple(sorted(set(range(Na)) - {i} | {a})), tuple(range(Nb))))
+    for j in range(Nb - 3, Nb):
+        for b in range(Nb, Nb + 5):
+            dets.append((tuple(range(Na)), tuple(sorted(set(range(Nb)) - {j} | {b}))))
+    dets = dets[:nd]
+    ou = np.array([d[0] for d in dets], dtype=np.int32)
+    od = np.array([d[1] for d in dets], dtype=np.int32)
+    rng = np.random.default_rng(0)
+    ci = np.exp(-np.arange(nd) / 4.0) * rng.normal(size=nd)
+    ci[0] = 1.0
+    ci = ci / np.linalg.norm(ci) + 0j
+    tu, td = MD.trials_from_occ(M, ou), MD.trials_from_occ(M, od)
+    Gc = _t(MD.g_charge(tu, td, ci))
+    prop = A.build_propagator(_t(s.h1e_mod), _t(s.chol), _t(tu[0]), _t(td[0]), 0.005,
+                              G_charge=Gc)
+    sysobj = A.MultiDetSystem(prop, _t(s.chol), _t(ou), _t(od), _t(ci))
+    pa = sysobj.trial_up.unsqueeze(0).repeat(W, 1, 1).contiguous()
+    pb = sysobj.trial_dn.unsqueeze(0).repeat(W, 1, 1).contiguous()
+    w = torch.ones(W, dtype=torch.float64, device="cuda")
+    eh = torch.zeros(W, dtype=torch.complex128, device="cuda")
+    for step in range(1, 4):
+        ov = sysobj.step(pa, pb, w, eh, -250.0, key=A.PhiloxKey(42, step))
+    torch.cuda.synchronize()
+    assert torch.all(torch.isfinite(ov)) and torch.all(torch.isfinite(w))
+    ov2 = A.greens_function_multidet_overlap_only(pa, pb, _t(ou), _t(od), _t(ci))
+    assert rel_err(_n(ov2), _n(ov)) < 1e-12
+    et, _, _ = sysobj.local_energy(pa, pb, _t(s.h1e), s.enuc)
+    assert torch.all(torch.isfinite(et))
