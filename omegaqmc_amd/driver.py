@@ -19,16 +19,17 @@ import torch
 
 from . import _lib
 from ._lib import check, ptr
-from .afqmc_gto import (C128, F64, PhiloxKey, System, WEIGHT_CLIP_FRACTION, _r,
+from .afqmc_gto import (C128, F64, MultiDetSystem, PhiloxKey, System, WEIGHT_CLIP_FRACTION, _r,
                         build_propagator, determinant_energy_from_integrals, do_binning_analysis,
-                        half_rotate_cholesky, prepare_afqmc_integrals_from_arrays)
+                        half_rotate_cholesky, multidet_charge_density,
+                        prepare_afqmc_integrals_from_arrays)
 
 
 class AFQMCDriver:
     """Set up once (integrals, trial, propagator), run many times."""
 
     def __init__(self, integrals, dt=0.005, e_hf=None, eshift0=None, verbose=True,
-                 precision="fp64", fbbound=None, exp_nmax=6):
+                 precision="fp64", fbbound=None, exp_nmax=6, trial=None, det_chunk_size=5):
         self.dt = float(dt)
         self.verbose = verbose
         self.precision = precision
@@ -45,17 +46,43 @@ class AFQMCDriver:
                                                      self.ndown)
         self.ehf = float(e_hf)
         dev = self.chol.device
-        self.trial_up = torch.eye(self.nbasis, self.nup, dtype=F64, device=dev)
-        self.trial_dn = torch.eye(self.nbasis, self.ndown, dtype=F64, device=dev)
-        self.rchol_a, self.rchol_b = half_rotate_cholesky(self.chol, self.trial_up, self.trial_dn)
-        self.propagator = build_propagator(self.h1e_mod, self.chol, self.trial_up, self.trial_dn,
-                                           self.dt)
+        self.multidet = trial is not None
+        if self.multidet:
+            # afqmc_gto.py:1163-1225: dets from occupation strings, the
+            # dominant det initialises the walkers, the propagator uses the
+            # |c|^2-weighted (or the given) charge density
+            eye = torch.eye(self.nbasis, dtype=F64, device=dev)
+            occ_up = torch.as_tensor(np.asarray(trial['occ_up']), dtype=torch.long, device=dev)
+            occ_dn = torch.as_tensor(np.asarray(trial['occ_dn']), dtype=torch.long, device=dev)
+            self.trials_up = eye[:, occ_up].permute(1, 0, 2).contiguous()
+            self.trials_dn = eye[:, occ_dn].permute(1, 0, 2).contiguous()
+            self.ci_coeffs = torch.as_tensor(np.asarray(trial['ci_coeffs']), device=dev).to(C128)
+            self.trial_up, self.trial_dn = self.trials_up[0], self.trials_dn[0]
+            if trial.get('rdm1') is not None:
+                Gc = torch.as_tensor(np.asarray(trial['rdm1']), device=dev).to(C128)
+            else:
+                Gc = multidet_charge_density(self.trials_up, self.trials_dn, self.ci_coeffs)
+            self.propagator = build_propagator(self.h1e_mod, self.chol, self.trial_up,
+                                               self.trial_dn, self.dt, G_charge=Gc)
+        else:
+            self.trial_up = torch.eye(self.nbasis, self.nup, dtype=F64, device=dev)
+            self.trial_dn = torch.eye(self.nbasis, self.ndown, dtype=F64, device=dev)
+            self.rchol_a, self.rchol_b = half_rotate_cholesky(self.chol, self.trial_up,
+                                                              self.trial_dn)
+            self.propagator = build_propagator(self.h1e_mod, self.chol, self.trial_up,
+                                               self.trial_dn, self.dt)
         mf = self.propagator['mf_shift']
         self.e_const = float(self.enuc + 0.5 * torch.sum(mf * mf).real)
         self.eshift0 = eshift0
-        self.system = System(self.propagator, self.chol, self.rchol_a, self.rchol_b,
-                             self.trial_up, self.trial_dn, fbbound=fbbound, exp_nmax=exp_nmax,
-                             precision=precision)
+        if self.multidet:
+            self.system = MultiDetSystem(self.propagator, self.chol, self.trials_up,
+                                         self.trials_dn, self.ci_coeffs, fbbound=fbbound,
+                                         exp_nmax=exp_nmax, det_chunk_size=det_chunk_size,
+                                         precision=precision)
+        else:
+            self.system = System(self.propagator, self.chol, self.rchol_a, self.rchol_b,
+                                 self.trial_up, self.trial_dn, fbbound=fbbound,
+                                 exp_nmax=exp_nmax, precision=precision)
 
     # -- buffers ------------------------------------------------------------
     def init_walkers(self, W):
@@ -190,9 +217,13 @@ def get_afqmc_func_from_integrals(h1, chol, ecore, nelec, mo_coeff=None, e_hf=No
                                   trial=None, eshift0=None, verbose=True, chol_chunk_g=128,
                                   det_chunk_size=5, multidet_method='explicit',
                                   precision="fp64"):
-    """afqmc_gto.py:1592-1631 (single-determinant trial)."""
-    if trial is not None:
-        raise NotImplementedError("multi-determinant trials: use omegaqmc_amd.multidet")
+    """afqmc_gto.py:1592-1631.  ``trial`` (multi-det) is a dict with 'ci_coeffs',
+    'occ_up', 'occ_dn' (and optionally 'rdm1'); only the explicit multi-det
+    method is built (``multidet_method='explicit'``)."""
+    if trial is not None and multidet_method != 'explicit':
+        raise NotImplementedError("only multidet_method='explicit' is built")
+    if trial is not None and trial.get('mo_coeff') is not None:
+        raise NotImplementedError("rotate the integrals into the trial basis (mo_coeff=None)")
     integrals = prepare_afqmc_integrals_from_arrays(h1, chol, ecore, nelec, mo_coeff=mo_coeff)
     return AFQMCDriver(integrals, dt=dt, e_hf=e_hf, eshift0=eshift0, verbose=verbose,
-                       precision=precision)
+                       precision=precision, trial=trial, det_chunk_size=det_chunk_size)

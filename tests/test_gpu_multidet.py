@@ -171,3 +171,44 @@ def test_multidet_step_at_full_size(afq):
     assert rel_err(_n(ov2), _n(ov)) < 1e-12
     et, _, _ = sysobj.local_energy(pa, pb, _t(s.h1e), s.enuc)
     assert torch.all(torch.isfinite(et))
+
+
+def test_multidet_driver_trajectory_matches_oracle(afq):
+    """12 driver steps (QR every 5, comb every 10) with a 7-det trial: walkers,
+    weights and hybrid energies follow the oracle (same fields and comb
+    offsets); the block energy matches local_energy_multidet's estimator."""
+    import torch
+    from omegaqmc_amd.driver import AFQMCDriver
+    M, Na, Nb, G, nd, W, seed = 12, 3, 2, 20, 7, 16, 1
+    s, ou, od, ci, tu, td, _, _ = _case(M, Na, Nb, G, nd, W, seed)
+    integ = {'h1e': _t(s.h1e), 'h1e_mod': _t(s.h1e_mod), 'chol': _t(s.chol), 'enuc': s.enuc,
+             'nbasis': M, 'nup': Na, 'ndown': Nb}
+    gd = AFQMCDriver(integ, verbose=False, dt=0.01,
+                     trial={'ci_coeffs': ci, 'occ_up': ou, 'occ_dn': od}, det_chunk_size=3)
+    prop = MD.build_propagator_multidet(s.h1e_mod, s.chol, tu, td, ci, 0.01)
+    st = gd.init_walkers(W)
+    pa = np.tile(tu[0][None].astype(complex), (W, 1, 1))
+    pb = np.tile(td[0][None].astype(complex), (W, 1, 1))
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    eshift = -1.1
+    for step in range(1, 13):
+        if step % 5 == 0:
+            pa, pb, _, _ = O.orthogonalize_walkers(pa, pb)
+        xi = P.aux_fields(42, step, W, G)
+        pa, pb, w, ov, eh = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu,
+                                                          td, ci, eshift, xi, det_chunk_size=3)
+        if step > 1:
+            w = np.clip(w, 0.0, np.sum(np.abs(w)) * O.WEIGHT_CLIP_FRACTION)
+        if step % 10 == 0:
+            w, pa, pb, eh, _ = O.population_control_comb(w, pa, pb, P.comb_offset(42, step), eh)
+        gd.step(st, step, 42, eshift, 5, 10)
+        for x, y in ((st['phia'], pa), (st['phib'], pb), (st['weights'], w),
+                     (st['e_hybrid'], eh)):
+            assert rel_err(_n(x), y) < 1e-9, step
+    e = gd.block_energy(st)
+    torch.cuda.synchronize()
+    re, _, _ = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc, 3)
+    assert rel_err(_n(e), re) < 1e-9
+    eb = float(st['stats'][0].item())
+    np.testing.assert_allclose(eb, np.sum(w * re.real) / np.sum(w), rtol=1e-9)
