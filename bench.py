@@ -197,6 +197,46 @@ def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10):
     return worst, trace
 
 
+def run_multidet(W, steps, warmup, precision, seed=42):
+    """The multi-determinant workload: M=150, N=20+18, Nchol=800, 32 dets."""
+    import torch
+    from omegaqmc_amd.driver import AFQMCDriver
+    from omegaqmc_amd.synth import make_system, multidet_trial
+    M, Na, Nb, G = 150, 20, 18, 800
+    s = make_system(M, Na, Nb, G, seed=seed, rotate_device="cuda", normalise=True)
+    dev = torch.device("cuda", 0)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    integ = {'h1e': t(s.h1e), 'h1e_mod': t(s.h1e_mod), 'chol': t(s.chol), 'enuc': s.enuc,
+             'nbasis': M, 'nup': Na, 'ndown': Nb}
+    drv = AFQMCDriver(integ, verbose=False, precision=precision,
+                      trial=multidet_trial(M, Na, Nb, 32))
+    st = drv.init_walkers(W)
+    eshift = drv.ehf - drv.e_const
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        drv.step(st, step, seed, eshift, 5, 10)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step += 1
+        drv.step(st, step, seed, eshift, 5, 10)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    drv.block_energy(st)
+    torch.cuda.synchronize()
+    e0.record()
+    drv.block_energy(st)
+    e1.record()
+    torch.cuda.synchronize()
+    return {"walker_steps_per_s": round(W / (ms * 1e-3), 2), "ms_per_step": round(ms, 4),
+            "block_energy_ms": round(e0.elapsed_time(e1), 2),
+            "e_block": float(st['stats'][0].item())}
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling HBM byte counters."""
     import torch
@@ -245,6 +285,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-multidet", action="store_true",
+                    help="skip the 32-determinant M=150 workload")
     ap.add_argument("--acc-steps", type=int, default=30,
                     help="steps of the same-trajectory energy comparison")
     args = ap.parse_args()
@@ -319,6 +361,19 @@ def main():
                 "frac": round(ach / peak, 4), "traffic": traffic,
                 "ms_per_launch": round(per_launch_ms, 4), "flops_per_launch": flops,
                 "symbol": sym}
+
+    if not args.no_multidet:
+        md = {"workload": "multi-det trial: M=150 N=20+18 Nchol=800 32 dets W=1024 dt=0.005"}
+        for prec in ("fp64", "bf16x3"):
+            print(f"[bench] multi-det {prec} step", file=sys.stderr, flush=True)
+            try:
+                r = run_multidet(W, max(3, args.steps // 2), 1, prec)
+            except _lib.AFQError as e:
+                md[f"{prec}_error"] = str(e)[:200]
+                continue
+            md[prec] = r
+            torch.cuda.empty_cache()
+        extra["multidet"] = md
 
     cpu = None
     if not args.no_cpu:

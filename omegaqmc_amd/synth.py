@@ -159,6 +159,38 @@ CONFIGS = {
 }
 
 
+def multidet_trial(nbasis: int, nup: int, ndown: int, ndet: int = 32, seed: int = 11):
+    """Multi-determinant trial of the M=150 workload: the aufbau determinant,
+    then alpha singles, beta singles and alpha-beta doubles near the Fermi
+    level (a deterministic list in the h1 eigenbasis), coefficients
+    c_k = N(0, 1) exp(-k/4) with c_0 = 1, normalised.  Returns
+    {'ndet', 'ci_coeffs', 'occ_up', 'occ_dn'} as the reference driver takes."""
+    aufa, aufb = list(range(nup)), list(range(ndown))
+    dets = [(tuple(aufa), tuple(aufb))]
+
+    def single(occ, n, i, a):
+        return tuple(sorted(set(occ) - {i} | {a}))
+
+    k = 3
+    sa = [(single(aufa, nup, i, a), tuple(aufb)) for i in range(nup - 1, max(-1, nup - 1 - k), -1)
+          for a in range(nup, min(nbasis, nup + k))]
+    sb = [(tuple(aufa), single(aufb, ndown, j, b)) for j in range(ndown - 1, max(-1, ndown - 1 - k), -1)
+          for b in range(ndown, min(nbasis, ndown + k))]
+    dab = [(x[0], y[1]) for x in sa for y in sb]
+    for d in sa + sb + dab:
+        if len(dets) >= ndet:
+            break
+        if d not in dets:
+            dets.append(d)
+    rng = np.random.default_rng(seed)
+    ci = rng.normal(size=len(dets)) * np.exp(-np.arange(len(dets)) / 4.0)
+    ci[0] = 1.0
+    ci = ci / np.linalg.norm(ci)
+    return {'ndet': len(dets), 'ci_coeffs': ci.astype(np.complex128),
+            'occ_up': np.array([d[0] for d in dets], dtype=np.int32),
+            'occ_dn': np.array([d[1] for d in dets], dtype=np.int32)}
+
+
 def uneven_batch_specs(nsys: int = 64, seed: int = 7):
     """Systems of the uneven batch: M ~ U{40..200}, N = M // 8, G = 5 M,
     walkers from {128, 256, 512, 1024}."""
