@@ -336,27 +336,40 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   };
   // acc = E . T (E real, planes [2][16 RT][Kp]); row tiles this wave does not
   // own read a valid row and are not used
-  auto onebody = [&]() {
-    zero();
-    for (int ks = 0; ks < Kp / 32; ++ks) {
-      uint4 eh[MT], el[MT];
+  auto ldE = [&](uint4* eh, uint4* el, int ks) {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      const long r = 16 * min(wv + PW * u, RT - 1) + li;
+      eh[u] = gld(E + r * Kp + 32 * ks + 8 * lq);
+      el[u] = gld(E + Eplane + r * Kp + 32 * ks + 8 * lq);
+    }
+  };
+  auto estep = [&](const uint4* eh, const uint4* el, int ks) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      uint4 b[4];
+      bfrag(x, ks, b);
 #pragma unroll
       for (int u = 0; u < MT; ++u) {
-        const long r = 16 * min(wv + PW * u, RT - 1) + li;
-        eh[u] = gld(E + r * Kp + 32 * ks + 8 * lq);
-        el[u] = gld(E + Eplane + r * Kp + 32 * ks + 8 * lq);
+        if (!owns(u)) continue;
+        are[u][x] = mfma3(eh[u], el[u], b[0], b[1], are[u][x]);
+        aim[u][x] = mfma3(eh[u], el[u], b[2], b[3], aim[u][x]);
       }
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        uint4 b[4];
-        bfrag(x, ks, b);
-#pragma unroll
-        for (int u = 0; u < MT; ++u) {
-          if (!owns(u)) continue;
-          are[u][x] = mfma3(eh[u], el[u], b[0], b[1], are[u][x]);
-          aim[u][x] = mfma3(eh[u], el[u], b[2], b[3], aim[u][x]);
-        }
-      }
+    }
+  };
+  // k-steps in pairs with two static fragment buffers: the next k-step's
+  // expH1 rows are in flight while this one runs
+  auto onebody = [&]() {
+    zero();
+    const int nks = Kp / 32;
+    uint4 e0h[MT], e0l[MT], e1h[MT], e1l[MT];
+    ldE(e0h, e0l, 0);
+    for (int ks = 0; ks < nks; ks += 2) {
+      ldE(e1h, e1l, ks + 1 < nks ? ks + 1 : nks - 1);
+      estep(e0h, e0l, ks);
+      if (ks + 1 >= nks) break;
+      ldE(e0h, e0l, ks + 2 < nks ? ks + 2 : nks - 1);
+      estep(e1h, e1l, ks + 1);
     }
   };
   // acc = V . T (V complex, block-major planes).  VHS fragments of row tile u
@@ -386,14 +399,20 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   // (row tile, k-step); the k-steps go in pairs so that, with the row-tile
   // loop unrolled, every buffer choice is static and no loaded register is
   // copied (a copy would wait for the load).
+  // Row tiles are walked one at a time with the fragments of the tile after
+  // next in flight.  Three fragment buffers rotate by sub-step (row tile,
+  // k-step); the k-steps go in threes so that, with the row-tile loop
+  // unrolled, every buffer choice is static and no loaded register is copied
+  // (a copy would wait for the load).
   auto term = [&]() {
     zero();
     const int nks = Kp / 32;
-    uint4 v0[4], v1[4];
-    ldv(v0, 0, 0);
-    for (int ks = 0; ks < nks; ks += 2) {
+    uint4 vb[3][4];
+    ldv(vb[0], 0, 0);
+    ldv(vb[1], 1 % MT, (1 / MT) < nks ? 1 / MT : nks - 1);
+    for (int ks = 0; ks < nks; ks += 3) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < 3; ++h) {
         const int kk = ks + h;
         const bool live = kk < nks;  // uniform
         uint4 b[3][4];
@@ -404,15 +423,10 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int sidx = h * MT + u;
-          const int un = (u + 1 < MT) ? u + 1 : 0;
-          const int kn = (u + 1 < MT) ? kk : kk + 1;
-          if ((sidx & 1) == 0) {
-            ldv(v1, un, kn < nks ? kn : nks - 1);
-            if (live) vstep(v0, b, u);
-          } else {
-            ldv(v0, un, kn < nks ? kn : nks - 1);
-            if (live) vstep(v1, b, u);
-          }
+          const int t2 = sidx + 2;
+          const int kn = ks + t2 / MT;
+          ldv(vb[t2 % 3], t2 % MT, kn < nks ? kn : nks - 1);
+          if (live) vstep(vb[sidx % 3], b, u);
         }
       }
     }
