@@ -170,6 +170,24 @@ __global__ void k_md_gt(long n, int M, const cplx* __restrict__ Ga, const cplx* 
   }
 }
 
+// Gpk[w][c] = Gs[p][q] + Gs[q][p] (p < q) or Gs[p][p], Gs = Ga + Gb, over the
+// packed upper triangle (colmap[c] = (p M + q, q M + p or -1)): with L
+// symmetric, sum_pq L_pq Gs_qp = sum_c Lpacked_c Gpk_c.
+__global__ void k_md_gpack(long n, int M, int P, const int2* __restrict__ colmap,
+                           const cplx* __restrict__ Ga, const cplx* __restrict__ Gb,
+                           cplx* __restrict__ Gpk) {
+  const long MM = (long)M * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long w = e / P;
+    const int c = (int)(e % P);
+    const int2 cm = colmap[c];
+    cplx v = cadd(Ga[w * MM + cm.x], Gb[w * MM + cm.x]);
+    if (cm.y >= 0) v = cadd(v, cadd(Ga[w * MM + cm.y], Gb[w * MM + cm.y]));
+    Gpk[e] = v;
+  }
+}
+
 // rchol_I[g][i][q] = L[g][occ_I[i]][q]
 __global__ void k_md_rchol(long n, int M, int N, const int* __restrict__ occ,
                            const double* __restrict__ L, double* __restrict__ out) {
@@ -377,7 +395,7 @@ size_t md_step_bytes(const afq_system* s, int W) {
   const long M = s->M, G = s->G;
   size_t n = md_bytes(W, s->M, s->Na, s->Nb, s->ndet, true, true);
   n += a256(16UL * W);                   // oo
-  n += a256(16UL * W * G);               // vbias
+  n += a256(16UL * W * G * FB_MAX_SPLITS);  // vbias (split-K partials)
   n += a256(16UL * W * G);               // xs
   n += 2 * a256(16UL * W);               // cfb, cmf
   n += a256(8UL * W * G);                // xi
@@ -414,7 +432,7 @@ int md_ensure(afq_system* s, int W, MdStep* st, MdBufs* ovb) {
     return (double*)r;
   };
   st->oo = take(16UL * C);
-  st->vbias = take(16UL * C * G);
+  st->vbias = take(16UL * C * G * FB_MAX_SPLITS);
   st->xs = take(16UL * C * G);
   st->cfb = take(16UL * C);
   st->cmf = take(16UL * C);
@@ -452,9 +470,31 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
                  true, true, st.b);
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
-  // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp as one GEMM over (p, q)
+  // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp: over the packed triangle on the
+  //    FP64 MFMA force-bias kernel (split-K) when L is symmetric, else one
+  //    generic GEMM over (p, q)
   phase_begin(sys, PH_FORCEBIAS);
-  {
+  int nsplit = 1;
+  if (sys->chol_packed) {
+    const int P = sys->packed_P;
+    const long n = (long)W * P;
+    hipLaunchKernelGGL(k_md_gpack, dim3(4096), dim3(256), 0, s, n, M, P, sys->colmap,
+                       (const cplx*)st.b.Ga, (const cplx*)st.b.Gb, (cplx*)st.Gt);
+    AFQ_CHECK_LAUNCH();
+    FbSplits sp{};
+    const int nsp = 4;
+    const long chunk = (((long)P + nsp - 1) / nsp + 15) / 16 * 16;
+    for (long k0 = 0; k0 < P && sp.n < FB_MAX_SPLITS; k0 += chunk) {
+      sp.gh[sp.n] = (const cplx*)st.Gt;
+      sp.lh[sp.n] = sys->chol_packed;
+      sp.K[sp.n] = P;
+      sp.kbeg[sp.n] = (int)k0;
+      sp.kend[sp.n] = (int)std::min<long>(P, k0 + chunk);
+      ++sp.n;
+    }
+    if ((rc = fb_mfma_f64(s, W, G, sp, st.vbias))) return rc;
+    nsplit = sp.n;
+  } else {
     const long n = (long)W * M * M;
     hipLaunchKernelGGL(k_md_gt, dim3(4096), dim3(256), 0, s, n, M, (const cplx*)st.b.Ga,
                        (const cplx*)st.b.Gb, (cplx*)st.Gt);
@@ -467,6 +507,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     if ((rc = gemm_fp64(s, g))) return rc;
   }
   phase_end(sys, PH_FORCEBIAS);
+  if (nsplit < 1) nsplit = 1;
   // 4-6: shift, VHS, expH1 . exp(VHS) . expH1 with the single-det kernels
   if (sys->precision == AFQ_BF16X3) {
     bf16_t* X = nullptr;
@@ -475,7 +516,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     const Bf3System& b = *(const Bf3System*)sys->chol_lp;
     phase_begin(sys, PH_SHIFT);
     rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr,
-                      nullptr, st.cfb, st.cmf, 1, nullptr, X, b.Gp);
+                      nullptr, st.cfb, st.cmf, nsplit, nullptr, X, b.Gp);
     phase_end(sys, PH_SHIFT);
     if (rc) return rc;
     phase_begin(sys, PH_VHS);
@@ -489,7 +530,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   } else {
     phase_begin(sys, PH_SHIFT);
     rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr, st.xs,
-                      st.cfb, st.cmf, 1);
+                      st.cfb, st.cmf, nsplit);
     phase_end(sys, PH_SHIFT);
     if (rc) return rc;
     phase_begin(sys, PH_VHS);
