@@ -217,6 +217,12 @@ def run_multidet(W, steps, warmup, precision, seed=42):
         step += 1
         drv.step(st, step, seed, eshift, 5, 10)
     torch.cuda.synchronize()
+    from omegaqmc_amd import _lib
+    L = _lib.load()
+    _lib.check(L.afq_system_timing_enable(drv.system.handle, 1), "timing")
+    tot = (_lib.ctypes.c_double * 16)()
+    cnt = (_lib.ctypes.c_int64 * 16)()
+    L.afq_system_timing_read(drv.system.handle, tot, cnt, 16, 1)  # reset
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -226,6 +232,9 @@ def run_multidet(W, steps, warmup, precision, seed=42):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
+    _lib.check(L.afq_system_timing_read(drv.system.handle, tot, cnt, 16, 1), "timing")
+    phases = {name: round(tot[i] / steps, 4) for i, name in enumerate(_lib.PHASES) if cnt[i]}
+    L.afq_system_timing_enable(drv.system.handle, 0)
     drv.block_energy(st)
     torch.cuda.synchronize()
     e0.record()
@@ -234,7 +243,7 @@ def run_multidet(W, steps, warmup, precision, seed=42):
     torch.cuda.synchronize()
     return {"walker_steps_per_s": round(W / (ms * 1e-3), 2), "ms_per_step": round(ms, 4),
             "block_energy_ms": round(e0.elapsed_time(e1), 2),
-            "e_block": float(st['stats'][0].item())}
+            "phases_ms_per_step": phases, "e_block": float(st['stats'][0].item())}
 
 
 def copy_calibration(mib):

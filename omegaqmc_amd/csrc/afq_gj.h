@@ -34,7 +34,8 @@ constexpr int GJ_WAVE_LDS = (GJ_NMAX * GJ_LD + GJ_NMAX) * 16 + 64 * 4;
 // getrf) and eliminates column k from every other row in place (in-place
 // Gauss-Jordan inversion with the row permutation kept implicit): afterwards
 // row p_k holds row k of S^-1, whose column j sits in the storage column of
-// column p_j.  det S = sign(p) prod_k d_k with d_k the pivots.
+// column p_j.  det S = sign(p) prod_k d_k with d_k the pivots, accumulated as
+// a mantissa-exponent pair so that no partial product over- or underflows.
 //
 // The row is kept rotated so that the active column is always register 0:
 // the update of step k writes column j+1 into register j, and the new column
@@ -49,10 +50,11 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
   const int l = threadIdx.x & 63;
   bool active = l < N;
   mystep = 0;
-  double logabs = 0.0;
-  cplx ph = mkc(1.0, 0.0);
+  // det = sign(p) * pr * 2^e2, pr kept in [0.5, 1) in its larger component
+  cplx pr = mkc(1.0, 0.0);
+  int e2 = 0;
   int parity = 0;
-  bool sing = false;
+  bool sing = false;  // an exact zero pivot: det 0 (slogdet sign 0) whatever follows
   for (int k = 0; k < N; ++k) {
     const cplx f = a[0];
     double v = -1.0;
@@ -82,12 +84,13 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
     parity ^= __popcll(am & ((1ull << p) - 1ull)) & 1;
     wave_sync();
     const cplx d = bro[0];
-    const double ad = hypot(d.x, d.y);
-    if (ad == 0.0) {
-      sing = true;
-    } else {
-      logabs += log(ad);
-      ph = cmul(ph, mkc(d.x / ad, d.y / ad));
+    sing = sing || (d.x == 0.0 && d.y == 0.0);
+    pr = cmul(pr, d);
+    {
+      int e;
+      (void)frexp(fmax(fabs(pr.x), fabs(pr.y)), &e);
+      pr = mkc(ldexp(pr.x, -e), ldexp(pr.y, -e));
+      e2 += e;
     }
     const cplx id = cdiv(mkc(1.0, 0.0), d);
     const cplx g = me ? mkc(1.0 - id.x, -id.y) : cmul(f, id);
@@ -102,11 +105,8 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
     wave_sync();
   }
   if (l == 0) {
-    if (sing) *ov = mkc(0.0, 0.0);
-    else {
-      const cplx r = cscale(ph, exp(logabs));
-      *ov = parity ? mkc(-r.x, -r.y) : r;
-    }
+    const cplx r = sing ? mkc(0.0, 0.0) : mkc(ldexp(pr.x, e2), ldexp(pr.y, e2));
+    *ov = parity ? mkc(-r.x, -r.y) : r;
   }
 }
 
@@ -185,19 +185,19 @@ __device__ __forceinline__ void ghalf_mfma_wave(const cplx* Inv, int LD, int N, 
   }
 }
 
-// Scatter the Gauss-Jordan result (registers, rotated N times) into S^-1 in
-// LDS: row p_k of the result is row k of S^-1; storage column c (now in
-// register (c - N) mod NB) is column p_c.
+// Scatter the Gauss-Jordan result (registers, rotated N times) into S^-1
+// (row stride ld, in LDS or global memory): row p_k of the result is row k of
+// S^-1; storage column c (now in register (c - N) mod NB) is column p_c.
 template <int NB>
 __device__ __forceinline__ void gj_store_inverse(const cplx (&a)[NB], int N, int mystep,
-                                                 const int* order, cplx* Inv) {
+                                                 const int* order, cplx* Inv, int ld = GJ_LD) {
   const int l = threadIdx.x & 63;
   if (l < N) {
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
       int c = q + N;
       if (c >= NB) c -= NB;
-      if (c < N) Inv[mystep * GJ_LD + order[c]] = a[q];
+      if (c < N) Inv[(long)mystep * ld + order[c]] = a[q];
     }
   }
   wave_sync();

@@ -4,11 +4,24 @@
 
 namespace afq {
 
+// What one multi-determinant evaluation forms besides the overlaps:
+//   MD_OVERLAP  overlaps only;
+//   MD_INVERSE  every S_I^-1 and the scattered sum B_s = sum_I z_I T_I^* S_I^-1
+//               / overlap, from which G_s = B_s phi_s^T (one GEMM per spin);
+//   MD_GHALF    every Ghalf_I (the local energy needs them) and G.
+enum MdKind { MD_OVERLAP = 0, MD_INVERSE = 1, MD_GHALF = 2 };
+
 // Scratch of one multi-determinant evaluation for W walkers (bytes), and the
 // pieces carved out of it.
 struct MdBufs {
   double* gha = nullptr;   // Ghalf_a of every det: [I][w][Na][M] complex
   double* ghb = nullptr;   // Ghalf_b: [I][w][Nb][M]
+  double* inva = nullptr;  // S_Ia^-1 of every det: [I][w][Na][Na] complex
+  double* invb = nullptr;  // [I][w][Nb][Nb]
+  double* Ba = nullptr;    // B_a: [w][M][Na] complex
+  double* Bb = nullptr;    // [w][M][Nb]
+  int* ioa = nullptr;      // row of orbital p in det I, or -1: [I][M]
+  int* iob = nullptr;
   double* oda = nullptr;   // per-det overlaps [w][I] complex
   double* odb = nullptr;
   double* z = nullptr;     // masked weights c_I^* o_Ia o_Ib: [w][I] complex (0 if masked)
@@ -17,15 +30,16 @@ struct MdBufs {
   double* Ga = nullptr;    // (W, M, M) complex
   double* Gb = nullptr;
 };
-size_t md_bytes(int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull);
-MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull);
+size_t md_bytes(int W, int M, int Na, int Nb, int nd, int kind, bool gfull);
+MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, int kind, bool gfull);
 
-// Per-det overlaps (and with ghalf, every Ghalf_I), the masked weights and
-// the multi-det overlap; with gfull also Ga, Gb (greens_th4.py:75-148).
-// occ_* are device int32 (nd, N_s) with entries in [0, M).
+// Per-det overlaps, the masked weights and the multi-det overlap, plus what
+// `kind` names (greens_th4.py:75-148).  With kind != MD_OVERLAP and b.Ga set,
+// G is written: Ga and Gb, or with gsum their sum into Ga alone.  occ_* are
+// device int32 (nd, N_s) with entries in [0, M).
 int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, const double* phia,
-              const double* phib, const int* occa, const int* occb, const double* ci,
-              bool ghalf, bool gfull, const MdBufs& b);
+              const double* phib, const int* occa, const int* occb, const double* ci, int kind,
+              bool gsum, const MdBufs& b);
 
 // Host check of occupation strings (device arrays): entries in [0, M), distinct per det.
 int md_check_occ(hipStream_t s, int nd, int N, int M, const int32_t* occ);

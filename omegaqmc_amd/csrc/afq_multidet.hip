@@ -45,15 +45,23 @@ using gj::GJ_WAVE_LDS;
 constexpr double MASK_RTOL = 1e-14;   // greens_th4.py:24
 constexpr double MASK_ATOL = 1e-300;  // greens_th4.py:25
 
-// Grid W * nd (det fastest), two wavefronts: wave s = spin s.
+// LDS of one wavefront of k_md_gj: the pivot row (NB complex), the pivot
+// order (64 ints) and, in mode 1, S^-1 (N x GJ_LD complex).
+size_t md_wave_lds(int mode, int NB, int N) {
+  return 16UL * NB + 256 + (mode == 1 ? 16UL * N * GJ_LD : 0);
+}
+
+// Grid W * nd (det fastest), two wavefronts: wave s = spin s.  mode 0: the
+// per-det overlaps; 1: also Ghalf_I = S_I^-1 phi^T; 2: also S_I^-1 itself.
 template <int NB>
 __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, int Nb, int nd,
-                                               const cplx* __restrict__ phia,
+                                               int wlds, const cplx* __restrict__ phia,
                                                const cplx* __restrict__ phib,
                                                const int* __restrict__ occa,
                                                const int* __restrict__ occb, cplx* __restrict__ oda,
                                                cplx* __restrict__ odb, cplx* __restrict__ gha,
-                                               cplx* __restrict__ ghb) {
+                                               cplx* __restrict__ ghb, cplx* __restrict__ inva,
+                                               cplx* __restrict__ invb) {
   extern __shared__ __align__(16) char smem[];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int I = blockIdx.x % nd;
@@ -64,10 +72,10 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
     if (l == 0) *od = mkc(1.0, 0.0);
     return;  // no workgroup barrier in this kernel
   }
-  char* base = smem + wv * GJ_WAVE_LDS;
-  cplx* Inv = (cplx*)base;
-  cplx* bro = Inv + GJ_NMAX * GJ_LD;
-  int* order = (int*)(bro + GJ_NMAX);
+  char* base = smem + (long)wv * wlds;
+  cplx* bro = (cplx*)base;
+  int* order = (int*)(bro + NB);
+  cplx* Inv = (cplx*)(order + 64);
   const cplx* phi = wv ? phib + w * M * Nb : phia + w * M * Na;
   const int* occ = (wv ? occb : occa) + (long)I * N;
   cplx a[NB];
@@ -76,10 +84,64 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
     a[j] = (l < N && j < N) ? phi[(long)occ[j] * N + l] : mkc(0.0, 0.0);
   int mystep = 0;
   gj::wave_gj<NB>(a, N, bro, order, mystep, od);
-  if (mode & 1) {
+  if (mode == 1) {
     gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
     cplx* G = (wv ? ghb : gha) + ((long)I * W + w) * N * M;
     gj::ghalf_mfma_wave<(NB + 3) / 4>(Inv, GJ_LD, N, M, phi, G, nullptr, 0, 0, 0);
+  } else if (mode == 2) {
+    cplx* iv = (wv ? invb : inva) + ((long)I * W + w) * N * N;
+    gj::gj_store_inverse<NB>(a, N, mystep, order, iv, N);
+  }
+}
+
+// io[I][p] = i with occ_I[i] = p, or -1
+__global__ void k_md_invocc(int nd, int N, int M, const int* __restrict__ occ,
+                            int* __restrict__ io) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nd * M) return;
+  const int I = (int)(e / M), p = (int)(e % M);
+  int r = -1;
+  for (int i = 0; i < N; ++i)
+    if (occ[(long)I * N + i] == p) r = i;
+  io[e] = r;
+}
+
+// B_s[w][p][j] = sum_I z_I S_I^-1[i][j] (p = occ_I[i], kept dets only) /
+// overlap, the dets summed per chunk and the chunks in order.  Then
+// G_s = T^* Ghalf summed over dets = B_s phi_s^T.  One thread per element of
+// both spins' B, grid-stride over W M (Na + Nb).
+__global__ __launch_bounds__(256) void k_md_bmat(long n, int W, int M, int Na, int Nb, int nd,
+                                                 int chunk, const int* __restrict__ ioa,
+                                                 const int* __restrict__ iob,
+                                                 const cplx* __restrict__ z,
+                                                 const int* __restrict__ keep,
+                                                 const cplx* __restrict__ ovlp,
+                                                 const cplx* __restrict__ inva,
+                                                 const cplx* __restrict__ invb,
+                                                 cplx* __restrict__ Ba, cplx* __restrict__ Bb) {
+  const long per = (long)M * (Na + Nb);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long w = e / per;
+    long r = e % per;
+    const bool sb = r >= (long)M * Na;
+    if (sb) r -= (long)M * Na;
+    const int N = sb ? Nb : Na;
+    const int p = (int)(r / N), j = (int)(r % N);
+    const int* io = sb ? iob : ioa;
+    const cplx* inv = sb ? invb : inva;
+    cplx acc = mkc(0.0, 0.0);
+    for (int c0 = 0; c0 < nd; c0 += chunk) {
+      cplx cs = mkc(0.0, 0.0);
+      const int c1 = min(nd, c0 + chunk);
+      for (int I = c0; I < c1; ++I) {
+        const int i = io[(long)I * M + p];
+        if (i < 0 || !keep[w * nd + I]) continue;
+        cfma(cs, z[w * nd + I], inv[(((long)I * W + w) * N + i) * N + j]);
+      }
+      acc = cadd(acc, cs);
+    }
+    (sb ? Bb : Ba)[(w * M + p) * N + j] = cdiv(acc, ovlp[w]);
   }
 }
 
@@ -157,7 +219,8 @@ __global__ __launch_bounds__(256) void k_md_gfull(int W, int M, int Na, int Nb, 
   }
 }
 
-// Gt[w][q][p] = Ga[w][p][q] + Gb[w][p][q]  (the force bias contracts L_pq G_qp)
+// Gt[w][q][p] = Ga[w][p][q] + Gb[w][p][q]  (the force bias contracts L_pq G_qp;
+// Gb NULL: Ga already holds the sum)
 __global__ void k_md_gt(long n, int M, const cplx* __restrict__ Ga, const cplx* __restrict__ Gb,
                         cplx* __restrict__ Gt) {
   const long MM = (long)M * M;
@@ -166,11 +229,12 @@ __global__ void k_md_gt(long n, int M, const cplx* __restrict__ Ga, const cplx* 
     const long w = e / MM, r = e % MM;
     const int q = (int)(r / M), p = (int)(r % M);
     const long src = w * MM + (long)p * M + q;
-    Gt[e] = cadd(Ga[src], Gb[src]);
+    Gt[e] = Gb ? cadd(Ga[src], Gb[src]) : Ga[src];
   }
 }
 
-// Gpk[w][c] = Gs[p][q] + Gs[q][p] (p < q) or Gs[p][p], Gs = Ga + Gb, over the
+// Gpk[w][c] = Gs[p][q] + Gs[q][p] (p < q) or Gs[p][p], Gs = Ga + Gb (Ga alone
+// when Gb is NULL), over the
 // packed upper triangle (colmap[c] = (p M + q, q M + p or -1)): with L
 // symmetric, sum_pq L_pq Gs_qp = sum_c Lpacked_c Gpk_c.
 __global__ void k_md_gpack(long n, int M, int P, const int2* __restrict__ colmap,
@@ -182,8 +246,8 @@ __global__ void k_md_gpack(long n, int M, int P, const int2* __restrict__ colmap
     const long w = e / P;
     const int c = (int)(e % P);
     const int2 cm = colmap[c];
-    cplx v = cadd(Ga[w * MM + cm.x], Gb[w * MM + cm.x]);
-    if (cm.y >= 0) v = cadd(v, cadd(Ga[w * MM + cm.y], Gb[w * MM + cm.y]));
+    cplx v = Gb ? cadd(Ga[w * MM + cm.x], Gb[w * MM + cm.x]) : Ga[w * MM + cm.x];
+    if (cm.y >= 0) v = cadd(v, Gb ? cadd(Ga[w * MM + cm.y], Gb[w * MM + cm.y]) : Ga[w * MM + cm.y]);
     Gpk[e] = v;
   }
 }
@@ -230,11 +294,16 @@ size_t a256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 }  // namespace
 
-size_t md_bytes(int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull) {
+size_t md_bytes(int W, int M, int Na, int Nb, int nd, int kind, bool gfull) {
   size_t o = 0;
-  if (ghalf || gfull) {
+  if (kind == MD_GHALF) {
     o += a256(16UL * nd * W * Na * M);
     o += a256(16UL * nd * W * Nb * M);
+  }
+  if (kind == MD_INVERSE) {
+    o += a256(16UL * nd * W * Na * Na) + a256(16UL * nd * W * Nb * Nb);
+    o += a256(16UL * W * M * Na) + a256(16UL * W * M * Nb);
+    o += 2 * a256(4UL * nd * M);
   }
   o += 2 * a256(16UL * W * nd);        // oda, odb
   o += a256(16UL * W * nd);            // z
@@ -244,7 +313,7 @@ size_t md_bytes(int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull) {
   return o;
 }
 
-MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, bool ghalf, bool gfull) {
+MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, int kind, bool gfull) {
   MdBufs b;
   char* p = (char*)base;
   auto take = [&](size_t n) {
@@ -252,9 +321,17 @@ MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, bool ghalf, bo
     p += a256(n);
     return r;
   };
-  if (ghalf || gfull) {
+  if (kind == MD_GHALF) {
     b.gha = (double*)take(16UL * nd * W * Na * M);
     b.ghb = (double*)take(16UL * nd * W * Nb * M);
+  }
+  if (kind == MD_INVERSE) {
+    b.inva = (double*)take(16UL * nd * W * Na * Na);
+    b.invb = (double*)take(16UL * nd * W * Nb * Nb);
+    b.Ba = (double*)take(16UL * W * M * Na);
+    b.Bb = (double*)take(16UL * W * M * Nb);
+    b.ioa = (int*)take(4UL * nd * M);
+    b.iob = (int*)take(4UL * nd * M);
   }
   b.oda = (double*)take(16UL * W * nd);
   b.odb = (double*)take(16UL * W * nd);
@@ -286,24 +363,27 @@ int md_check_occ(hipStream_t s, int nd, int N, int M, const int32_t* occ) {
 }
 
 int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, const double* phia,
-              const double* phib, const int* occa, const int* occb, const double* ci, bool ghalf,
-              bool gfull, const MdBufs& b) {
+              const double* phib, const int* occa, const int* occb, const double* ci, int kind,
+              bool gsum, const MdBufs& b) {
   if (W <= 0) return AFQ_OK;
   const int Nmax = std::max(Na, Nb);
   if (Nmax > GJ_NMAX) return fail(AFQ_EUNSUPPORTED, "multi-det: nocc %d > %d", Nmax, GJ_NMAX);
-  const size_t sm = 2 * (size_t)GJ_WAVE_LDS;
+  if (gsum && kind == MD_GHALF) return fail(AFQ_EINVAL, "multi-det: summed G needs MD_INVERSE");
+  const int mode = kind == MD_GHALF ? 1 : kind == MD_INVERSE ? 2 : 0;
+  const int NB = Nmax <= 8 ? 8 : (Nmax + 7) / 8 * 8;
+  const size_t wl = md_wave_lds(mode, NB, Nmax);
+  const size_t sm = 2 * wl;
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "multi-det: LDS %zu", sm);
   if ((long)W * nd > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "multi-det: grid too large");
-  const int mode = (ghalf || gfull) ? 1 : 0;
   const dim3 grid((unsigned)((long)W * nd));
-#define AFQ_MDGJ(NB)                                                                        \
-  hipLaunchKernelGGL((k_md_gj<NB>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd,         \
-                     (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda,          \
-                     (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb)
-  if (Nmax <= 8) AFQ_MDGJ(8);
-  else if (Nmax <= 16) AFQ_MDGJ(16);
-  else if (Nmax <= 24) AFQ_MDGJ(24);
-  else if (Nmax <= 32) AFQ_MDGJ(32);
+#define AFQ_MDGJ(NBV)                                                                        \
+  hipLaunchKernelGGL((k_md_gj<NBV>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd, (int)wl, \
+                     (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda,           \
+                     (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb, (cplx*)b.inva, (cplx*)b.invb)
+  if (NB == 8) AFQ_MDGJ(8);
+  else if (NB == 16) AFQ_MDGJ(16);
+  else if (NB == 24) AFQ_MDGJ(24);
+  else if (NB == 32) AFQ_MDGJ(32);
   else AFQ_MDGJ(40);
 #undef AFQ_MDGJ
   AFQ_CHECK_LAUNCH();
@@ -311,16 +391,44 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
                      std::max(1, chunk), (const cplx*)b.oda, (const cplx*)b.odb, (const cplx*)ci,
                      (cplx*)b.z, b.keep, (cplx*)b.ovlp);
   AFQ_CHECK_LAUNCH();
-  if (ghalf || gfull) {
+  if (kind == MD_GHALF) {
     hipLaunchKernelGGL(k_md_gfull, dim3(W, 2), dim3(256), 0, s, W, M, Na, Nb, nd, occa, occb,
                        (const cplx*)b.z, b.keep, (const cplx*)b.ovlp, (cplx*)b.gha, (cplx*)b.ghb,
-                       (cplx*)b.Ga, (cplx*)b.Gb, gfull ? 1 : 0);
+                       (cplx*)b.Ga, (cplx*)b.Gb, b.Ga ? 1 : 0);
     AFQ_CHECK_LAUNCH();
+  } else if (kind == MD_INVERSE) {
+    const int nio = cdiv_int((long)nd * M, 256);
+    hipLaunchKernelGGL(k_md_invocc, dim3(nio), dim3(256), 0, s, nd, Na, M, occa, b.ioa);
+    AFQ_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_md_invocc, dim3(nio), dim3(256), 0, s, nd, Nb, M, occb, b.iob);
+    AFQ_CHECK_LAUNCH();
+    const long n = (long)W * M * (Na + Nb);
+    if (n > 0) {
+      hipLaunchKernelGGL(k_md_bmat, dim3((unsigned)std::min<long>(cdiv_int(n, 256), 16384)),
+                         dim3(256), 0, s, n, W, M, Na, Nb, nd, std::max(1, chunk), b.ioa, b.iob,
+                         (const cplx*)b.z, b.keep, (const cplx*)b.ovlp, (const cplx*)b.inva,
+                         (const cplx*)b.invb, (cplx*)b.Ba, (cplx*)b.Bb);
+      AFQ_CHECK_LAUNCH();
+    }
+    if (b.Ga) {
+      // G_s[w][p][q] = sum_j B_s[w][p][j] phi_s[w][q][j]; an empty spin gives 0
+      for (int sp = 0; sp < 2; ++sp) {
+        const int N = sp ? Nb : Na;
+        GemmArgs g;
+        g.m = M; g.n = M; g.k = N; g.batch = W;
+        g.A = sp ? b.Bb : b.Ba; g.lda_m = N; g.lda_k = 1; g.sA = (long)M * N; g.a_cplx = true;
+        g.B = sp ? phib : phia; g.ldb_k = 1; g.ldb_n = N; g.sB = (long)M * N; g.b_cplx = true;
+        g.C = (cplx*)((sp && !gsum) ? b.Gb : b.Ga); g.ldc = M; g.sC = (long)M * M;
+        g.beta = (sp && gsum) ? 1.0 : 0.0;
+        int rc = gemm_fp64(s, g);
+        if (rc) return rc;
+      }
+    }
   }
   return AFQ_OK;
 }
 
-// FP64 multi-det local energy given the buffers of md_greens(ghalf, gfull).
+// FP64 multi-det local energy given the buffers of md_greens(MD_GHALF) with G.
 // scratch: E2 (nd W complex) | rchol_a | rchol_b | X (W G complex) | ec | ex
 //          | exchange slab (gc N W N complex)
 static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int* gc_out) {
@@ -393,7 +501,7 @@ struct MdStep {
 
 size_t md_step_bytes(const afq_system* s, int W) {
   const long M = s->M, G = s->G;
-  size_t n = md_bytes(W, s->M, s->Na, s->Nb, s->ndet, true, true);
+  size_t n = md_bytes(W, s->M, s->Na, s->Nb, s->ndet, MD_INVERSE, true);
   n += a256(16UL * W);                   // oo
   n += a256(16UL * W * G * FB_MAX_SPLITS);  // vbias (split-K partials)
   n += a256(16UL * W * G);               // xs
@@ -401,7 +509,7 @@ size_t md_step_bytes(const afq_system* s, int W) {
   n += a256(8UL * W * G);                // xi
   n += a256(16UL * W * M * M);           // Gt
   n += a256(16UL * W * M * M);           // FP64 VHS
-  n += md_bytes(W, s->M, s->Na, s->Nb, s->ndet, false, false);  // overlap-only pass
+  n += md_bytes(W, s->M, s->Na, s->Nb, s->ndet, MD_OVERLAP, false);  // overlap-only pass
   return n;
 }
 
@@ -424,8 +532,8 @@ int md_ensure(afq_system* s, int W, MdStep* st, MdBufs* ovb) {
   const int C = s->md_cap;
   const long M = s->M, G = s->G;
   char* p = (char*)s->md_work;
-  st->b = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, true, true);
-  p += md_bytes(C, s->M, s->Na, s->Nb, s->ndet, true, true);
+  st->b = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, MD_INVERSE, true);
+  p += md_bytes(C, s->M, s->Na, s->Nb, s->ndet, MD_INVERSE, true);
   auto take = [&](size_t n) {
     char* r = p;
     p += a256(n);
@@ -439,9 +547,9 @@ int md_ensure(afq_system* s, int W, MdStep* st, MdBufs* ovb) {
   st->xi = take(8UL * C * G);
   st->Gt = take(16UL * C * M * M);
   st->vhs = take(16UL * C * M * M);
-  *ovb = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, false, false);
+  *ovb = md_carve(p, C, s->M, s->Na, s->Nb, s->ndet, MD_OVERLAP, false);
   // the buffers are laid out for C walkers; re-carve for W keeps strides W
-  st->b = md_carve(s->md_work, W, s->M, s->Na, s->Nb, s->ndet, true, true);
+  st->b = md_carve(s->md_work, W, s->M, s->Na, s->Nb, s->ndet, MD_INVERSE, true);
   return AFQ_OK;
 }
 
@@ -464,10 +572,10 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     if (rc) return rc;
     xi = st.xi;
   }
-  // 1: multi-det G and overlap
+  // 1: multi-det G (Ga + Gb into st.b.Ga) and overlap
   phase_begin(sys, PH_GREENS);
   rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 true, true, st.b);
+                 MD_INVERSE, true, st.b);
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp: over the packed triangle on the
@@ -479,7 +587,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     const int P = sys->packed_P;
     const long n = (long)W * P;
     hipLaunchKernelGGL(k_md_gpack, dim3(4096), dim3(256), 0, s, n, M, P, sys->colmap,
-                       (const cplx*)st.b.Ga, (const cplx*)st.b.Gb, (cplx*)st.Gt);
+                       (const cplx*)st.b.Ga, (const cplx*)nullptr, (cplx*)st.Gt);
     AFQ_CHECK_LAUNCH();
     FbSplits sp{};
     const int nsp = 4;
@@ -497,7 +605,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   } else {
     const long n = (long)W * M * M;
     hipLaunchKernelGGL(k_md_gt, dim3(4096), dim3(256), 0, s, n, M, (const cplx*)st.b.Ga,
-                       (const cplx*)st.b.Gb, (cplx*)st.Gt);
+                       (const cplx*)nullptr, (cplx*)st.Gt);
     AFQ_CHECK_LAUNCH();
     GemmArgs g;
     g.m = W; g.n = G; g.k = M * M;
@@ -548,10 +656,10 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   }
   // 7: multi-det overlap of the propagated walkers
   phase_begin(sys, PH_OVERLAP);
-  MdBufs ob = md_carve(ovb.oda, W, M, Na, Nb, nd, false, false);
+  MdBufs ob = md_carve(ovb.oda, W, M, Na, Nb, nd, MD_OVERLAP, false);
   ob.ovlp = ovlp_new;
   rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 false, false, ob);
+                 MD_OVERLAP, false, ob);
   phase_end(sys, PH_OVERLAP);
   if (rc) return rc;
   // 8
@@ -567,18 +675,16 @@ int system_energy_multidet(afq_system* sys, int W, const double* phia, const dou
                            double* e_2b) {
   hipStream_t s = sys->ctx->stream;
   const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G, nd = sys->ndet;
-  MdStep st;
-  MdBufs ovb;
-  int rc = md_ensure(sys, W, &st, &ovb);
-  if (rc) return rc;
-  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 true, true, st.b);
-  if (rc) return rc;
-  const size_t need = md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
-  void* scr = scratch(sys->ctx, need);
+  const size_t nb = md_bytes(W, M, Na, Nb, nd, MD_GHALF, true);
+  const size_t ne = md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
+  char* scr = (char*)scratch(sys->ctx, nb + ne);
   if (!scr) return AFQ_ENOMEM;
+  const MdBufs b = md_carve(scr, W, M, Na, Nb, nd, MD_GHALF, true);
+  int rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b,
+                     sys->ci, MD_GHALF, false, b);
+  if (rc) return rc;
   return md_energy(s, W, M, Na, Nb, G, nd, sys->det_chunk, h1e, sys->chol, sys->occ_a,
-                   sys->occ_b, enuc, st.b, scr, e_tot, e_1b, e_2b);
+                   sys->occ_b, enuc, b, scr + nb, e_tot, e_1b, e_2b);
 }
 
 }  // namespace afq
@@ -599,15 +705,15 @@ int afq_greens_function_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
   if (rc || W == 0) return rc;
-  const size_t n = afq::md_bytes(W, M, Na, Nb, nd, true, false);
+  const size_t n = afq::md_bytes(W, M, Na, Nb, nd, afq::MD_INVERSE, false);
   void* scr = afq::scratch(ctx, n);
   if (!scr) return AFQ_ENOMEM;
-  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, true, false);
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, afq::MD_INVERSE, false);
   b.Ga = Ga;
   b.Gb = Gb;
   b.ovlp = overlap;
   return afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
-                        true, true, b);
+                        afq::MD_INVERSE, false, b);
 }
 
 int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int W, int M, int Na, int Nb, int nd,
@@ -622,12 +728,12 @@ int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int W, int M, int Na
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
   if (rc || W == 0) return rc;
-  void* scr = afq::scratch(ctx, afq::md_bytes(W, M, Na, Nb, nd, false, false));
+  void* scr = afq::scratch(ctx, afq::md_bytes(W, M, Na, Nb, nd, afq::MD_OVERLAP, false));
   if (!scr) return AFQ_ENOMEM;
-  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, false, false);
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, afq::MD_OVERLAP, false);
   b.ovlp = overlap;
   return afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
-                        false, false, b);
+                        afq::MD_OVERLAP, false, b);
 }
 
 int afq_local_energy_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, int nd,
@@ -643,13 +749,13 @@ int afq_local_energy_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int G,
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
   if (rc || W == 0) return rc;
-  const size_t nb = afq::md_bytes(W, M, Na, Nb, nd, true, true);
+  const size_t nb = afq::md_bytes(W, M, Na, Nb, nd, afq::MD_GHALF, true);
   const size_t ne = afq::md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
   char* scr = (char*)afq::scratch(ctx, nb + ne);
   if (!scr) return AFQ_ENOMEM;
-  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, true, true);
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, afq::MD_GHALF, true);
   if ((rc = afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
-                           true, true, b)))
+                           afq::MD_GHALF, false, b)))
     return rc;
   return afq::md_energy(s, W, M, Na, Nb, G, nd, det_chunk_size, h1e, chol, occ_up, occ_dn, enuc,
                         b, scr + nb, e_tot, e_1b, e_2b);
