@@ -71,7 +71,7 @@ KERNEL_OF_PHASE = {
 
 # rocprofv3 symbol of the dominant kernel at the M=300 workload (MT = 16-row
 # tiles per wave for M=300)
-KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop_f64<3>", ("taylor", "bf16x3"): "k_prop_bf3<5>",
+KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop_f64<3>", ("taylor", "bf16x3"): "k_prop_bf3<3>",
                  ("vhs", "fp64"): "k_vhs_f64<true>", ("vhs", "bf16x3"): "k_bf3_gemm<0>",
                  ("force_bias", "fp64"): "k_fb_f64", ("force_bias", "bf16x3"): "k_bf3_gemm<1>",
                  ("greens", "fp64"): "k_greens<false>", ("greens", "bf16x3"): "k_greens<false>"}

@@ -35,9 +35,11 @@ int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt
 // Ghalf planes [2][2W][K_s] for both spins -> vbias partials (float2) part[s][W][G].
 int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, const bf16_t* Ghb,
                    float* part, int* nsplit_out);
-// phi <- expH1 . Taylor(VHS) . expH1 . phi in place, FP64 walkers.
-int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, double* phia,
-                  double* phib, int nmax);
+// phi <- expH1 . Taylor(VHS) . expH1 . phi in place, FP64 walkers; t0 is a
+// scratch of bf3_prop_scratch_bytes(b, W).
+size_t bf3_prop_scratch_bytes(const Bf3System& b, int W);
+int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, float* t0,
+                  double* phia, double* phib, int nmax);
 // Exchange energy e_x[w] = 1/2 sum_s sum_g sum_ij T_ij T_ji, T = Lhalf_s[g] Ghalf_s^T,
 // from the split-bf16 Ghalf planes (N_s <= 40); part: bf3_exchange_part_bytes.
 bool bf3_exchange_supported(const Bf3System& b);

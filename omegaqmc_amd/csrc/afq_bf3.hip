@@ -11,8 +11,9 @@
 //                     owns one block for 128 walkers and writes it and its
 //                     transpose in the block-major layout the propagator reads.
 //   k_bf3_gemm<FB>    split-K force bias, FP32 partials summed in FP64 later.
-//   k_prop_bf3        expH1 . sum_n VHS^n/n! . expH1 on a 48-column block of one
-//                     spin of one walker, the block kept in LDS as bf16 planes.
+//   k_prop_bf3        expH1 . sum_n VHS^n/n! . expH1 (Horner form) on a 48-column
+//                     block of one spin of one walker, the block kept in LDS as
+//                     bf16 planes.
 //
 // MFMA layout (16x16x32 bf16, checked on the device): A[i][k] in lane
 // i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8); D[i][j] in lane
@@ -271,12 +272,19 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 // ----------------------------------------------------------------------------
 // fused propagator on a 48-column block of one spin
 // ----------------------------------------------------------------------------
-constexpr int PW = 4, PT = 64 * PW, PC = 48;
+// The Taylor sum runs in Horner form, y <- T0 + V y / n for n = nmax..1 with
+// y = T0 = expH1 phi to start (the same polynomial as afqmc_gto.py:451-469):
+// only T0 has to be kept besides the current y, and it waits in a global
+// scratch in the accumulators' own lane order (t0s, RT * 3 * 64 * 8 floats per
+// workgroup), so the registers hold one accumulator set and two waves fit
+// per SIMD.
+constexpr int PW = 8, PT = 64 * PW, PC = 48;
 
 template <int MT>
 __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, int Na, int Nb,
                                                  int ncba, int ncbb, const bf16_t* __restrict__ E,
-                                                 const bf16_t* __restrict__ Vb, double* phia,
+                                                 const bf16_t* __restrict__ Vb,
+                                                 float4* __restrict__ t0s, double* phia,
                                                  double* phib, int nmax) {
   extern __shared__ __align__(16) bf16_t Ts[];  // [4][PC][Kp + 8]
   const int KR = Kp + 8;
@@ -316,7 +324,9 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   }
   __syncthreads();
 
-  f4v are[MT][3], aim[MT][3], rre[MT][3], rim[MT][3];
+  f4v are[MT][3], aim[MT][3];
+  float4* T0 = t0s + (long)(w * per + cbx) * RT * 3 * 64 * 2;
+  auto t0_at = [&](int u, int x) { return T0 + ((long)((wv + PW * u) * 3 + x) * 64 + l) * 2; };
   auto owns = [&](int u) { return wv + PW * u < RT; };
   auto zero = [&]() {
 #pragma unroll
@@ -461,28 +471,34 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   __syncthreads();
   to_lds(are, aim);
 #pragma unroll
-  for (int u = 0; u < MT; ++u)
+  for (int u = 0; u < MT; ++u) {
+    if (!owns(u)) continue;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) { rre[u][x] = are[u][x]; rim[u][x] = aim[u][x]; }
+    for (int x = 0; x < 3; ++x) {
+      float4* d = t0_at(u, x);
+      d[0] = make_float4(are[u][x][0], are[u][x][1], are[u][x][2], are[u][x][3]);
+      d[1] = make_float4(aim[u][x][0], aim[u][x][1], aim[u][x][2], aim[u][x][3]);
+    }
+  }
   __syncthreads();
-  for (int n = 1; n <= nmax; ++n) {
+  for (int n = nmax; n >= 1; --n) {
     term();
     const float rn = 1.0f / (float)n;
 #pragma unroll
-    for (int u = 0; u < MT; ++u)
+    for (int u = 0; u < MT; ++u) {
+      if (!owns(u)) continue;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        are[u][x] *= rn;
-        aim[u][x] *= rn;
-        rre[u][x] += are[u][x];
-        rim[u][x] += aim[u][x];
+        const float4* d = t0_at(u, x);
+        const float4 tr = d[0], ti = d[1];
+        are[u][x] = f4v{tr.x, tr.y, tr.z, tr.w} + are[u][x] * rn;
+        aim[u][x] = f4v{ti.x, ti.y, ti.z, ti.w} + aim[u][x] * rn;
       }
+    }
     __syncthreads();
     to_lds(are, aim);
     __syncthreads();
   }
-  to_lds(rre, rim);
-  __syncthreads();
   onebody();
 #pragma unroll
   for (int u = 0; u < MT; ++u) {
@@ -649,20 +665,27 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
   return AFQ_OK;
 }
 
-int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, double* phia,
-                  double* phib, int nmax) {
+size_t bf3_prop_scratch_bytes(const Bf3System& b, int W) {
+  const long ncb = (b.Na + PC - 1) / PC + (b.Nb + PC - 1) / PC;
+  return (size_t)W * ncb * b.RT * 3 * 64 * 2 * sizeof(float4);
+}
+
+int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, float* t0,
+                  double* phia, double* phib, int nmax) {
   const int ncba = (b.Na + PC - 1) / PC, ncbb = (b.Nb + PC - 1) / PC;
   if (W <= 0 || ncba + ncbb == 0) return AFQ_OK;
   const size_t sm = (size_t)4 * PC * (b.Kp + 8) * sizeof(bf16_t);
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: LDS %zu", sm);
+  if (b.RT > 4 * PW) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: M=%d", b.M);
+  if (!t0) return fail(AFQ_EINVAL, "bf16x3 propagator: no scratch");
   const dim3 grid(W * (ncba + ncbb)), blk(PT);
 #define AFQ_PROP_BF3(MT)                                                                      \
   hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, ncba, \
-                     ncbb, b.E, Vb, phia, phib, nmax)
-  if (b.RT <= 2 * PW) AFQ_PROP_BF3(2);
+                     ncbb, b.E, Vb, (float4*)t0, phia, phib, nmax)
+  if (b.RT <= PW) AFQ_PROP_BF3(1);
+  else if (b.RT <= 2 * PW) AFQ_PROP_BF3(2);
   else if (b.RT <= 3 * PW) AFQ_PROP_BF3(3);
-  else if (b.RT <= 5 * PW) AFQ_PROP_BF3(5);
-  else AFQ_PROP_BF3(8);
+  else AFQ_PROP_BF3(4);
 #undef AFQ_PROP_BF3
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;

@@ -19,7 +19,7 @@ struct LowpWork {
   void* buf = nullptr;
   size_t bytes = 0;
   // offsets
-  size_t gh_a, gh_b, oo, part, X, cfb, cmf, xi, vhs;
+  size_t gh_a, gh_b, oo, part, X, cfb, cmf, xi, vhs, t0;
 };
 
 constexpr int FB_SPLITS_MAX = 4;
@@ -40,6 +40,7 @@ size_t lay(const Bf3System& b, int W, LowpWork* L) {
   L->cmf = take(16L * W);
   L->xi = take(8L * W * b.G);
   L->vhs = take(bf3_vhs_bytes(b, W));
+  L->t0 = take(bf3_prop_scratch_bytes(b, W));
   return o;
 }
 
@@ -75,14 +76,16 @@ int ensure(afq_system* s, int W, LowpWork** out) {
 
 }  // namespace
 
-// The split-bf16 xshifted planes and VHS of the reduced-precision workspace
-// (used by the multi-determinant step, whose force bias is FP64).
-int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs) {
+// The split-bf16 xshifted planes, VHS and propagator scratch of the
+// reduced-precision workspace (used by the multi-determinant step, whose force
+// bias is FP64).
+int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs, float** t0) {
   LowpWork* L = nullptr;
   int rc = ensure(sys, W, &L);
   if (rc) return rc;
   *X = (bf16_t*)((char*)L->buf + L->X);
   *vhs = (bf16_t*)((char*)L->buf + L->vhs);
+  *t0 = (float*)((char*)L->buf + L->t0);
   return AFQ_OK;
 }
 
@@ -165,7 +168,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   if (rc) return rc;
   // 2, 5, 6
   phase_begin(sys, PH_TAYLOR);
-  rc = bf3_propagate(s, b, W, vhs, phia, phib, sys->nmax);
+  rc = bf3_propagate(s, b, W, vhs, (float*)(base + L->t0), phia, phib, sys->nmax);
   phase_end(sys, PH_TAYLOR);
   if (rc) return rc;
   // 7, 8 in FP64

@@ -34,7 +34,7 @@ namespace afq {
 
 void phase_begin(afq_system* s, int ph);
 void phase_end(afq_system* s, int ph);
-int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs);
+int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs, float** t0);
 
 namespace {
 
@@ -620,7 +620,8 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   if (sys->precision == AFQ_BF16X3) {
     bf16_t* X = nullptr;
     bf16_t* vb = nullptr;
-    if ((rc = lowp_step_buffers(sys, W, &X, &vb))) return rc;
+    float* t0 = nullptr;
+    if ((rc = lowp_step_buffers(sys, W, &X, &vb, &t0))) return rc;
     const Bf3System& b = *(const Bf3System*)sys->chol_lp;
     phase_begin(sys, PH_SHIFT);
     rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr,
@@ -632,7 +633,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     phase_end(sys, PH_VHS);
     if (rc) return rc;
     phase_begin(sys, PH_TAYLOR);
-    rc = bf3_propagate(s, b, W, vb, phia, phib, sys->nmax);
+    rc = bf3_propagate(s, b, W, vb, t0, phia, phib, sys->nmax);
     phase_end(sys, PH_TAYLOR);
     if (rc) return rc;
   } else {
