@@ -35,8 +35,8 @@ int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, 
 
 // stabilisation / population control (afq_ortho.hip, afq_comb.hip)
 int ortho_launch(hipStream_t s, int W, int M, int Na, int Nb, double* phia, double* phib,
-                 double* log_detR, double* sign_detR, double* scratch2w);
-size_t ortho_scratch_bytes(int W);
+                 double* log_detR, double* sign_detR, void* scratch);
+size_t ortho_scratch_bytes(int W, int Na, int Nb);
 int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
                 const double* phia, const double* phib, double r, const double* eh,
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,

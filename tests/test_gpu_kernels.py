@@ -171,16 +171,45 @@ def test_update_weights_edge_cases(afq):
     np.testing.assert_allclose(_n(en), ren, rtol=1e-12, atol=1e-12)
 
 
-def test_orthogonalize(afq):
-    rng = np.random.default_rng(9)
-    W, M, Na, Nb = 10, 37, 6, 5
+@pytest.mark.parametrize("W,M,Na,Nb", [(10, 37, 6, 5), (9, 50, 1, 0), (6, 64, 17, 16),
+                                        (5, 90, 33, 24), (4, 120, 40, 40), (3, 120, 45, 41),
+                                        (2, 100, 64, 7)])
+def test_orthogonalize(afq, W, M, Na, Nb):
+    """Every register-elimination size (N <= 40), the LDS factorisation
+    (N > 40) and an empty spin, against the Householder oracle."""
+    rng = np.random.default_rng(9 + M + Na)
     pa = rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na))
     pb = rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb))
     qa, qb, ld, _ = afq.orthogonalize_walkers(_t(pa), _t(pb))
     rqa, rqb, rld, _ = O.orthogonalize_walkers(pa, pb)
     assert rel_err(_n(qa), rqa) < TOL
-    assert rel_err(_n(qb), rqb) < TOL
+    if Nb:
+        assert rel_err(_n(qb), rqb) < TOL
     assert rel_err(_n(ld), rld) < TOL
+
+
+def test_orthogonalize_walkers_are_independent(afq):
+    """A walker whose columns are nearly dependent (condition ~1e12: its Gram
+    matrix is singular to working precision) takes the shifted or the failed
+    branch on its own; the other walkers of the batch match the oracle."""
+    rng = np.random.default_rng(21)
+    W, M, Na, Nb = 6, 80, 12, 9
+    pa = rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na))
+    pb = rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb))
+    pa[2, :, 5] = pa[2, :, 4] * (1 + 1e-12) + 1e-12 * rng.normal(size=M)
+    qa, qb, ld, _ = afq.orthogonalize_walkers(_t(pa), _t(pb))
+    rqa, rqb, rld, _ = O.orthogonalize_walkers(pa, pb)
+    qa, qb, ld = _n(qa), _n(qb), _n(ld)
+    ok = [w for w in range(W) if w != 2]
+    assert rel_err(qa[ok], rqa[ok]) < TOL
+    assert rel_err(qb, rqb) < TOL
+    assert rel_err(ld[ok], rld[ok]) < TOL
+    if np.isfinite(ld[2]):
+        # the shifted passes: orthonormal columns spanning phi
+        q = qa[2]
+        assert np.abs(q.conj().T @ q - np.eye(Na)).max() < 1e-10
+        r = q.conj().T @ pa[2]
+        assert np.linalg.norm(q @ r - pa[2]) < 1e-10 * np.linalg.norm(pa[2])
 
 
 @pytest.mark.parametrize("W", [1, 7, 256, 1024, 20000])
