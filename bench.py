@@ -335,6 +335,7 @@ def main():
             extra[f"{prec}_walker_steps_per_s"] = round(W / (r["ms_per_step"] * 1e-3), 2)
             extra[f"{prec}_phases_ms_per_step"] = {k: round(v["ms_total"] / args.steps, 4)
                                                    for k, v in r["phases"].items()}
+            extra[f"{prec}_block_energy_ms"] = round(r["energy_ms"], 2)
             del r["drv"], r["state"]
             torch.cuda.empty_cache()
             print(f"[bench] {prec} vs FP64 energy, same trajectory ({args.acc_steps} steps)",

@@ -12,7 +12,7 @@ struct Bf3System {
   int Gp;        // G padded to a multiple of 32
   int nblk;      // upper-triangle 16x16 blocks of an M x M matrix (RT(RT+1)/2)
   int Kp;        // 16*RT padded to a multiple of 32 (K of the propagator products)
-  int ldq;       // M padded to a multiple of 8: row stride of (i, q) in Lhalf / Ghalf planes
+  int ldq;       // M padded to a multiple of 32: row stride of (i, q) in Lhalf / Ghalf planes
   int Ka, Kb;    // N_s * ldq padded to multiples of 32 (K of the force bias)
   // L_g block-packed: column c = blk*256 + i*16 + j  <->  (16P+i, 16Q+j), P <= Q;
   // stored column-major in g: Lb[plane][c][Gp]

@@ -538,7 +538,7 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
   b->Gp = (G + 31) / 32 * 32;
   b->nblk = b->RT * (b->RT + 1) / 2;
   b->Kp = (16 * b->RT + 31) / 32 * 32;
-  b->ldq = (M + 7) / 8 * 8;
+  b->ldq = (M + GK - 1) / GK * GK;  // whole K steps of the exchange GEMM
   b->Ka = (s->Na * b->ldq + 31) / 32 * 32;
   b->Kb = (s->Nb * b->ldq + 31) / 32 * 32;
   b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = nullptr; b->Lhb = nullptr; b->E = nullptr;
@@ -707,15 +707,36 @@ namespace {
 constexpr int XNW = 4, XNT = 64 * XNW, XWT = 80, XRT = 2 * XWT, XCT = 2 * XWT, XGR = 8;
 constexpr int XMAXN = XWT / 2;
 constexpr int XSLOT = (2 * XRT) * 4 / XNT;  // 16-byte staging slots per thread and operand
-constexpr int XEPI = XWT * (XMAXN + 1);     // floats of one wave's epilogue block
+constexpr int XNP = XMAXN + 1;              // row stride of the transposed block
+// floats of one wave's epilogue block: 80 rows of XNP and a dummy area that
+// absorbs the writes of columns outside the tile
+constexpr int XDUM = XWT * XNP + 16;
+constexpr int XEPI = XDUM + 112;
+// one staging buffer: planes (A hi, A lo, B hi, B lo) of XRT rows x GK, rows
+// of 64 bytes in blocks of 16 rows written by one LDS-DMA instruction each
+constexpr int XSTAGE = 4 * XRT * GK;  // bf16
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+// buffer load of 16 bytes per lane straight into LDS: lane l's bytes land at
+// lds + 16 l (lds uniform across the wavefront)
+__device__ void dma16(i32x4 rsrc, __attribute__((address_space(3))) unsigned* lds, int size,
+                      int voffset, int soffset, int offset, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+// 16-byte chunk position of K chunk c in row li of a 16-row block: the XOR
+// keeps MFMA operand reads (16 rows, one chunk) free of bank conflicts with
+// either 32 or 64 LDS banks
+__device__ __forceinline__ int xswz(int li, int c) { return c ^ (((li >> 1) ^ (li >> 3)) & 3); }
 
 struct Bf3Exch {
   const bf16_t* a;  // Lhalf hi plane (lo plane at +aplane): row (g, i) at g ka + i ldq
   const bf16_t* b;  // Ghalf hi plane (lo plane at +bplane): row (w, c, j) at (2 w + c) kb + j ldq
   int aplane, bplane;
   int ka, kb;
-  int N, G, W, ldq, Mp;
+  int N, G, W, ldq;
   int ng, nw, ntr, ntc;
+  int nb;        // logical tiles: row tiles padded to groups of XGR, times ntc
+  int mN, m2N;   // ceil(2^16 / N), ceil(2^16 / 2N): x / N = (x mN) >> 16 for x < 512
   double* part;  // [ntr][W] complex
 };
 
@@ -726,221 +747,291 @@ __device__ __forceinline__ void xwave_sync() {
 }
 
 // Workgroup tile: 160 rows (g, i) x 160 columns (walker, re/im, j), 4 waves of
-// 80 x 80 in a 2 x 2 grid, K step 32; two workgroups share a CU so that one
-// computes while the other waits on its loads.
+// 80 x 80 in a 2 x 2 grid, K step 32; two workgroups per CU.  The grid is
+// persistent: XCD x owns a contiguous range of logical tiles and its
+// workgroups take them in turn, so at any time an XCD works on neighbouring
+// tiles (inside a group of XGR row tiles the row tile runs fastest, so the L2
+// holds both operands).  Operand chunks go from global memory straight into
+// two LDS stages by buffer-load DMA (no staging registers): the next K step's
+// chunk is in flight while this one computes, one barrier per step.  The
+// epilogue reuses both stages.
 __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
   extern __shared__ __align__(16) bf16_t lds[];
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
   const int wr = wv >> 1, wc = wv & 1;
   const int li = l & 15, lq = l >> 4;
-  // each XCD walks a contiguous range of tiles; inside it XGR row tiles share
-  // every column tile (row tile fastest), so the L2 holds both operands
-  const int v = xcd_remap(blockIdx.x, gridDim.x);
-  const int grp = v / (XGR * a.ntc), in = v % (XGR * a.ntc);
-  const int tr = grp * XGR + in % XGR, tc = in / XGR;
-  if (tr >= a.ntr) return;  // whole workgroup: before any barrier
   const int N = a.N, NR = a.ng * N, NC = a.nw * 2 * N;
-  const int kq = t & 3;
+  const int nx = gridDim.x >> 3, jx = blockIdx.x >> 3;
+  const int per = (a.nb + 7) >> 3;
+  const int v0 = (blockIdx.x & 7) * per, v1 = min(a.nb, v0 + per);
+  auto tile_of = [&](int v, int& tr, int& tc) {
+    const int grp = v / (XGR * a.ntc), in = v % (XGR * a.ntc);
+    tr = grp * XGR + in % XGR;
+    tc = in / XGR;
+  };
+  // the first tile at or after v with a real row tile (padding only in the last group)
+  auto next_tile = [&](int v) {
+    for (; v < v1; v += nx) {
+      int tr, tc;
+      tile_of(v, tr, tc);
+      if (tr < a.ntr) break;
+    }
+    return v;
+  };
+  int v = next_tile(v0 + jx);
+  if (v >= v1) return;  // whole workgroup: before any barrier
 
-  // staging slots: operand rows 0..2*XRT-1 = (plane, row); slot i covers
-  // row (t >> 2) + 64 i, K chunk kq
+  auto rsrc = [](const void* p, int bytes) {
+    const unsigned long long ad = (unsigned long long)p;
+    i32x4 r = {(int)(unsigned)ad, (int)((ad >> 32) & 0xffff), bytes, 0x00020000};
+    r.x = __builtin_amdgcn_readfirstlane(r.x);
+    r.y = __builtin_amdgcn_readfirstlane(r.y);
+    r.z = __builtin_amdgcn_readfirstlane(r.z);
+    return r;
+  };
+  const i32x4 rsa = rsrc(a.a, 4 * a.aplane), rsb = rsrc(a.b, 4 * a.bplane);
+
+  // DMA slots: wave wv's instruction i fills 16-row block wv + 4 i of the A
+  // planes (rows 0..2*XRT-1 = (plane, row)) and of the B planes; lane l
+  // fetches row l / 4 of the block, K chunk xswz(l / 4, l % 4).  An empty
+  // slot (row outside the tile's g blocks or walkers) reads offset 0: it only
+  // feeds outputs the epilogue ignores.
+  const int lrow = l >> 2, lkc = xswz(lrow, l & 3);
   int oa[XSLOT], ob[XSLOT];
-  bool va[XSLOT], vb[XSLOT];
-#pragma unroll
-  for (int i = 0; i < XSLOT; ++i) {
-    const int rp = (t >> 2) + 64 * i, p = rp >= XRT, r = rp - p * XRT;
-    {
-      const int wrr = r / XWT, rr = r % XWT;
-      const int g = (tr * 2 + wrr) * a.ng + rr / N, ii = rr % N;
-      va[i] = rr < NR && g < a.G;
-      oa[i] = va[i] ? p * a.aplane + g * a.ka + ii * a.ldq + 8 * kq : 0;
-    }
-    {
-      const int wcc = r / XWT, cc = r % XWT;
-      const int w = (tc * 2 + wcc) * a.nw + cc / (2 * N);
-      const int rem = cc % (2 * N), ri = rem / N, j = rem % N;
-      vb[i] = cc < NC && w < a.W;
-      ob[i] = vb[i] ? p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * kq : 0;
-    }
-  }
-  typedef unsigned v4u __attribute__((ext_vector_type(4)));
-  typedef __attribute__((address_space(1))) const v4u gv4u;
-  const gv4u* ga = (const gv4u*)(a.a);
-  const gv4u* gb = (const gv4u*)(a.b);
-  // Plain global loads back to back; an empty slot (row outside the tile's
-  // g blocks or walkers) reads offset 0: it only feeds outputs the epilogue
-  // ignores.  K chunks at or beyond ldq do feed valid outputs and are zeroed
-  // when stored (mask applied at the store, so the loads stay in flight).
-  v4u ra[XSLOT], rb[XSLOT];
-  unsigned kmask = ~0u;
-  auto load = [&](int k0) {
-    const bool kin = k0 + 8 * kq < a.ldq;
-    kmask = kin ? ~0u : 0u;
+  auto slots = [&](int tr, int tc) {
+    int tt = t;  // opaque per tile: keeps the index arithmetic out of registers between tiles
+    asm volatile("" : "+v"(tt));
+    const int ww = tt >> 6, lr = (tt & 63) >> 2;
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
-      ra[i] = ga[(va[i] && kin ? oa[i] + k0 : 0) >> 3];
-      rb[i] = gb[(vb[i] && kin ? ob[i] + k0 : 0) >> 3];
+      const int rp = 16 * (ww + 4 * i) + lr, p = rp >= XRT, r = rp - p * XRT;
+      const int wrr = r >= XWT, rr = r - wrr * XWT;  // row-wave half, row in it
+      {
+        const int gq = (rr * a.mN) >> 16, ii = rr - gq * N;
+        const int g = (tr * 2 + wrr) * a.ng + gq;
+        const bool ok = rr < NR && g < a.G;
+        oa[i] = ok ? 2 * (p * a.aplane + g * a.ka + ii * a.ldq + 8 * lkc) : 0;
+      }
+      {
+        const int wq = (rr * a.m2N) >> 16, rem = rr - wq * 2 * N;
+        const int ri = rem >= N, j = rem - ri * N;
+        const int w = (tc * 2 + wrr) * a.nw + wq;
+        const bool ok = rr < NC && w < a.W;
+        ob[i] = ok ? 2 * (p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * lkc) : 0;
+      }
     }
   };
-  auto store = [&]() {
+  // chunk at K offset k0 into stage st (rows are ldq long, a multiple of GK,
+  // zero past M: chunks never run into the next row)
+  auto fetch = [&](int st, int k0) {
+    __attribute__((address_space(3))) unsigned* base =
+        (__attribute__((address_space(3))) unsigned*)(lds + st * XSTAGE);
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
-      const int rp = (t >> 2) + 64 * i;
-      *(v4u*)(lds + rp * LROW + 8 * kq) = ra[i] & kmask;
-      *(v4u*)(lds + (2 * XRT + rp) * LROW + 8 * kq) = rb[i] & kmask;
+      const int blk = wv + 4 * i;  // 16-row block, uniform
+      dma16(rsa, base + blk * 256, 16, oa[i], 2 * k0, 0, 0);
+      dma16(rsb, base + (2 * XRT / 16 + blk) * 256, 16, ob[i], 2 * k0, 0, 0);
     }
   };
+  auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
+  const int nk = a.ldq / GK;
 
-  f4v acc[5][5];
-#pragma unroll
-  for (int u = 0; u < 5; ++u)
-#pragma unroll
-    for (int w = 0; w < 5; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
+  float* Rt = (float*)lds + wv * XEPI;
+  double* cs = (double*)((float*)lds + XNW * XEPI) + wv * 2 * XWT;  // [80][2]
+  double* wsum = (double*)((float*)lds + XNW * XEPI) + XNW * 2 * XWT;  // [XNW][nw][2]
+  // lane offset of (row li, chunk lq) inside a 16-row block, in bf16
+  const int frag = li * GK + 8 * xswz(li, lq);
 
-  const int nk = a.Mp / GK;
-  load(0);
-  store();
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load((kt + 1) * GK);
-    uint4 ah[5], al[5];
-#pragma unroll
-    for (int u = 0; u < 5; ++u) {
-      const int r = XWT * wr + 16 * u + li;
-      ah[u] = *(const uint4*)(lds + r * LROW + 8 * lq);
-      al[u] = *(const uint4*)(lds + (XRT + r) * LROW + 8 * lq);
-    }
-#pragma unroll
-    for (int w = 0; w < 5; ++w) {
-      const int c = XWT * wc + 16 * w + li;
-      const uint4 bh = *(const uint4*)(lds + (2 * XRT + c) * LROW + 8 * lq);
-      const uint4 bl = *(const uint4*)(lds + (3 * XRT + c) * LROW + 8 * lq);
-#pragma unroll
-      for (int u = 0; u < 5; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
-    }
+  for (; v < v1; v = next_tile(v + nx)) {
+    int tr, tc;
+    tile_of(v, tr, tc);
+    slots(tr, tc);
+    fetch(0, 0);
+    dma_wait();
     __syncthreads();
-    if (kt + 1 < nk) {
-      store();
+
+    f4v acc[5][5];
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+#pragma unroll
+      for (int w = 0; w < 5; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < nk; ++kt) {
+      if (kt + 1 < nk) fetch((kt + 1) & 1, (kt + 1) * GK);
+      const bf16_t* st = lds + (kt & 1) * XSTAGE + frag;
+      uint4 ah[5], al[5];
+#pragma unroll
+      for (int u = 0; u < 5; ++u) {
+        const int r = XWT * wr + 16 * u;  // block-aligned row
+        ah[u] = *(const uint4*)(st + r * GK);
+        al[u] = *(const uint4*)(st + (XRT + r) * GK);
+      }
+#pragma unroll
+      for (int w = 0; w < 5; ++w) {
+        const int c = XWT * wc + 16 * w;
+        const uint4 bh = *(const uint4*)(st + (2 * XRT + c) * GK);
+        const uint4 bl = *(const uint4*)(st + (3 * XRT + c) * GK);
+#pragma unroll
+        for (int u = 0; u < 5; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
+      }
+      dma_wait();
       __syncthreads();
     }
-  }
 
-  // Epilogue.  The lane holding T_c[i][j] (c = re/im) of walker wi writes it
-  // transposed, Rt[wi][c][j][i]; after a wave barrier it reads T_re[j][i] and
-  // T_im[j][i] at the same place and accumulates its share of sum_ij T_ij T_ji
-  // per column (products in FP64: the terms cancel strongly).  Column sums
-  // are then reduced per walker.
-  const int NP = N + 1;
-  float* Rt = (float*)lds + wv * XEPI;
-  int wb[5], rb0[5], rb1[5];
-  bool cv[5], cim_[5];
+    // Epilogue.  The lane holding T_c[i][j] (c = re/im) of walker wi writes
+    // it transposed, Rt[wi][c][j][i] (row stride XNP); after a wave barrier
+    // it reads T_re[j][i] and T_im[j][i] at the same place and accumulates
+    // its share of sum_ij T_ij T_ji per column (products in FP64: the terms
+    // cancel strongly): a re lane adds (x T_re[j][i], x T_im[j][i]), an im
+    // lane (-x T_im[j][i], x T_re[j][i]), i.e. the same two products read
+    // from swapped places, the first negated once at the end.  Every address
+    // is a lane base plus a compile-time offset; only the 16-row tiles that
+    // meet the g block are visited, and only tiles that straddle its edge
+    // test their rows.  Columns outside the tile write to a dummy area and
+    // read valid (ignored) data.  Column sums are then reduced per walker.
+    int wbase[5], pbase[5], qbase[5];
+    bool cimv[5];
+    {
+      int lj = li;  // opaque per tile, as in slots()
+      asm volatile("" : "+v"(lj));
 #pragma unroll
-  for (int w = 0; w < 5; ++w) {
-    const int col = 16 * w + li;
-    cv[w] = col < NC;
-    const int wi = cv[w] ? col / (2 * N) : 0;
-    const int rem = col - wi * 2 * N;
-    cim_[w] = rem >= N;
-    const int j = rem - (cim_[w] ? N : 0);
-    wb[w] = ((wi * 2 + (cim_[w] ? 1 : 0)) * N + j) * NP;
-    rb0[w] = (wi * 2) * N * NP + j;
-    rb1[w] = (wi * 2 + 1) * N * NP + j;
-  }
-  double cre[5], cimg[5];
-#pragma unroll
-  for (int w = 0; w < 5; ++w) cre[w] = cimg[w] = 0.0;
-  const int gw0 = (tr * 2 + wr) * a.ng;
-  for (int gi = 0; gi < a.ng; ++gi) {
-    if (gw0 + gi >= a.G) break;  // wave-uniform
-#pragma unroll
-    for (int u = 0; u < 5; ++u)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * u + 4 * lq + r - gi * N;
-        if (i >= 0 && i < N) {
-#pragma unroll
-          for (int w = 0; w < 5; ++w)
-            if (cv[w]) Rt[wb[w] + i] = acc[u][w][r];
-        }
+      for (int w = 0; w < 5; ++w) {
+        const int col = 16 * w + lj;
+        const bool cv = col < NC;
+        const int wi = cv ? (col * a.m2N) >> 16 : 0;
+        const int rem = col - wi * 2 * N;
+        const bool cim = cv && rem >= N;
+        const int j = cv ? rem - (cim ? N : 0) : 0;
+        cimv[w] = cim;
+        wbase[w] = (cv ? ((wi * 2 + (cim ? 1 : 0)) * N + j) * XNP : XDUM) + 4 * lq;
+        const int r0 = (wi * 2) * N * XNP + j, r1 = (wi * 2 + 1) * N * XNP + j;
+        pbase[w] = (cim ? r1 : r0) + 4 * lq * XNP;
+        qbase[w] = (cim ? r0 : r1) + 4 * lq * XNP;
       }
-    xwave_sync();
+    }
+    double cre[5], cimg[5];
 #pragma unroll
-    for (int u = 0; u < 5; ++u)
+    for (int w = 0; w < 5; ++w) cre[w] = cimg[w] = 0.0;
+    const int gw0 = (tr * 2 + wr) * a.ng;
+    for (int gi = 0; gi < a.ng; ++gi) {
+      if (gw0 + gi >= a.G) break;  // wave-uniform
+      const int lo = gi * N;       // first row of this g block in the wave tile
+      // row i = 16 u + 4 lq + r - lo of the block
+      float* wp[5];
+      const float* pp[5];
+      const float* qp[5];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = 16 * u + 4 * lq + r - gi * N;
-        if (i >= 0 && i < N) {
-          const int iNP = i * NP;
+      for (int w = 0; w < 5; ++w) {
+        wp[w] = Rt + (wbase[w] - lo);
+        pp[w] = Rt + (pbase[w] - lo * XNP);
+        qp[w] = Rt + (qbase[w] - lo * XNP);
+      }
 #pragma unroll
-          for (int w = 0; w < 5; ++w) {
-            if (!cv[w]) continue;
-            const double x = acc[u][w][r];
-            const double tre = Rt[rb0[w] + iNP];  // T_re[j][i]
-            const double tim = Rt[rb1[w] + iNP];  // T_im[j][i]
-            const double p = cim_[w] ? -tim : tre;
-            const double q = cim_[w] ? tre : tim;
-            cre[w] += x * p;
-            cimg[w] += x * q;
+      for (int u = 0; u < 5; ++u) {
+        const int rlo = 16 * u - lo;
+        if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
+        const bool full = rlo >= 0 && rlo + 16 <= N;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = rlo + 4 * lq + r;
+          if (full || (i >= 0 && i < N)) {
+#pragma unroll
+            for (int w = 0; w < 5; ++w) wp[w][16 * u + r] = acc[u][w][r];
           }
         }
       }
-    xwave_sync();
-  }
-  // column sums over the four row groups of the lane layout, then per walker
-  double* cs = (double*)((float*)lds + XNW * XEPI) + wv * 2 * XWT;  // [80][2]
+      xwave_sync();
 #pragma unroll
-  for (int w = 0; w < 5; ++w) {
-    double x = cre[w], y = cimg[w];
-    x += __shfl_xor(x, 16);
-    x += __shfl_xor(x, 32);
-    y += __shfl_xor(y, 16);
-    y += __shfl_xor(y, 32);
-    if (lq == 0) {
-      cs[2 * (16 * w + li)] = x;
-      cs[2 * (16 * w + li) + 1] = y;
+      for (int u = 0; u < 5; ++u) {
+        const int rlo = 16 * u - lo;
+        if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
+        const bool full = rlo >= 0 && rlo + 16 <= N;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = rlo + 4 * lq + r;
+          if (full || (i >= 0 && i < N)) {
+#pragma unroll
+            for (int w = 0; w < 5; ++w) {
+              float xf = acc[u][w][r];
+              asm volatile("" : "+v"(xf));  // convert here, not hoisted out of the g loop
+              const double x = xf;
+              cre[w] += x * (double)pp[w][(16 * u + r) * XNP];
+              cimg[w] += x * (double)qp[w][(16 * u + r) * XNP];
+            }
+          }
+        }
+      }
+      xwave_sync();
     }
-  }
-  xwave_sync();
-  double* wsum = (double*)((float*)lds + XNW * XEPI) + XNW * 2 * XWT;  // [XNW][nw][2]
-  if (l < a.nw) {
-    double x = 0.0, y = 0.0;
-    for (int c = 2 * N * l; c < 2 * N * (l + 1); ++c) {
-      x += cs[2 * c];
-      y += cs[2 * c + 1];
+#pragma unroll
+    for (int w = 0; w < 5; ++w)
+      if (cimv[w]) cre[w] = -cre[w];
+    // column sums over the four row groups of the lane layout, then per walker
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+      double x = cre[w], y = cimg[w];
+      x += __shfl_xor(x, 16);
+      x += __shfl_xor(x, 32);
+      y += __shfl_xor(y, 16);
+      y += __shfl_xor(y, 32);
+      if (lq == 0) {
+        cs[2 * (16 * w + li)] = x;
+        cs[2 * (16 * w + li) + 1] = y;
+      }
     }
-    wsum[(wv * a.nw + l) * 2] = x;
-    wsum[(wv * a.nw + l) * 2 + 1] = y;
-  }
-  __syncthreads();
-  // the two row waves of a column hold disjoint g ranges of the same walkers
-  if (t < 2 * a.nw) {
-    const int wcc = t / a.nw, wi = t % a.nw;
-    const int w = (tc * 2 + wcc) * a.nw + wi;
-    if (w < a.W) {
-      const double re = wsum[((wcc * a.nw) + wi) * 2] + wsum[(((2 + wcc) * a.nw) + wi) * 2];
-      const double im = wsum[((wcc * a.nw) + wi) * 2 + 1] + wsum[(((2 + wcc) * a.nw) + wi) * 2 + 1];
-      a.part[((long)tr * a.W + w) * 2] = re;
-      a.part[((long)tr * a.W + w) * 2 + 1] = im;
+    xwave_sync();
+    if (l < a.nw) {
+      double x = 0.0, y = 0.0;
+      for (int c = 2 * N * l; c < 2 * N * (l + 1); ++c) {
+        x += cs[2 * c];
+        y += cs[2 * c + 1];
+      }
+      wsum[(wv * a.nw + l) * 2] = x;
+      wsum[(wv * a.nw + l) * 2 + 1] = y;
     }
+    __syncthreads();
+    // the two row waves of a column hold disjoint g ranges of the same walkers
+    if (t < 2 * a.nw) {
+      const int wcc = t / a.nw, wi = t % a.nw;
+      const int w = (tc * 2 + wcc) * a.nw + wi;
+      if (w < a.W) {
+        const double re = wsum[((wcc * a.nw) + wi) * 2] + wsum[(((2 + wcc) * a.nw) + wi) * 2];
+        const double im =
+            wsum[((wcc * a.nw) + wi) * 2 + 1] + wsum[(((2 + wcc) * a.nw) + wi) * 2 + 1];
+        a.part[((long)tr * a.W + w) * 2] = re;
+        a.part[((long)tr * a.W + w) * 2 + 1] = im;
+      }
+    }
+    __syncthreads();  // the epilogue area is the staging of the next tile
   }
 }
 
-// e_x[w] = 1/2 sum_s sum_tr part_s[tr][w], in a fixed order
-__global__ void k_exch_reduce(int W, int ntra, const double* __restrict__ pa, int ntrb,
-                              const double* __restrict__ pb, double* __restrict__ ex) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= W) return;
+// e_x[w] = 1/2 sum_s sum_tr part_s[tr][w], in a fixed order: a block of
+// 64 walkers x 4 row-tile lanes; lane q sums the row tiles r = q mod 4 of
+// both spins, then the four lane sums are added in order.
+__global__ __launch_bounds__(256) void k_exch_reduce(int W, int ntra, const double* __restrict__ pa,
+                                                     int ntrb, const double* __restrict__ pb,
+                                                     double* __restrict__ ex) {
+  __shared__ double red[4][64][2];
+  const int wl = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + wl;
   double re = 0.0, im = 0.0;
-  for (int r = 0; r < ntra; ++r) {
-    re += pa[((long)r * W + w) * 2];
-    im += pa[((long)r * W + w) * 2 + 1];
+  if (w < W) {
+    for (int r = q; r < ntra; r += 4) {
+      re += pa[((long)r * W + w) * 2];
+      im += pa[((long)r * W + w) * 2 + 1];
+    }
+    for (int r = q; r < ntrb; r += 4) {
+      re += pb[((long)r * W + w) * 2];
+      im += pb[((long)r * W + w) * 2 + 1];
+    }
   }
-  for (int r = 0; r < ntrb; ++r) {
-    re += pb[((long)r * W + w) * 2];
-    im += pb[((long)r * W + w) * 2 + 1];
+  red[q][wl][0] = re;
+  red[q][wl][1] = im;
+  __syncthreads();
+  if (q == 0 && w < W) {
+    ex[2 * w] = 0.5 * (((red[0][wl][0] + red[1][wl][0]) + red[2][wl][0]) + red[3][wl][0]);
+    ex[2 * w + 1] = 0.5 * (((red[0][wl][1] + red[1][wl][1]) + red[2][wl][1]) + red[3][wl][1]);
   }
-  ex[2 * w] = 0.5 * re;
-  ex[2 * w + 1] = 0.5 * im;
 }
 
 // e_c[w] = 1/2 sum_g X_g^2 with X_g = sum of the split-K force-bias partials
@@ -975,10 +1066,10 @@ void exch_geom(int N, int G, int W, int* ng, int* nw, int* ntr, int* ntc) {
 }
 
 size_t exch_lds_bytes() {
-  const size_t stage = (size_t)(4 * XRT) * LROW * sizeof(bf16_t);
+  const size_t stages = (size_t)2 * XSTAGE * sizeof(bf16_t);
   const size_t epi = (size_t)XNW * XEPI * sizeof(float) + (size_t)XNW * 2 * XWT * sizeof(double) +
                      (size_t)XNW * XWT * 2 * sizeof(double);
-  return std::max(stage, epi);
+  return std::max(stages, epi);
 }
 
 }  // namespace
@@ -1009,8 +1100,8 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     if (Ns[sp] == 0 || b.G == 0) continue;
     Bf3Exch a{};
     const long aplane = (long)b.G * Ks[sp], bplane = 2L * W * Ks[sp];
-    if (2 * aplane >= (1L << 31) || 2 * bplane >= (1L << 31))
-      return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: operand planes exceed 2^31 elements");
+    if (4 * aplane >= (1L << 31) || 4 * bplane >= (1L << 31))
+      return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: operand planes exceed 2^31 bytes");
     a.a = lh[sp];
     a.b = gh[sp];
     a.aplane = (int)aplane;
@@ -1021,17 +1112,23 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     a.G = b.G;
     a.W = W;
     a.ldq = b.ldq;
-    a.Mp = (b.ldq + GK - 1) / GK * GK;
     exch_geom(a.N, b.G, W, &a.ng, &a.nw, &a.ntr, &a.ntc);
     a.part = part + (sp == 1 ? (size_t)ntrs[0] * W * 2 : 0);
     parts[sp] = a.part;
     ntrs[sp] = a.ntr;
     const long nb = (long)cdiv_int(a.ntr, XGR) * XGR * a.ntc;
-    if (nb > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: grid too large");
-    hipLaunchKernelGGL(k_exch_bf3, dim3((unsigned)nb), dim3(XNT), sm, s, a);
+    if (nb > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: too many tiles");
+    a.nb = (int)nb;
+    a.mN = (65536 + a.N - 1) / a.N;
+    a.m2N = (65536 + 2 * a.N - 1) / (2 * a.N);
+    if (a.ldq % GK) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: row stride %d", a.ldq);
+    // persistent: two workgroups per CU, a multiple of 8 (one share per XCD)
+    const long nwg = std::min<long>((long)num_cus() * 2, (nb + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_exch_bf3, dim3((unsigned)std::max<long>(8, nwg / 8 * 8)), dim3(XNT), sm,
+                       s, a);
     AFQ_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, ntrs[0], parts[0],
+  hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 64)), dim3(256), 0, s, W, ntrs[0], parts[0],
                      ntrs[1], parts[1], ex);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;

@@ -141,6 +141,8 @@ int hip_fail(hipError_t e, const char* what);
 void* scratch(afq_ctx* ctx, size_t bytes);
 // Largest dynamic LDS allocation a workgroup may request on the current device.
 size_t max_lds_bytes();
+// compute units of the current device
+int num_cus();
 
 }  // namespace afq
 

@@ -58,6 +58,17 @@ size_t max_lds_bytes() {
   return cached;
 }
 
+int num_cus() {
+  static int cached = 0;
+  if (cached) return cached;
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+    v = 256;
+  cached = v;
+  return cached;
+}
+
 }  // namespace afq
 
 extern "C" {
