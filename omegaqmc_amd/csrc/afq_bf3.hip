@@ -707,11 +707,12 @@ namespace {
 constexpr int XNW = 4, XNT = 64 * XNW, XWT = 80, XRT = 2 * XWT, XCT = 2 * XWT, XGR = 8;
 constexpr int XMAXN = XWT / 2;
 constexpr int XSLOT = (2 * XRT) * 4 / XNT;  // 16-byte staging slots per thread and operand
-constexpr int XNP = XMAXN + 1;              // row stride of the transposed block
-// floats of one wave's epilogue block: 80 rows of XNP and a dummy area that
-// absorbs the writes of columns outside the tile
-constexpr int XDUM = XWT * XNP + 16;
-constexpr int XEPI = XDUM + 112;
+// One wave's epilogue block: the transposed T of one g block as (re, im)
+// float2 pairs, rows (walker, j) of XS2 pairs (XMAXN rows), then a dummy area
+// that absorbs the writes of rows and columns outside the tile.
+constexpr int XS2 = XMAXN + 1;
+constexpr int XDUM = 2 * XMAXN * XS2;  // floats
+constexpr int XEPI = XDUM + 144;
 // one staging buffer: planes (A hi, A lo, B hi, B lo) of XRT rows x GK, rows
 // of 64 bytes in blocks of 16 rows written by one LDS-DMA instruction each
 constexpr int XSTAGE = 4 * XRT * GK;  // bf16
@@ -878,18 +879,18 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
     }
 
     // Epilogue.  The lane holding T_c[i][j] (c = re/im) of walker wi writes
-    // it transposed, Rt[wi][c][j][i] (row stride XNP); after a wave barrier
-    // it reads T_re[j][i] and T_im[j][i] at the same place and accumulates
-    // its share of sum_ij T_ij T_ji per column (products in FP64: the terms
-    // cancel strongly): a re lane adds (x T_re[j][i], x T_im[j][i]), an im
-    // lane (-x T_im[j][i], x T_re[j][i]), i.e. the same two products read
-    // from swapped places, the first negated once at the end.  Every address
-    // is a lane base plus a compile-time offset; only the 16-row tiles that
-    // meet the g block are visited, and only tiles that straddle its edge
-    // test their rows.  Columns outside the tile write to a dummy area and
-    // read valid (ignored) data.  Column sums are then reduced per walker.
-    int wbase[5], pbase[5], qbase[5];
-    bool cimv[5];
+    // it transposed into the wave's block, Rt2[wi][j][i].c as (re, im)
+    // pairs; after a wave barrier one 8-byte read at Rt2[wi][i][j] returns
+    // (T_re[j][i], T_im[j][i]) = (p, q) and the lane accumulates A += x p,
+    // B += x q.  sum_ij T_ij T_ji then takes (A, B) from re lanes and
+    // (-B, A) from im lanes.  Products and the lane's partial sums of one g
+    // block (at most 16 terms) are FP32, the sums over g blocks, lanes and
+    // tiles FP64.  Full 16-row tiles are straight-line code with compile-time
+    // LDS offsets; the tiles that straddle a g-block edge clamp their rows
+    // (zero weight) and send the writes of rows outside the block, like those
+    // of columns outside the tile, to the dummy area.
+    int wb2[5], rf2[5];
+    bool cvv[5], cimv[5];
     {
       int lj = li;  // opaque per tile, as in slots()
       asm volatile("" : "+v"(lj));
@@ -901,70 +902,91 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
         const int rem = col - wi * 2 * N;
         const bool cim = cv && rem >= N;
         const int j = cv ? rem - (cim ? N : 0) : 0;
+        cvv[w] = cv;
         cimv[w] = cim;
-        wbase[w] = (cv ? ((wi * 2 + (cim ? 1 : 0)) * N + j) * XNP : XDUM) + 4 * lq;
-        const int r0 = (wi * 2) * N * XNP + j, r1 = (wi * 2 + 1) * N * XNP + j;
-        pbase[w] = (cim ? r1 : r0) + 4 * lq * XNP;
-        qbase[w] = (cim ? r0 : r1) + 4 * lq * XNP;
+        wb2[w] = ((wi * N + j) * XS2 + 4 * lq) * 2 + (cim ? 1 : 0);  // floats, + 2 (i - 4 lq)
+        rf2[w] = (wi * N + 4 * lq) * XS2 + j;  // pairs, + (i - 4 lq) XS2
       }
     }
-    double cre[5], cimg[5];
+    const float2* Rt2 = (const float2*)Rt;
+    double cA[5], cB[5];
 #pragma unroll
-    for (int w = 0; w < 5; ++w) cre[w] = cimg[w] = 0.0;
+    for (int w = 0; w < 5; ++w) cA[w] = cB[w] = 0.0;
     const int gw0 = (tr * 2 + wr) * a.ng;
     for (int gi = 0; gi < a.ng; ++gi) {
       if (gw0 + gi >= a.G) break;  // wave-uniform
       const int lo = gi * N;       // first row of this g block in the wave tile
-      // row i = 16 u + 4 lq + r - lo of the block
-      float* wp[5];
-      const float* pp[5];
-      const float* qp[5];
+      // row of the block for (u, r): i = 16 u - lo + 4 lq + r
+      int wcur[5];
 #pragma unroll
-      for (int w = 0; w < 5; ++w) {
-        wp[w] = Rt + (wbase[w] - lo);
-        pp[w] = Rt + (pbase[w] - lo * XNP);
-        qp[w] = Rt + (qbase[w] - lo * XNP);
-      }
+      for (int w = 0; w < 5; ++w) wcur[w] = cvv[w] ? wb2[w] - 2 * lo : XDUM;
 #pragma unroll
       for (int u = 0; u < 5; ++u) {
         const int rlo = 16 * u - lo;
         if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
-        const bool full = rlo >= 0 && rlo + 16 <= N;
+        if (rlo >= 0 && rlo + 16 <= N) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = rlo + 4 * lq + r;
-          if (full || (i >= 0 && i < N)) {
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int w = 0; w < 5; ++w) wp[w][16 * u + r] = acc[u][w][r];
+            for (int w = 0; w < 5; ++w) Rt[wcur[w] + 2 * (16 * u + r)] = acc[u][w][r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = rlo + 4 * lq + r;
+            const bool ok = i >= 0 && i < N;
+#pragma unroll
+            for (int w = 0; w < 5; ++w)
+              Rt[(ok ? wcur[w] : XDUM) + 2 * (16 * u + r)] = acc[u][w][r];
           }
         }
       }
       xwave_sync();
+      float A[5], B[5];
+#pragma unroll
+      for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
 #pragma unroll
       for (int u = 0; u < 5; ++u) {
         const int rlo = 16 * u - lo;
         if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
-        const bool full = rlo >= 0 && rlo + 16 <= N;
+        if (rlo >= 0 && rlo + 16 <= N) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = rlo + 4 * lq + r;
-          if (full || (i >= 0 && i < N)) {
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
-              float xf = acc[u][w][r];
-              asm volatile("" : "+v"(xf));  // convert here, not hoisted out of the g loop
-              const double x = xf;
-              cre[w] += x * (double)pp[w][(16 * u + r) * XNP];
-              cimg[w] += x * (double)qp[w][(16 * u + r) * XNP];
+              const float x = acc[u][w][r];
+              const float2 v = Rt2[rf2[w] - lo * XS2 + (16 * u + r) * XS2];
+              A[w] = fmaf(x, v.x, A[w]);
+              B[w] = fmaf(x, v.y, B[w]);
+            }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = rlo + 4 * lq + r;
+            const bool ok = i >= 0 && i < N;
+            const int ic = min(max(i, 0), N - 1);
+#pragma unroll
+            for (int w = 0; w < 5; ++w) {
+              const float x = ok ? acc[u][w][r] : 0.f;
+              const float2 v = Rt2[rf2[w] + (ic - 4 * lq) * XS2];
+              A[w] = fmaf(x, v.x, A[w]);
+              B[w] = fmaf(x, v.y, B[w]);
             }
           }
         }
       }
+#pragma unroll
+      for (int w = 0; w < 5; ++w) {
+        cA[w] += (double)A[w];
+        cB[w] += (double)B[w];
+      }
       xwave_sync();
     }
+    double cre[5], cimg[5];
 #pragma unroll
-    for (int w = 0; w < 5; ++w)
-      if (cimv[w]) cre[w] = -cre[w];
+    for (int w = 0; w < 5; ++w) {
+      cre[w] = cimv[w] ? -cB[w] : cA[w];
+      cimg[w] = cimv[w] ? cA[w] : cB[w];
+    }
     // column sums over the four row groups of the lane layout, then per walker
 #pragma unroll
     for (int w = 0; w < 5; ++w) {
