@@ -42,6 +42,25 @@ __device__ __forceinline__ f4v mfma3(const uint4& ah, const uint4& al, const uin
   return mfma_bf(ah, bh, c);
 }
 
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// bf16 pair (round to nearest even) packed in 32 bits, element a low
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f2v{a, b}, bf2v));
+}
+// four FP32 values -> hi = bf16(x), lo = bf16(x - hi) (x - hi is exact in FP32)
+__device__ __forceinline__ void split4(f4v v, uint2& hi, uint2& lo) {
+  hi.x = pk_bf16(v[0], v[1]);
+  hi.y = pk_bf16(v[2], v[3]);
+  const float r0 = v[0] - __uint_as_float(hi.x << 16);
+  const float r1 = v[1] - __uint_as_float(hi.x & 0xffff0000u);
+  const float r2 = v[2] - __uint_as_float(hi.y << 16);
+  const float r3 = v[3] - __uint_as_float(hi.y & 0xffff0000u);
+  lo.x = pk_bf16(r0, r1);
+  lo.y = pk_bf16(r2, r3);
+}
+
 __device__ __forceinline__ uint4 neg8(uint4 a) {
   return make_uint4(a.x ^ 0x80008000u, a.y ^ 0x80008000u, a.z ^ 0x80008000u, a.w ^ 0x80008000u);
 }
@@ -441,28 +460,32 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       }
     }
   };
-  // acc -> LDS planes (rows < M)
+  // acc -> LDS planes (rows < M), four rows of a lane at a time, split in
+  // FP32 (x - bf16(x) is exact in FP32: the same planes as the FP64 split)
   auto to_lds = [&](f4v (*xr)[3], f4v (*xi)[3]) {
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
       if (!owns(u)) continue;
       const int r0 = 16 * (wv + PW * u) + 4 * lq;
+      const bool edge = 16 * (wv + PW * u) + 16 > M;  // uniform: the tile crosses row M
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        bf16_t q[4][4];
+        f4v vr = xr[u][x], vi = xi[u][x];
+        if (edge) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const bool ok = r0 + r < M;
-          split2(ok ? (double)xr[u][x][r] : 0.0, q[0][r], q[1][r]);
-          split2(ok ? (double)xi[u][x][r] : 0.0, q[2][r], q[3][r]);
+          for (int r = 0; r < 4; ++r) {
+            const bool ok = r0 + r < M;
+            vr[r] = ok ? vr[r] : 0.f;
+            vi[r] = ok ? vi[r] : 0.f;
+          }
         }
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          uint2 pk;
-          pk.x = q[p][0] | ((unsigned)q[p][1] << 16);
-          pk.y = q[p][2] | ((unsigned)q[p][3] << 16);
-          *(uint2*)(T(p, 16 * x + li) + r0) = pk;
-        }
+        uint2 hr, lr, hi, li2;
+        split4(vr, hr, lr);
+        split4(vi, hi, li2);
+        *(uint2*)(T(0, 16 * x + li) + r0) = hr;
+        *(uint2*)(T(1, 16 * x + li) + r0) = lr;
+        *(uint2*)(T(2, 16 * x + li) + r0) = hi;
+        *(uint2*)(T(3, 16 * x + li) + r0) = li2;
       }
     }
   };
