@@ -324,22 +324,46 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 
   auto T = [&](int p, int c) { return Ts + (p * PC + c) * KR; };
 
-  // stage phi (FP64) as four bf16 planes, zero outside M x N
-  for (int e = t; e < PC * Kp; e += PT) {
-    const int c = e % PC, r = e / PC;
-    double re = 0.0, im = 0.0;
-    if (r < M && c0 + c < N) {
-      const double* z = phi + 2 * (((long)w * M + r) * N + c0 + c);
-      re = z[0];
-      im = z[1];
+  // stage phi (FP64) as four bf16 planes, zero outside M x N: an item is a
+  // row pair (2 rp, 2 rp + 1) of one column, written as one 32-bit word per
+  // plane; SB items per thread have their loads in flight together (clamped
+  // in-bounds addresses, zeroed after the load)
+  {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    constexpr int SB = 5;
+    const int np = PC * (Kp / 2);
+    const d2v* pw = (const d2v*)phi + (long)w * M * N;
+    for (int e0 = t; e0 < np; e0 += PT * SB) {
+      d2v v[SB][2];
+#pragma unroll
+      for (int k = 0; k < SB; ++k) {
+        const int e = e0 + PT * k;
+        const int c = e % PC, rp = e / PC;
+        const int cc = min(c0 + c, N - 1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = 2 * rp + h;
+          const d2v x = pw[(long)min(r, M - 1) * N + cc];
+          const bool ok = e < np && r < M && c0 + c < N;
+          v[k][h] = ok ? x : d2v{0.0, 0.0};
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < SB; ++k) {
+        const int e = e0 + PT * k;
+        if (e >= np) continue;
+        const int c = e % PC, rp = e / PC;
+        bf16_t q[4][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          split2(v[k][h].x, q[0][h], q[1][h]);
+          split2(v[k][h].y, q[2][h], q[3][h]);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          *(unsigned*)(T(p, c) + 2 * rp) = q[p][0] | ((unsigned)q[p][1] << 16);
+      }
     }
-    bf16_t a, b, cc, d;
-    split2(re, a, b);
-    split2(im, cc, d);
-    T(0, c)[r] = a;
-    T(1, c)[r] = b;
-    T(2, c)[r] = cc;
-    T(3, c)[r] = d;
   }
   __syncthreads();
 
