@@ -24,8 +24,6 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int GJ_NMAX = 40;
 constexpr int GJ_LD = GJ_NMAX + 1;
-// LDS of one wavefront: S^-1 (GJ_NMAX x GJ_LD complex), the pivot row, the pivot order
-constexpr int GJ_WAVE_LDS = (GJ_NMAX * GJ_LD + GJ_NMAX) * 16 + 64 * 4;
 
 // ----------------------------------------------------------------------------
 // Register Gauss-Jordan path (identity trials, N <= 40): one wavefront per
@@ -114,43 +112,62 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
 // Column tiles (16 orbitals) go in pairs with two static fragment buffers: the
 // phi fragments of the next tile are loaded while the current one runs its
 // N/16 row tiles (each phi fragment feeds every row tile).
-template <int KQ>
-__device__ __forceinline__ void ghalf_tile(const cplx* Inv, int LD, int N, int M, int tj,
+template <int KQ, int TI>
+__device__ __forceinline__ void ghalf_tile(const cplx (&ia)[TI][KQ], int N, int M, int tj,
                                            const cplx (&b)[KQ], cplx* __restrict__ Ghalf,
                                            bf16_t* __restrict__ gb, long Ks, long gplane,
-                                           int ldq) {
+                                           int ldq, bf16_t* sb) {
   const int l = threadIdx.x & 63;
   const int li = l & 15, lq = l >> 4;
   const int RTn = (N + 15) >> 4;
   const int q = 16 * tj + li;
-  for (int ti = 0; ti < RTn; ++ti) {
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti) {
+    if (ti >= RTn) break;  // uniform
     d4v cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-    const int ai = 16 * ti + li;
 #pragma unroll
     for (int kk = 0; kk < KQ; ++kk) {
-      const int k = 4 * kk + lq;
-      const cplx a = (ai < N && k < N) ? Inv[ai * LD + k] : mkc(0.0, 0.0);
+      const cplx a = ia[ti][kk];
       cr = mfma64(a.x, b[kk].x, cr);
       ci = mfma64(a.x, b[kk].y, ci);
       cr = mfma64(-a.y, b[kk].y, cr);
       ci = mfma64(a.y, b[kk].x, ci);
     }
-    if (q >= M) continue;
+    if (Ghalf && q < M) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 16 * ti + lq + 4 * r;
-      if (i >= N) continue;
-      if (Ghalf) Ghalf[(long)i * M + q] = mkc(cr[r], ci[r]);
-      if (gb) {
-        const long k = (long)i * ldq + q;
-        bf16_t h, lo;
-        split2(cr[r], h, lo);
-        gb[k] = h;
-        gb[gplane + k] = lo;
-        split2(ci[r], h, lo);
-        gb[Ks + k] = h;
-        gb[gplane + Ks + k] = lo;
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * ti + lq + 4 * r;
+        if (i < N) Ghalf[(long)i * M + q] = mkc(cr[r], ci[r]);
       }
+    }
+    if (gb) {
+      // split planes of the 16 x 16 tile through the wave's LDS staging
+      // (sb[plane][row][col]), then 16-byte row chunks to global memory;
+      // columns past M are written as the zeros the padding must hold
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int il = lq + 4 * r;
+        bf16_t h0, l0, h1, l1;
+        split2(q < M ? cr[r] : 0.0, h0, l0);
+        split2(q < M ? ci[r] : 0.0, h1, l1);
+        sb[(0 * 16 + il) * 16 + li] = h0;
+        sb[(1 * 16 + il) * 16 + li] = l0;
+        sb[(2 * 16 + il) * 16 + li] = h1;
+        sb[(3 * 16 + il) * 16 + li] = l1;
+      }
+      wave_sync();
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int chunk = l + 64 * hh;  // (plane, row, half) of 8 bf16
+        const int p = chunk >> 5, il = (chunk >> 1) & 15, half = chunk & 1;
+        const int i = 16 * ti + il;
+        if (i < N) {
+          const long off = (p & 1 ? gplane : 0) + (p & 2 ? Ks : 0) + (long)i * ldq + 16 * tj +
+                           8 * half;
+          *(uint4*)(gb + off) = *(const uint4*)(sb + (p * 16 + il) * 16 + 8 * half);
+        }
+      }
+      wave_sync();
     }
   }
 }
@@ -168,20 +185,34 @@ __device__ __forceinline__ void ghalf_frag(const cplx* __restrict__ phi, int N, 
   }
 }
 
+// The S^-1 fragments of every row tile (row 16 ti + li, column 4 kk + lq) are
+// read from LDS once and held in registers for all column tiles.  gb (split
+// planes) needs sb: 2048 bytes of LDS per wavefront.
 template <int KQ>
 __device__ __forceinline__ void ghalf_mfma_wave(const cplx* Inv, int LD, int N, int M,
                                                 const cplx* __restrict__ phi,
                                                 cplx* __restrict__ Ghalf, bf16_t* __restrict__ gb,
-                                                long Ks, long gplane, int ldq) {
+                                                long Ks, long gplane, int ldq, bf16_t* sb) {
+  constexpr int TI = (4 * KQ + 15) / 16;
+  const int l = threadIdx.x & 63;
+  const int li = l & 15, lq = l >> 4;
+  cplx ia[TI][KQ];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int kk = 0; kk < KQ; ++kk) {
+      const int ai = 16 * ti + li, k = 4 * kk + lq;
+      ia[ti][kk] = (ai < N && k < N) ? Inv[ai * LD + k] : mkc(0.0, 0.0);
+    }
   const int RTm = (M + 15) >> 4;
   cplx b0[KQ], b1[KQ];
   ghalf_frag<KQ>(phi, N, M, 0, b0);
   for (int tj = 0; tj < RTm; tj += 2) {
     if (tj + 1 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 1, b1);
-    ghalf_tile<KQ>(Inv, LD, N, M, tj, b0, Ghalf, gb, Ks, gplane, ldq);
+    ghalf_tile<KQ, TI>(ia, N, M, tj, b0, Ghalf, gb, Ks, gplane, ldq, sb);
     if (tj + 1 >= RTm) break;
     if (tj + 2 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 2, b0);
-    ghalf_tile<KQ>(Inv, LD, N, M, tj + 1, b1, Ghalf, gb, Ks, gplane, ldq);
+    ghalf_tile<KQ, TI>(ia, N, M, tj + 1, b1, Ghalf, gb, Ks, gplane, ldq, sb);
   }
 }
 

@@ -257,11 +257,19 @@ __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, 
 
 using gj::GJ_LD;
 using gj::GJ_NMAX;
-using gj::GJ_WAVE_LDS;
+
+// LDS of one wavefront of k_greens_gj: the pivot row and order, plus for
+// Ghalf S^-1 (N x GJ_LD complex) and the 2 KB staging of the split planes.
+size_t gj_wave_lds(int mode, int N, bool planes) {
+  return 16UL * GJ_NMAX + 256 + ((mode & 1) ? 16UL * N * GJ_LD : 0) +
+         (((mode & 1) && planes) ? 2048 : 0);
+}
 
 // One walker per workgroup of two wavefronts, wave s = spin s; identity trials.
-template <int NB>
-__global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int Nb,
+// GH: the Ghalf modes (a separate instantiation, so that the overlap-only
+// kernel keeps the small register footprint of the elimination alone).
+template <int NB, bool GH>
+__global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int Nb, int wlds,
                                                    const cplx* __restrict__ phia,
                                                    const cplx* __restrict__ phib,
                                                    cplx* __restrict__ Gha, cplx* __restrict__ Ghb,
@@ -272,10 +280,11 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
   const long w = blockIdx.x;
   const long W = gridDim.x;
   const int N = wv ? Nb : Na;
-  char* base = smem + wv * GJ_WAVE_LDS;
-  cplx* Inv = (cplx*)base;
-  cplx* bro = Inv + GJ_NMAX * GJ_LD;
+  char* base = smem + (long)wv * wlds;
+  cplx* bro = (cplx*)base;
   int* order = (int*)(bro + GJ_NMAX);
+  cplx* Inv = (cplx*)(order + 64);
+  bf16_t* sb = (bf16_t*)(Inv + ((mode & 1) ? N * GJ_LD : 0));
   const cplx* phi = (wv ? phib + w * M * Nb : phia + w * M * Na);
   if (N == 0) {
     if (l == 0) ov[wv] = mkc(1.0, 0.0);
@@ -286,14 +295,14 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
     for (int j = 0; j < NB; ++j) a[j] = (l < N && j < N) ? phi[(long)j * N + l] : mkc(0.0, 0.0);
     int mystep = 0;
     gj::wave_gj<NB>(a, N, bro, order, mystep, &ov[wv]);
-    if (mode & 1) {
+    if (GH && (mode & 1)) {
       gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
       const bool gh = true;
       cplx* G = wv ? Ghb : Gha;
       bf16_t* gb = wv ? gbf.b : gbf.a;
       const long Ks = wv ? gbf.Kb : gbf.Ka;
       gj::ghalf_mfma_wave<(NB + 3) / 4>(Inv, GJ_LD, N, M, phi, (gh && G) ? G + w * N * M : nullptr,
-                                    gb ? gb + 2 * w * Ks : nullptr, Ks, 2 * W * Ks, gbf.ld);
+                                    gb ? gb + 2 * w * Ks : nullptr, Ks, 2 * W * Ks, gbf.ld, sb);
     }
   }
   __syncthreads();
@@ -328,9 +337,19 @@ int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const d
   }
   const bool eye_both = ((mode >> 1) & 1 || Na == 0) && ((mode >> 2) & 1 || Nb == 0);
   const int Nmax = Na > Nb ? Na : Nb;
-  if (eye_both && Nmax <= GJ_NMAX && 2 * GJ_WAVE_LDS + 64 <= (long)max_lds_bytes()) {
-    const size_t sm2 = 2 * (size_t)GJ_WAVE_LDS;
-#define AFQ_GJ(NB)                                                                            hipLaunchKernelGGL((k_greens_gj<NB>), dim3(W), dim3(128), sm2, s, mode, M, Na, Nb,                              (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb,                                 (cplx*)ovlp, g)
+  const bool planes = g.a || g.b;
+  const size_t wl = (gj_wave_lds(mode, Nmax, planes) + 15) / 16 * 16;
+  if (eye_both && Nmax <= GJ_NMAX && 2 * wl + 64 <= max_lds_bytes()) {
+    const size_t sm2 = 2 * wl;
+#define AFQ_GJ(NB)                                                                          \
+  if (mode & 1)                                                                               \
+    hipLaunchKernelGGL((k_greens_gj<NB, true>), dim3(W), dim3(128), sm2, s, mode, M, Na, Nb,  \
+                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb, \
+                       (cplx*)ovlp, g);                                                       \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_greens_gj<NB, false>), dim3(W), dim3(128), sm2, s, mode, M, Na, Nb, \
+                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb, \
+                       (cplx*)ovlp, g)
     if (Nmax <= 8) AFQ_GJ(8);
     else if (Nmax <= 16) AFQ_GJ(16);
     else if (Nmax <= 24) AFQ_GJ(24);

@@ -40,7 +40,6 @@ namespace {
 
 using gj::GJ_LD;
 using gj::GJ_NMAX;
-using gj::GJ_WAVE_LDS;
 
 constexpr double MASK_RTOL = 1e-14;   // greens_th4.py:24
 constexpr double MASK_ATOL = 1e-300;  // greens_th4.py:25
@@ -87,7 +86,7 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
   if (mode == 1) {
     gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
     cplx* G = (wv ? ghb : gha) + ((long)I * W + w) * N * M;
-    gj::ghalf_mfma_wave<(NB + 3) / 4>(Inv, GJ_LD, N, M, phi, G, nullptr, 0, 0, 0);
+    gj::ghalf_mfma_wave<(NB + 3) / 4>(Inv, GJ_LD, N, M, phi, G, nullptr, 0, 0, 0, nullptr);
   } else if (mode == 2) {
     cplx* iv = (wv ? invb : inva) + ((long)I * W + w) * N * N;
     gj::gj_store_inverse<NB>(a, N, mystep, order, iv, N);
