@@ -15,6 +15,17 @@ __device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
 
 __device__ __forceinline__ double cabs1(cplx a) { return fabs(a.x) + fabs(a.y); }
 
+// 64-bit DPP move (lanes the control does not write keep their value)
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned long long dpp64(unsigned long long x) {
+  const int lo = __builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(x >> 32), (int)(x >> 32), CTRL, ROWS, 0xF, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) {
+  return a > b ? a : b;
+}
+
 // Orders the LDS accesses of one wavefront (lanes exchange data through LDS).
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -29,7 +40,7 @@ constexpr int GJ_LD = GJ_NMAX + 1;
 // Register Gauss-Jordan path (identity trials, N <= 40): one wavefront per
 // spin, lane i holds row i of S in registers.  Step k picks the pivot row p_k
 // of column k among the rows not used yet (first maximal |re| + |im|, as
-// getrf) and eliminates column k from every other row in place (in-place
+// getrf, by a DPP max-reduction over one 64-bit key per lane) and eliminates column k from every other row in place (in-place
 // Gauss-Jordan inversion with the row permutation kept implicit): afterwards
 // row p_k holds row k of S^-1, whose column j sits in the storage column of
 // column p_j.  det S = sign(p) prod_k d_k with d_k the pivots, accumulated as
@@ -55,19 +66,24 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
   bool sing = false;  // an exact zero pivot: det 0 (slogdet sign 0) whatever follows
   for (int k = 0; k < N; ++k) {
     const cplx f = a[0];
-    double v = -1.0;
+    // pivot key: |re| + |im| with its low 6 mantissa bits replaced by 63 - lane
+    // (the first of equal maxima wins); a NaN row ranks below every number
+    // and is taken only if nothing else is left; finished rows rank last
+    unsigned long long key = 0;
     if (active) {
-      v = cabs1(f);
-      if (v != v) v = -0.5;  // a NaN row is taken only if nothing else is left
+      const double v = cabs1(f);
+      key = (v != v) ? (1ull << 6)
+                     : (((__builtin_bit_cast(unsigned long long, v) >> 6) + 2) << 6);
+      key |= (unsigned long long)(63 - l);
     }
-    int bi = l;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov2 = __shfl_xor(v, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov2 > v || (ov2 == v && oi < bi)) { v = ov2; bi = oi; }
-    }
-    const int p = __builtin_amdgcn_readfirstlane(bi);
+    key = umax64(key, dpp64<0xB1>(key));         // quad_perm [1, 0, 3, 2]
+    key = umax64(key, dpp64<0x4E>(key));         // quad_perm [2, 3, 0, 1]
+    key = umax64(key, dpp64<0x141>(key));        // half-row mirror
+    key = umax64(key, dpp64<0x140>(key));        // row mirror
+    key = umax64(key, dpp64<0x142, 0xA>(key));   // row_bcast15 into rows 1, 3
+    key = umax64(key, dpp64<0x143, 0xC>(key));   // row_bcast31 into rows 2, 3
+    const unsigned klo = __builtin_amdgcn_readlane((unsigned)key, 63);
+    const int p = 63 - (int)(klo & 63u);
     const bool me = l == p;
     if (me) {
 #pragma unroll
