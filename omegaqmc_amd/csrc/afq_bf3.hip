@@ -61,6 +61,11 @@ __device__ __forceinline__ void split4(f4v v, uint2& hi, uint2& lo) {
   lo.y = pk_bf16(r2, r3);
 }
 
+// 16-byte chunk position of K chunk c in row li of a 16-row block: the XOR
+// keeps MFMA operand reads (16 rows, one chunk) free of bank conflicts with
+// either 32 or 64 LDS banks
+__device__ __forceinline__ int xswz(int li, int c) { return c ^ (((li >> 1) ^ (li >> 3)) & 3); }
+
 __device__ __forceinline__ uint4 neg8(uint4 a) {
   return make_uint4(a.x ^ 0x80008000u, a.y ^ 0x80008000u, a.z ^ 0x80008000u, a.w ^ 0x80008000u);
 }
@@ -124,7 +129,7 @@ __global__ void k_prep_lh(long n, int N, int M, int ldq, long K, long plane,
 // ----------------------------------------------------------------------------
 // the GEMM core: 256 x 256 tile, K step 32, 8 waves of 64 x 128
 // ----------------------------------------------------------------------------
-constexpr int GB = 256, GK = 32, GT = 512, LROW = GK + 8;
+constexpr int GB = 256, GK = 32, GT = 512;
 
 struct Bf3Split {
   const bf16_t* a[2];  // A planes (rows x lda)
@@ -159,10 +164,10 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   const int row0 = tm * GB, col0 = tn * GB;
   const Bf3Split& S = a.sp[blockIdx.y];
 
-  // two stage buffers: the next K step is stored while this one is consumed
-  constexpr int GSTAGE = 4 * GB * LROW;
-  auto Arow = [&](int buf, int p, int r) { return lds + buf * GSTAGE + (p * GB + r) * LROW; };
-  auto Brow = [&](int buf, int p, int c) { return lds + buf * GSTAGE + ((2 + p) * GB + c) * LROW; };
+  // two stage buffers: the next K step is stored while this one is consumed;
+  // rows of GK with the 16-byte chunks XOR-swizzled within 16-row blocks
+  // (conflict-free MFMA operand reads)
+  constexpr int GSTAGE = 4 * GB * GK;
 
   // staging slots (fixed over K): slot i covers plane i / 2, row
   // ((t + GT i) & 1023) / 4, K chunk t % 4.  Row pointers are formed once so
@@ -193,10 +198,12 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int c = t + GT * i, r = (c & 1023) >> 2, kq = c & 3;
-      *(v4u*)(Arow(buf, i >> 1, r) + 8 * kq) = ra[i];
-      *(v4u*)(Brow(buf, i >> 1, r) + 8 * kq) = rb[i];
+      const int pos = r * GK + 8 * xswz(r & 15, kq);
+      *(v4u*)(lds + buf * GSTAGE + (i >> 1) * GB * GK + pos) = ra[i];
+      *(v4u*)(lds + buf * GSTAGE + (2 + (i >> 1)) * GB * GK + pos) = rb[i];
     }
   };
+  const int frag = li * GK + 8 * xswz(li, lq);  // (row li, chunk lq) of a 16-row block
 
   f4v acc[4][8];
 #pragma unroll
@@ -213,18 +220,19 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   for (int kt = 0; kt < nk; ++kt) {
     if (kt + 1 < nk) load(S.kbeg + (kt + 1) * GK);
     const int buf = kt & 1;
+    const bf16_t* st = lds + buf * GSTAGE + frag;
     uint4 ah[4], al[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int r = 64 * wr + 16 * u + li;
-      ah[u] = *(const uint4*)(Arow(buf, 0, r) + 8 * lq);
-      al[u] = *(const uint4*)(Arow(buf, 1, r) + 8 * lq);
+      const int r = 64 * wr + 16 * u;  // block-aligned row
+      ah[u] = *(const uint4*)(st + r * GK);
+      al[u] = *(const uint4*)(st + (GB + r) * GK);
     }
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
-      const int c = 128 * wc + 16 * w + li;
-      const uint4 bh = *(const uint4*)(Brow(buf, 0, c) + 8 * lq);
-      const uint4 bl = *(const uint4*)(Brow(buf, 1, c) + 8 * lq);
+      const int c = 128 * wc + 16 * w;
+      const uint4 bh = *(const uint4*)(st + (2 * GB + c) * GK);
+      const uint4 bl = *(const uint4*)(st + (3 * GB + c) * GK);
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
     }
@@ -641,7 +649,7 @@ void bf3_release(Bf3System* b) {
   b->Lb = nullptr; b->blk_pq = nullptr; b->Lha = b->Lhb = nullptr; b->E = nullptr;
 }
 
-static size_t gemm_lds() { return (size_t)2 * 4 * GB * LROW * sizeof(bf16_t); }
+static size_t gemm_lds() { return (size_t)2 * 4 * GB * GK * sizeof(bf16_t); }
 
 int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt, bf16_t* Vb) {
   if (W <= 0) return AFQ_OK;
@@ -771,10 +779,6 @@ __device__ void dma16(i32x4 rsrc, __attribute__((address_space(3))) unsigned* ld
                       int voffset, int soffset, int offset, int aux)
     __asm("llvm.amdgcn.raw.buffer.load.lds");
 
-// 16-byte chunk position of K chunk c in row li of a 16-row block: the XOR
-// keeps MFMA operand reads (16 rows, one chunk) free of bank conflicts with
-// either 32 or 64 LDS banks
-__device__ __forceinline__ int xswz(int li, int c) { return c ^ (((li >> 1) ^ (li >> 3)) & 3); }
 
 struct Bf3Exch {
   const bf16_t* a;  // Lhalf hi plane (lo plane at +aplane): row (g, i) at g ka + i ldq
