@@ -52,9 +52,18 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
 typedef unsigned short bf16_t;
 __device__ __forceinline__ bf16_t bf_bits(float x) { return __builtin_bit_cast(bf16_t, (__bf16)x); }
 __device__ __forceinline__ float bf_val(bf16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
+// FP64 -> FP32 (nearest), kept as its own step: the compiler otherwise folds
+// the two conversions of split2 into one exact FP64 -> BF16 rounding, which
+// it emulates with about eight more instructions (three of them FP64).
+__device__ __forceinline__ float f32_of(double x) {
+  float f = (float)x;
+  asm("" : "+v"(f));
+  return f;
+}
+// hi = bf16(f32(x)), lo = bf16(f32(x - hi)), each rounding to nearest even
 __device__ __forceinline__ void split2(double x, bf16_t& h, bf16_t& l) {
-  h = bf_bits((float)x);
-  l = bf_bits((float)(x - (double)bf_val(h)));
+  h = bf_bits(f32_of(x));
+  l = bf_bits(f32_of(x - (double)bf_val(h)));
 }
 
 // Split-bf16 copies of Ghalf written by the Green's-function kernel: plane
