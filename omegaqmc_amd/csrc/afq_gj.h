@@ -202,13 +202,17 @@ __device__ __forceinline__ void ghalf_frag(const cplx* __restrict__ phi, int N, 
 }
 
 // The S^-1 fragments of every row tile (row 16 ti + li, column 4 kk + lq) are
-// read from LDS once and held in registers for all column tiles.  gb (split
+// read once (LDS or global memory) and held in registers for all column
+// tiles tj0, tj0 + tstep, ...  DB: the phi fragments of the next tile are in
+// flight while this one runs (two fragment buffers; without, the registers
+// leave room for a second wavefront per SIMD to cover the loads).  gb (split
 // planes) needs sb: 2048 bytes of LDS per wavefront.
-template <int KQ>
+template <int KQ, bool DB = true>
 __device__ __forceinline__ void ghalf_mfma_wave(const cplx* Inv, int LD, int N, int M,
                                                 const cplx* __restrict__ phi,
                                                 cplx* __restrict__ Ghalf, bf16_t* __restrict__ gb,
-                                                long Ks, long gplane, int ldq, bf16_t* sb) {
+                                                long Ks, long gplane, int ldq, bf16_t* sb,
+                                                int tj0 = 0, int tstep = 1) {
   constexpr int TI = (4 * KQ + 15) / 16;
   const int l = threadIdx.x & 63;
   const int li = l & 15, lq = l >> 4;
@@ -221,14 +225,23 @@ __device__ __forceinline__ void ghalf_mfma_wave(const cplx* Inv, int LD, int N, 
       ia[ti][kk] = (ai < N && k < N) ? Inv[ai * LD + k] : mkc(0.0, 0.0);
     }
   const int RTm = (M + 15) >> 4;
+  if (!DB) {
+    cplx b[KQ];
+    for (int tj = tj0; tj < RTm; tj += tstep) {
+      ghalf_frag<KQ>(phi, N, M, tj, b);
+      ghalf_tile<KQ, TI>(ia, N, M, tj, b, Ghalf, gb, Ks, gplane, ldq, sb);
+    }
+    return;
+  }
   cplx b0[KQ], b1[KQ];
-  ghalf_frag<KQ>(phi, N, M, 0, b0);
-  for (int tj = 0; tj < RTm; tj += 2) {
-    if (tj + 1 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 1, b1);
+  if (tj0 >= RTm) return;
+  ghalf_frag<KQ>(phi, N, M, tj0, b0);
+  for (int tj = tj0; tj < RTm; tj += 2 * tstep) {
+    if (tj + tstep < RTm) ghalf_frag<KQ>(phi, N, M, tj + tstep, b1);
     ghalf_tile<KQ, TI>(ia, N, M, tj, b0, Ghalf, gb, Ks, gplane, ldq, sb);
-    if (tj + 1 >= RTm) break;
-    if (tj + 2 < RTm) ghalf_frag<KQ>(phi, N, M, tj + 2, b0);
-    ghalf_tile<KQ, TI>(ia, N, M, tj + 1, b1, Ghalf, gb, Ks, gplane, ldq, sb);
+    if (tj + tstep >= RTm) break;
+    if (tj + 2 * tstep < RTm) ghalf_frag<KQ>(phi, N, M, tj + 2 * tstep, b0);
+    ghalf_tile<KQ, TI>(ia, N, M, tj + tstep, b1, Ghalf, gb, Ks, gplane, ldq, sb);
   }
 }
 

@@ -273,7 +273,8 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
                                                    const cplx* __restrict__ phia,
                                                    const cplx* __restrict__ phib,
                                                    cplx* __restrict__ Gha, cplx* __restrict__ Ghb,
-                                                   cplx* __restrict__ ovlp, GhalfBf16 gbf) {
+                                                   cplx* __restrict__ ovlp, GhalfBf16 gbf,
+                                                   cplx* __restrict__ inv_out) {
   extern __shared__ __align__(16) char smem[];
   __shared__ cplx ov[2];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -295,6 +296,10 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
     for (int j = 0; j < NB; ++j) a[j] = (l < N && j < N) ? phi[(long)j * N + l] : mkc(0.0, 0.0);
     int mystep = 0;
     gj::wave_gj<NB>(a, N, bro, order, mystep, &ov[wv]);
+    if (!GH && inv_out)
+      gj::gj_store_inverse<NB>(a, N, mystep, order,
+                               inv_out + w * ((long)Na * Na + (long)Nb * Nb) + (wv ? (long)Na * Na : 0),
+                               N);
     if (GH && (mode & 1)) {
       gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
       const bool gh = true;
@@ -309,7 +314,42 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
   if (threadIdx.x == 0) ovlp[w] = cmul(ov[0], ov[1]);
 }
 
+// Ghalf = S^-1 phi^T for identity trials from S^-1 in global memory (row
+// major, N x N per (walker, spin), spin b after spin a; written by
+// k_greens_gj).  GHW wavefronts per (walker, spin), wavefront v taking the
+// 16-orbital column tiles v, v + GHW, ...; each holds the S^-1 fragments in
+// registers, and two wavefronts share a SIMD, so that one's phi loads and
+// epilogue (split, LDS staging, stores) run under the other's FP64 MFMAs.
+constexpr int GHW = 4;
+
+template <int KQ>
+__global__ __launch_bounds__(64 * GHW, 2) void k_ghalf_inv(int M, int Na, int Nb,
+                                                           const cplx* __restrict__ inv,
+                                                           const cplx* __restrict__ phia,
+                                                           const cplx* __restrict__ phib,
+                                                           cplx* __restrict__ Gha,
+                                                           cplx* __restrict__ Ghb, GhalfBf16 gbf) {
+  __shared__ __align__(16) bf16_t sbuf[GHW][1024];  // split-plane staging, 2 KB per wavefront
+  const int wv = threadIdx.x >> 6;
+  const long w = blockIdx.x, W = gridDim.x;
+  const int spin = blockIdx.y;
+  const int N = spin ? Nb : Na;
+  if (N == 0) return;  // whole workgroup; no barriers below
+  const cplx* Inv = inv + w * ((long)Na * Na + (long)Nb * Nb) + (spin ? (long)Na * Na : 0);
+  const cplx* phi = spin ? phib + w * M * Nb : phia + w * M * Na;
+  cplx* G = spin ? Ghb : Gha;
+  bf16_t* gb = spin ? gbf.b : gbf.a;
+  const long Ks = spin ? gbf.Kb : gbf.Ka;
+  gj::ghalf_mfma_wave<KQ, false>(Inv, N, N, M, phi, G ? G + w * N * M : nullptr,
+                          gb ? gb + 2 * w * Ks : nullptr, Ks, 2 * W * Ks, gbf.ld, sbuf[wv], wv,
+                          GHW);
+}
+
 }  // namespace
+
+size_t greens_inv_bytes(int W, int Na, int Nb) {
+  return (size_t)W * ((size_t)Na * Na + (size_t)Nb * Nb) * sizeof(cplx);
+}
 
 static int greens_qch(int Na, int Nb) { return (Na > 48 || Nb > 48) ? 8 : 16; }
 
@@ -321,7 +361,7 @@ size_t greens_smem_bytes(int Na, int Nb) {
 
 int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,
                   const double* phib, const double* tu, const double* td, double* Gha,
-                  double* Ghb, double* ovlp, const GhalfBf16* gbf) {
+                  double* Ghb, double* ovlp, const GhalfBf16* gbf, double* inv) {
   if (Na > 64 || Nb > 64) return fail(AFQ_EUNSUPPORTED, "nocc > 64 is not supported (got %d, %d)", Na, Nb);
   if (W <= 0) return AFQ_OK;
   const size_t sm = greens_smem_bytes(Na, Nb);
@@ -338,18 +378,30 @@ int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const d
   const bool eye_both = ((mode >> 1) & 1 || Na == 0) && ((mode >> 2) & 1 || Nb == 0);
   const int Nmax = Na > Nb ? Na : Nb;
   const bool planes = g.a || g.b;
-  const size_t wl = (gj_wave_lds(mode, Nmax, planes) + 15) / 16 * 16;
+  // with an S^-1 buffer, Ghalf is a second kernel (k_ghalf_inv) after the
+  // elimination, which then runs in its overlap-only form
+  const bool two = (mode & 1) && inv != nullptr;
+  const int gmode = two ? (mode & ~1) : mode;
+  const size_t wl = (gj_wave_lds(gmode, Nmax, planes) + 15) / 16 * 16;
   if (eye_both && Nmax <= GJ_NMAX && 2 * wl + 64 <= max_lds_bytes()) {
     const size_t sm2 = 2 * wl;
-#define AFQ_GJ(NB)                                                                          \
-  if (mode & 1)                                                                               \
-    hipLaunchKernelGGL((k_greens_gj<NB, true>), dim3(W), dim3(128), sm2, s, mode, M, Na, Nb,  \
-                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb, \
-                       (cplx*)ovlp, g);                                                       \
-  else                                                                                        \
-    hipLaunchKernelGGL((k_greens_gj<NB, false>), dim3(W), dim3(128), sm2, s, mode, M, Na, Nb, \
-                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb, \
-                       (cplx*)ovlp, g)
+#define AFQ_GJ(NB)                                                                            \
+  do {                                                                                          \
+  if (gmode & 1)                                                                                \
+    hipLaunchKernelGGL((k_greens_gj<NB, true>), dim3(W), dim3(128), sm2, s, gmode, M, Na, Nb,   \
+                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb,   \
+                       (cplx*)ovlp, g, nullptr);                                                \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_greens_gj<NB, false>), dim3(W), dim3(128), sm2, s, gmode, M, Na, Nb,  \
+                       (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb,   \
+                       (cplx*)ovlp, g, two ? (cplx*)inv : nullptr);                             \
+  if (two) {                                                                                    \
+    AFQ_CHECK_LAUNCH();                                                                         \
+    hipLaunchKernelGGL((k_ghalf_inv<(NB + 3) / 4>), dim3(W, 2), dim3(64 * GHW), 0, s, M, Na, Nb, \
+                       (const cplx*)inv, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha,      \
+                       (cplx*)Ghb, g);                                                          \
+  }                                                                                             \
+  } while (0)
     if (Nmax <= 8) AFQ_GJ(8);
     else if (Nmax <= 16) AFQ_GJ(16);
     else if (Nmax <= 24) AFQ_GJ(24);

@@ -19,7 +19,7 @@ struct LowpWork {
   void* buf = nullptr;
   size_t bytes = 0;
   // offsets
-  size_t gh_a, gh_b, oo, part, X, cfb, cmf, xi, vhs, t0;
+  size_t gh_a, gh_b, oo, part, X, cfb, cmf, xi, vhs, t0, inv;
 };
 
 constexpr int FB_SPLITS_MAX = 4;
@@ -41,6 +41,7 @@ size_t lay(const Bf3System& b, int W, LowpWork* L) {
   L->xi = take(8L * W * b.G);
   L->vhs = take(bf3_vhs_bytes(b, W));
   L->t0 = take(bf3_prop_scratch_bytes(b, W));
+  L->inv = take(greens_inv_bytes(W, b.Na, b.Nb));
   return o;
 }
 
@@ -142,7 +143,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   GhalfBf16 gbf{Na ? gha : nullptr, Nb ? ghb : nullptr, b.Ka, b.Kb, b.ldq};
   phase_begin(sys, PH_GREENS);
   rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
-                     nullptr, nullptr, oo, &gbf);
+                     nullptr, nullptr, oo, &gbf, (double*)(base + L->inv));
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   // 3: force bias (split-K, FP32 partials), shift in FP64

@@ -91,7 +91,7 @@ struct Layout {
   size_t total;
 };
 
-enum Slot { GHA, GHB, OVLP_OLD, VBIAS, XS, CFB, CMF, XI, VHS, A1, A2, B1, B2, T0, T1, NSLOT };
+enum Slot { GHA, GHB, OVLP_OLD, VBIAS, XS, CFB, CMF, XI, VHS, A1, A2, B1, B2, T0, T1, INV, NSLOT };
 
 Layout layout(const afq_system* s, int W) {
   const long M = s->M, Na = s->Na, Nb = s->Nb, G = s->G;
@@ -112,6 +112,7 @@ Layout layout(const afq_system* s, int W) {
   sz[B2] = 16L * W * M * Nb;
   sz[T0] = 16L * W * M * N;
   sz[T1] = 16L * W * M * N;
+  sz[INV] = greens_inv_bytes(W, (int)Na, (int)Nb);
   Layout L;
   size_t o = 0;
   for (int i = 0; i < NSLOT; ++i) {
@@ -250,7 +251,7 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   // 1
   phase_begin(sys, PH_GREENS);
   rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
-                     gha, ghb, oo);
+                     gha, ghb, oo, nullptr, work_slot(sys, W, INV));
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   (void)a1; (void)a2; (void)b1; (void)b2; (void)t0; (void)t1;
@@ -310,7 +311,7 @@ int energy_prologue(afq_system* sys, int W, const double* phia, const double* ph
   es->part = work_slot(sys, W, VHS);
   es->part_bytes = (size_t)W * M * M * 16;
   rc = greens_launch(s, 1 | sys->eye_mask, W, M, Na, Nb, phia, phib, sys->trial_up, sys->trial_dn,
-                     es->gha, es->ghb, oo, gbf);
+                     es->gha, es->ghb, oo, gbf, work_slot(sys, W, INV));
   if (rc) return rc;
   // hT_s = h1e @ conj(trial_s)  (M x N_s)
   double* hTa = hT;

@@ -76,7 +76,8 @@ def test_system_energy_random_walkers(afq):
         np.testing.assert_allclose(_n(x), y, rtol=1e-12, atol=1e-10)
 
 
-@pytest.mark.parametrize("M,Na,Nb,G,W,nsteps", [(11, 2, 2, 50, 64, 30), (20, 4, 3, 70, 32, 12)])
+@pytest.mark.parametrize("M,Na,Nb,G,W,nsteps", [(11, 2, 2, 50, 64, 30), (20, 4, 3, 70, 32, 12),
+                                                 (48, 37, 19, 60, 8, 6)])
 def test_trajectory_matches_oracle(afq, M, Na, Nb, G, W, nsteps):
     s, od, gd = _driver_pair(M, Na, Nb, G)
     seed = 42
