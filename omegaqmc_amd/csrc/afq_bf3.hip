@@ -19,6 +19,8 @@
 // i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8); D[i][j] in lane
 // j + 16 (i / 4), register i % 4.  With GEMM rows ordered (walker, re/im) a
 // lane's four output rows are (w, re), (w, im), (w+1, re), (w+1, im).
+#include <type_traits>
+
 #include "afq_bf3.h"
 #include "afq_kernels.h"
 
@@ -306,6 +308,13 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 // workgroup), so the registers hold one accumulator set and two waves fit
 // per SIMD.
 constexpr int PW = 8, PT = 64 * PW, PC = 48;
+// Horner stage n (y <- T0 + V y / n) reaches the result through n - 1 more
+// products by V / k, so a relative error e in its product V y moves phi by
+// about e |V|^n / n!.  Stages n >= TAYLOR_BF16_FROM use plain bf16 products
+// (e ~ 2^-8, hi planes only): at |V|_2 = 0.5 (M=300, Nchol=1800, dt=0.005)
+// that is 8e-7 (n = 5) and 7e-8 (n = 6) of phi, below the ~4e-6 of the split
+// products of the first stage.
+constexpr int TAYLOR_BF16_FROM = 5;
 
 template <int MT>
 __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, int Na, int Nb,
@@ -443,6 +452,30 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 #pragma unroll
     for (int p = 0; p < 4; ++p) v[p] = gld(blk + p * 256);
   };
+  // plain bf16 (hi planes only) for the high-order Horner stages
+  auto ldv_hi = [&](uint4* v, int u, int ks) {
+    const int Q = min(2 * ks + (lq >> 1), RT - 1), j0 = 8 * (lq & 1);
+    const int P = min(wv + PW * u, RT - 1);
+    const bf16_t* blk = Vw + ((long)P * RT + Q) * 1024 + li * 16 + j0;
+    v[0] = gld(blk);
+    v[2] = gld(blk + 512);
+  };
+  auto bfrag_hi = [&](int x, int ks, uint4* b) {
+    b[0] = *(const uint4*)(T(0, 16 * x + li) + 32 * ks + 8 * lq);
+    b[2] = *(const uint4*)(T(2, 16 * x + li) + 32 * ks + 8 * lq);
+  };
+  auto vstep_hi = [&](const uint4* vc, uint4 (*b)[4], int u) {
+    if (owns(u)) {
+      const uint4 ni = neg8(vc[2]);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        are[u][x] = mfma_bf(vc[0], b[x][0], are[u][x]);
+        are[u][x] = mfma_bf(ni, b[x][2], are[u][x]);
+        aim[u][x] = mfma_bf(vc[0], b[x][2], aim[u][x]);
+        aim[u][x] = mfma_bf(vc[2], b[x][0], aim[u][x]);
+      }
+    }
+  };
   auto vstep = [&](const uint4* vc, uint4 (*b)[4], int u) {
     if (owns(u)) {
       const uint4 ni = neg8(vc[2]), nl = neg8(vc[3]);
@@ -465,12 +498,18 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   // k-step); the k-steps go in threes so that, with the row-tile loop
   // unrolled, every buffer choice is static and no loaded register is copied
   // (a copy would wait for the load).
-  auto term = [&]() {
+  // LOW: plain bf16 products (the Horner stages n >= TAYLOR_BF16_FROM)
+  auto term = [&](auto lowc) {
+    constexpr bool LOW = decltype(lowc)::value;
     zero();
     const int nks = Kp / 32;
     uint4 vb[3][4];
-    ldv(vb[0], 0, 0);
-    ldv(vb[1], 1 % MT, (1 / MT) < nks ? 1 / MT : nks - 1);
+    auto ld = [&](uint4* v, int u, int ks) {
+      if constexpr (LOW) ldv_hi(v, u, ks);
+      else ldv(v, u, ks);
+    };
+    ld(vb[0], 0, 0);
+    ld(vb[1], 1 % MT, (1 / MT) < nks ? 1 / MT : nks - 1);
     for (int ks = 0; ks < nks; ks += 3) {
 #pragma unroll
       for (int h = 0; h < 3; ++h) {
@@ -479,15 +518,21 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
         uint4 b[3][4];
         if (live) {
 #pragma unroll
-          for (int x = 0; x < 3; ++x) bfrag(x, kk, b[x]);
+          for (int x = 0; x < 3; ++x) {
+            if constexpr (LOW) bfrag_hi(x, kk, b[x]);
+            else bfrag(x, kk, b[x]);
+          }
         }
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int sidx = h * MT + u;
           const int t2 = sidx + 2;
           const int kn = ks + t2 / MT;
-          ldv(vb[t2 % 3], t2 % MT, kn < nks ? kn : nks - 1);
-          if (live) vstep(vb[sidx % 3], b, u);
+          ld(vb[t2 % 3], t2 % MT, kn < nks ? kn : nks - 1);
+          if (live) {
+            if constexpr (LOW) vstep_hi(vb[sidx % 3], b, u);
+            else vstep(vb[sidx % 3], b, u);
+          }
         }
       }
     }
@@ -537,7 +582,8 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   }
   __syncthreads();
   for (int n = nmax; n >= 1; --n) {
-    term();
+    if (n >= TAYLOR_BF16_FROM) term(std::true_type{});
+    else term(std::false_type{});
     const float rn = 1.0f / (float)n;
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
