@@ -57,9 +57,24 @@ def phase_flops(M, Na, Nb, G, W, T=6):
     }
 
 
+# Horner stages of the BF16x3 propagator that run plain BF16 products (the
+# library's TAYLOR_BF16_FROM = 5 with 6 terms: stages 6 and 5)
+TAYLOR_BF16_STAGES = 2
+
+
+def bf16x3_taylor_peak(T=6):
+    """MFMA peak of the mixed propagator: the two half steps and stages
+    1..T-2 take three BF16 products per real product, the last two stages
+    one, so the algorithmic flops (8 T + 8) M^2 N cost (3 (8 (T - s) + 8) +
+    8 s) M^2 N at the BF16 rate."""
+    s = TAYLOR_BF16_STAGES
+    return PEAK["bf16"] * (8.0 * T + 8) / (3.0 * (8 * (T - s) + 8) + 8.0 * s)
+
+
 KERNEL_OF_PHASE = {
     ("taylor", "fp64"): "k_prop_f64 (expH1 + 6-term Taylor + expH1, FP64 MFMA)",
-    ("taylor", "bf16x3"): "k_prop_bf3 (expH1 + 6-term Taylor + expH1, BF16x3 MFMA)",
+    ("taylor", "bf16x3"): "k_prop_bf3 (expH1 + 6-term Taylor + expH1, BF16x3 MFMA, "
+                          "Horner stages 5-6 plain BF16)",
     ("vhs", "fp64"): "k_vhs_f64 (packed VHS build, FP64 MFMA)",
     ("vhs", "bf16x3"): "k_bf3_gemm<0> (VHS build, BF16x3 MFMA)",
     ("force_bias", "fp64"): "k_fb_f64 (split-K force bias, FP64 MFMA)",
@@ -364,6 +379,8 @@ def main():
         ach = flops / (per_launch_ms * 1e-3) / 1e12
         peak = PEAK["fp64"] if best_prec == "fp64" else (PEAK["fp32"] if best_prec == "fp32"
                                                          else PEAK["bf16"] / 3)
+        if dom == "taylor" and best_prec == "bf16x3":
+            peak = bf16x3_taylor_peak()
         sym = KERNEL_SYMBOL.get((dom, best_prec))
         traffic = measured_traffic(sym) if (sym and M == 300 and W == 1024) else None
         roof = {"bound": "mfma", "kernel": KERNEL_OF_PHASE.get((dom, best_prec), dom),
