@@ -26,6 +26,15 @@ __device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsig
   return a > b ? a : b;
 }
 
+// value of lane `lane` (uniform) of a complex register, as scalars
+__device__ __forceinline__ double rl64(double v, int lane) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ cplx rlc(cplx v, int lane) { return mkc(rl64(v.x, lane), rl64(v.y, lane)); }
+
 // Orders the LDS accesses of one wavefront (lanes exchange data through LDS).
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -51,10 +60,13 @@ constexpr int GJ_LD = GJ_NMAX + 1;
 // k (the pivot row's 1/d, the others' -f/d) goes to the last register.  Every
 // register index is static, so the step is straight-line VALU code.  With
 // g = f/d for the other rows and g = 1 - 1/d for the pivot row, the single
-// update a_j <- a_j - g * (unscaled pivot row)_j covers both.
+// update a_j <- a_j - g * (unscaled pivot row)_j covers both.  The pivot row
+// reaches the other lanes by readlane (scalar operands of the update), not
+// through LDS: one-lane LDS writes and broadcast reads cost a full LDS
+// instruction each, and the eight wavefronts of a CU would queue on them.
 // ----------------------------------------------------------------------------
 template <int NB>
-__device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* order,
+__device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, int* order,
                                         int& mystep, cplx* ov) {
   const int l = threadIdx.x & 63;
   bool active = l < N;
@@ -86,9 +98,6 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
     const int p = 63 - (int)(klo & 63u);
     const bool me = l == p;
     if (me) {
-#pragma unroll
-      for (int j = 1; j < NB; ++j) bro[j] = a[j];
-      bro[0] = f;
       active = false;
       mystep = k;
     }
@@ -96,8 +105,7 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
     // sign of the pivot permutation from its Lehmer code
     const unsigned long long am = __ballot(active || me);
     parity ^= __popcll(am & ((1ull << p) - 1ull)) & 1;
-    wave_sync();
-    const cplx d = bro[0];
+    const cplx d = rlc(f, p);
     sing = sing || (d.x == 0.0 && d.y == 0.0);
     pr = cmul(pr, d);
     {
@@ -112,12 +120,12 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, cplx* bro, int* or
 #pragma unroll
     for (int j = 0; j + 1 < NB; ++j) {
       cplx x = a[j + 1];
-      cfms(x, g, bro[j + 1]);
+      cfms(x, g, rlc(a[j + 1], p));
       a[j] = x;
     }
     a[NB - 1] = col;
-    wave_sync();
   }
+  wave_sync();  // order[] for gj_store_inverse
   if (l == 0) {
     const cplx r = sing ? mkc(0.0, 0.0) : mkc(ldexp(pr.x, e2), ldexp(pr.y, e2));
     *ov = parity ? mkc(-r.x, -r.y) : r;

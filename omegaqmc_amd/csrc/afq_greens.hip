@@ -258,11 +258,10 @@ __global__ __launch_bounds__(NT) void k_greens(int mode, int M, int Na, int Nb, 
 using gj::GJ_LD;
 using gj::GJ_NMAX;
 
-// LDS of one wavefront of k_greens_gj: the pivot row and order, plus for
-// Ghalf S^-1 (N x GJ_LD complex) and the 2 KB staging of the split planes.
+// LDS of one wavefront of k_greens_gj: the pivot order, plus for Ghalf S^-1
+// (N x GJ_LD complex) and the 2 KB staging of the split planes.
 size_t gj_wave_lds(int mode, int N, bool planes) {
-  return 16UL * GJ_NMAX + 256 + ((mode & 1) ? 16UL * N * GJ_LD : 0) +
-         (((mode & 1) && planes) ? 2048 : 0);
+  return 256 + ((mode & 1) ? 16UL * N * GJ_LD : 0) + (((mode & 1) && planes) ? 2048 : 0);
 }
 
 // One walker per workgroup of two wavefronts, wave s = spin s; identity trials.
@@ -282,8 +281,7 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
   const long W = gridDim.x;
   const int N = wv ? Nb : Na;
   char* base = smem + (long)wv * wlds;
-  cplx* bro = (cplx*)base;
-  int* order = (int*)(bro + GJ_NMAX);
+  int* order = (int*)base;
   cplx* Inv = (cplx*)(order + 64);
   bf16_t* sb = (bf16_t*)(Inv + ((mode & 1) ? N * GJ_LD : 0));
   const cplx* phi = (wv ? phib + w * M * Nb : phia + w * M * Na);
@@ -295,7 +293,7 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
 #pragma unroll
     for (int j = 0; j < NB; ++j) a[j] = (l < N && j < N) ? phi[(long)j * N + l] : mkc(0.0, 0.0);
     int mystep = 0;
-    gj::wave_gj<NB>(a, N, bro, order, mystep, &ov[wv]);
+    gj::wave_gj<NB>(a, N, order, mystep, &ov[wv]);
     if (!GH && inv_out)
       gj::gj_store_inverse<NB>(a, N, mystep, order,
                                inv_out + w * ((long)Na * Na + (long)Nb * Nb) + (wv ? (long)Na * Na : 0),

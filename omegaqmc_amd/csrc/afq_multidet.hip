@@ -44,10 +44,11 @@ using gj::GJ_NMAX;
 constexpr double MASK_RTOL = 1e-14;   // greens_th4.py:24
 constexpr double MASK_ATOL = 1e-300;  // greens_th4.py:25
 
-// LDS of one wavefront of k_md_gj: the pivot row (NB complex), the pivot
-// order (64 ints) and, in mode 1, S^-1 (N x GJ_LD complex).
+// LDS of one wavefront of k_md_gj: the pivot order (64 ints) and, in mode
+// 1, S^-1 (N x GJ_LD complex).
 size_t md_wave_lds(int mode, int NB, int N) {
-  return 16UL * NB + 256 + (mode == 1 ? 16UL * N * GJ_LD : 0);
+  (void)NB;
+  return 256 + (mode == 1 ? 16UL * N * GJ_LD : 0);
 }
 
 // Grid W * nd (det fastest), two wavefronts: wave s = spin s.  mode 0: the
@@ -72,8 +73,7 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
     return;  // no workgroup barrier in this kernel
   }
   char* base = smem + (long)wv * wlds;
-  cplx* bro = (cplx*)base;
-  int* order = (int*)(bro + NB);
+  int* order = (int*)base;
   cplx* Inv = (cplx*)(order + 64);
   const cplx* phi = wv ? phib + w * M * Nb : phia + w * M * Na;
   const int* occ = (wv ? occb : occa) + (long)I * N;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
   for (int j = 0; j < NB; ++j)
     a[j] = (l < N && j < N) ? phi[(long)occ[j] * N + l] : mkc(0.0, 0.0);
   int mystep = 0;
-  gj::wave_gj<NB>(a, N, bro, order, mystep, od);
+  gj::wave_gj<NB>(a, N, order, mystep, od);
   if (mode == 1) {
     gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
     cplx* G = (wv ? ghb : gha) + ((long)I * W + w) * N * M;
