@@ -259,10 +259,14 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
         bf16_t pl[4][8];
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-          const double re = -(double)a.sqdt * (double)acc[u][x][2 * h + 1];
-          const double im = (double)a.sqdt * (double)acc[u][x][2 * h];
-          split2(re, pl[0][x], pl[1][x]);
-          split2(im, pl[2][x], pl[3][x]);
+          // FP32 product (the nearest float to the exact one) and its exact
+          // FP32 split
+          const float re = -a.sqdt * acc[u][x][2 * h + 1];
+          const float im = a.sqdt * acc[u][x][2 * h];
+          pl[0][x] = bf_bits(re);
+          pl[1][x] = bf_bits(re - bf_val(pl[0][x]));
+          pl[2][x] = bf_bits(im);
+          pl[3][x] = bf_bits(im - bf_val(pl[2][x]));
           const int i = 8 * wc + x;
 #pragma unroll
           for (int p = 0; p < 4; ++p) base[offd + p * 256 + i * 16 + li] = pl[p][x];
@@ -613,8 +617,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       for (int r = 0; r < 4; ++r) {
         if (r0 + r >= M) continue;
         double* z = phi + 2 * (((long)w * M + r0 + r) * N + c);
-        z[0] = (double)are[u][x][r];
-        z[1] = (double)aim[u][x][r];
+        *(double2*)z = make_double2((double)are[u][x][r], (double)aim[u][x][r]);
       }
     }
   }

@@ -114,7 +114,16 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, int* order,
       pr = mkc(ldexp(pr.x, -e), ldexp(pr.y, -e));
       e2 += e;
     }
-    const cplx id = cdiv(mkc(1.0, 0.0), d);
+    // 1/d = conj(d) / |d|^2 with d first scaled by a power of two (exact), so
+    // that |d|^2 neither over- nor underflows: one division
+    cplx id;
+    {
+      int ed;
+      (void)frexp(fmax(fabs(d.x), fabs(d.y)), &ed);
+      const double sx = ldexp(d.x, -ed), sy = ldexp(d.y, -ed);
+      const double q = 1.0 / fma(sx, sx, sy * sy);
+      id = mkc(ldexp(sx * q, -ed), ldexp(-sy * q, -ed));
+    }
     const cplx g = me ? mkc(1.0 - id.x, -id.y) : cmul(f, id);
     const cplx col = me ? id : mkc(-g.x, -g.y);
 #pragma unroll
