@@ -259,6 +259,17 @@ int afq_system_create(afq_ctx* ctx, afq_system** out, int nbasis, int nup, int n
                       double dt, double fbbound, int exp_nmax, int precision);
 int afq_system_destroy(afq_system* sys);
 
+/* Optional ERI sign of each Cholesky vector, (pq|rs) = sum_g sign_g L_g,pq
+ * L_g,rs: the symmetrised plane-wave vectors of the electron gas
+ * (afqmc_pw_heg.py:161-254, chol_sign).  The step then uses the HS phase i
+ * for sign +1 and 1 for sign -1 (afqmc_pw_heg.py:895-922, chol_phase) and the
+ * local energy weights Coulomb and exchange by the sign
+ * (afqmc_pw_heg.py:414-457).  chol_sign: device (G,) of +1 / -1, kept alive
+ * and unchanged by the caller; NULL restores all +1.  Negative entries need
+ * an FP64 single-determinant system (AFQ_EUNSUPPORTED otherwise: the split
+ * BF16 VHS assumes symmetric L_g). */
+int afq_system_set_chol_sign(afq_system* sys, const double* chol_sign);
+
 /* afqmc_gto.py:630-725 propagate_walkers(...) for one step, in place on
  * phia/phib/weights/e_hybrid; overlap_new (W,) complex is written.  The
  * reference ignores its `overlap` argument, so there is none here.  xi (W,G)

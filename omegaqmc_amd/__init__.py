@@ -15,6 +15,8 @@ from .afqmc_gto import (  # noqa: F401
     MultiDetSystem, greens_function_multidet, greens_function_multidet_overlap_only,
     local_energy_multidet, multidet_charge_density, occupations_from_trials,
     propagate_walkers_multidet)
+from ._lib import AFQError  # noqa: F401
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
+from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401
 
 __version__ = "0.1.0"

@@ -60,6 +60,7 @@ SIGNATURES = {
     "afq_system_create": [_P, ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P,
                           _D, _D, _I, _I],
     "afq_system_destroy": [_P],
+    "afq_system_set_chol_sign": [_P, _P],
     "afq_greens_function_multidet": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P],
     "afq_greens_function_multidet_overlap_only": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P,
                                                   _I, _P],

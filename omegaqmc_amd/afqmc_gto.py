@@ -410,10 +410,14 @@ class System:
     """An ``afq_system``: integrals and trial prepared for one precision."""
 
     def __init__(self, propagator, chol, rchol_a, rchol_b, trial_up, trial_dn, fbbound=None,
-                 exp_nmax=6, precision="fp64"):
+                 exp_nmax=6, precision="fp64", chol_sign=None):
         self.chol = _r(chol)
         self.rchol_a = _r(rchol_a)
         self.rchol_b = _r(rchol_b)
+        if self.chol.shape[0] == 0:
+            # no Cholesky vectors (no two-body term): the ABI still takes
+            # non-NULL arrays, which are then never read
+            self._empty = torch.zeros(1, dtype=F64, device=self.chol.device)
         E = propagator['expH1']
         if isinstance(E, torch.Tensor) and E.is_complex():
             if float(torch.max(torch.abs(E.imag))) != 0.0:
@@ -431,12 +435,20 @@ class System:
         self.M, self.G = M, G
         self.Na, self.Nb = self.trial_up.shape[1], self.trial_dn.shape[1]
         h = _lib._P()
+        nz = (lambda t: ptr(self._empty) if t.numel() == 0 else ptr(t)) if G == 0 else ptr
         check(_L().afq_system_create(_ctx(), _lib.ctypes.byref(h), M, self.Na, self.Nb, G,
-                                     ptr(self.chol), ptr(self.rchol_a), ptr(self.rchol_b),
-                                     ptr(self.expH1), ptr(self.mf_shift), ptr(self.trial_up),
+                                     nz(self.chol), nz(self.rchol_a), nz(self.rchol_b),
+                                     ptr(self.expH1), nz(self.mf_shift), ptr(self.trial_up),
                                      ptr(self.trial_dn), self.dt, self.fbbound, self.nmax,
                                      _lib.PRECISIONS[precision]), "afq_system_create")
         self.handle = h
+        self.chol_sign = None
+        if chol_sign is not None and G > 0:
+            # afqmc_pw_heg.py:161-254: ERI sign of each vector (HS phase i / 1)
+            self.chol_sign = _r(chol_sign).reshape(G)
+            torch.cuda.synchronize()
+            check(_L().afq_system_set_chol_sign(h, ptr(self.chol_sign)),
+                  "afq_system_set_chol_sign")
 
     def step(self, phia, phib, weights, e_hybrid, eshift, xi=None, key=None):
         """In-place step; returns overlap_new."""
@@ -663,7 +675,9 @@ def do_binning_analysis(a):
     sdev = a.std()
     x = a - xbar
     ac = np.correlate(x, x, mode='full')
-    ac = ac / ac[ac.shape[0] // 2]
+    with np.errstate(invalid='ignore', divide='ignore'):
+        # a constant series gives 0/0: no positive lag then, kappa = 1
+        ac = ac / ac[ac.shape[0] // 2]
     kappa = 1.0
     i = (ac.shape[0] >> 1) + 1
     while i < ac.shape[0] and ac[i] >= 0:

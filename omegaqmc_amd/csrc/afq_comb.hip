@@ -209,8 +209,9 @@ int afq_population_control_apply(afq_ctx* ctx, int W, int M, int Na, int Nb, con
                                  double* eh_new) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0, "bad shape");
-  AFQ_REQUIRE(idx && w_new && phia_new && phib_new, "NULL argument");
-  AFQ_REQUIRE(phia_new != phia && phib_new != phib, "outputs must not alias inputs");
+  AFQ_REQUIRE(idx && w_new && (Na == 0 || phia_new) && (Nb == 0 || phib_new), "NULL argument");
+  AFQ_REQUIRE((Na == 0 || phia_new != phia) && (Nb == 0 || phib_new != phib),
+              "outputs must not alias inputs");
   AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
   return afq::gather_launch(ctx->stream, W, M, Na, Nb, idx, phia, phib, eh, w_new, phia_new,
                             phib_new, eh_new);
@@ -223,7 +224,8 @@ int afq_population_control_comb(afq_ctx* ctx, int W, int M, int Na, int Nb, cons
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0, "bad shape");
   AFQ_REQUIRE(r >= 0.0 && r < 1.0, "comb offset r=%g outside [0,1)", r);
-  AFQ_REQUIRE(phia_new != phia && phib_new != phib, "comb outputs must not alias inputs");
+  AFQ_REQUIRE((Na == 0 || phia_new != phia) && (Nb == 0 || phib_new != phib),
+              "comb outputs must not alias inputs");
   AFQ_REQUIRE(idx != nullptr && w_new != nullptr, "idx and weights_new are required");
   AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
   double* cum = nullptr;

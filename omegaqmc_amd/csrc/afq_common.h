@@ -104,6 +104,10 @@ struct afq_system {
   const double* mf_shift = nullptr;
   const double* trial_up = nullptr;
   const double* trial_dn = nullptr;
+  // optional sign of each Cholesky vector in the ERI, (pq|rs) = sum_g
+  // sign_g L_g,pq L_g,rs (+1 symmetric, -1 antisymmetric vectors of the
+  // plane-wave electron gas; caller-owned, device, (G,)); NULL: all +1
+  const double* chol_sign = nullptr;
   double dt = 0.005, fbbound = 1.0;
   int nmax = 6;
   int precision = AFQ_FP64;

@@ -18,11 +18,13 @@ namespace {
 
 constexpr int NT = 256;
 
-// e[w] (+)= 1/2 sum_{g<gs} sum_ij T[(g,i)][(w,j)] T[(g,j)][(w,i)],
-// T row-major (gs*N) x (W*N).  One workgroup per walker.
+// e[w] (+)= 1/2 sum_{g<gs} sign_g sum_ij T[(g,i)][(w,j)] T[(g,j)][(w,i)],
+// T row-major (gs*N) x (W*N); sign (optional) the slab's ERI signs
+// (afqmc_pw_heg.py:414-457).  One workgroup per walker.
 __global__ __launch_bounds__(NT) void k_exch_contract(int W, int N, int gs,
                                                       const cplx* __restrict__ T,
-                                                      cplx* __restrict__ e, int accumulate) {
+                                                      cplx* __restrict__ e, int accumulate,
+                                                      const double* __restrict__ sign) {
   __shared__ double red[32];
   const long w = blockIdx.x;
   const long ld = (long)W * N;
@@ -33,6 +35,7 @@ __global__ __launch_bounds__(NT) void k_exch_contract(int W, int N, int gs,
     int i = r / N, j = r % N;
     cplx a = T[((long)g * N + i) * ld + w * N + j];
     cplx b = T[((long)g * N + j) * ld + w * N + i];
+    if (sign && sign[g] < 0.0) a = mkc(-a.x, -a.y);
     cfma(acc, a, b);
   }
   acc = block_sum_c(acc, red);
@@ -42,15 +45,17 @@ __global__ __launch_bounds__(NT) void k_exch_contract(int W, int N, int gs,
   }
 }
 
-// e_coul[w] = 1/2 sum_g X[w][g]^2, X (W,G) complex.
+// e_coul[w] = 1/2 sum_g sign_g X[w][g]^2, X (W,G) complex, sign optional.
 __global__ __launch_bounds__(NT) void k_coulomb(int G, const cplx* __restrict__ X,
-                                                cplx* __restrict__ ec) {
+                                                cplx* __restrict__ ec,
+                                                const double* __restrict__ sign) {
   __shared__ double red[32];
   const long w = blockIdx.x;
   cplx acc = mkc(0, 0);
   for (int g = threadIdx.x; g < G; g += NT) {
     cplx x = X[w * G + g];
-    cfma(acc, x, x);
+    if (sign && sign[g] < 0.0) cfms(acc, x, x);
+    else cfma(acc, x, x);
   }
   acc = block_sum_c(acc, red);
   if (threadIdx.x == 0) ec[w] = cscale(acc, 0.5);
@@ -115,7 +120,7 @@ __global__ void k_combine(int W, const cplx* __restrict__ e1b, const cplx* __res
 // Exchange energy of one spin, FP64.  partial: scratch of >= gchunk*N*W*N complex.
 int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
                   const double* rchol, bool rchol_cplx, int gchunk, double* partial,
-                  double* e_exch, bool accumulate) {
+                  double* e_exch, bool accumulate, const double* sign) {
   if (W <= 0) return AFQ_OK;
   bool acc = accumulate;
   if (N == 0) {
@@ -134,7 +139,7 @@ int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf
     int rc = gemm_fp64(s, ga);
     if (rc) return rc;
     hipLaunchKernelGGL(k_exch_contract, dim3(W), dim3(NT), 0, s, W, N, gs, (const cplx*)partial,
-                       (cplx*)e_exch, acc ? 1 : 0);
+                       (cplx*)e_exch, acc ? 1 : 0, sign ? sign + g0 : nullptr);
     AFQ_CHECK_LAUNCH();
     acc = true;
   }
@@ -146,6 +151,7 @@ int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf
 static int coulomb_x(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                      const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
                      double* X) {
+  if (G == 0) return AFQ_OK;  // no vectors: X is empty
   const int Ns[2] = {Na, Nb};
   const double* R[2] = {ra, rb};
   const double* Gh[2] = {Gha, Ghb};
@@ -198,7 +204,8 @@ static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, i
                     bool rc_cplx, double* e_coul, double* e_exch) {
   int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, sc.X);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)sc.X, (cplx*)e_coul);
+  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)sc.X, (cplx*)e_coul,
+                     nullptr);
   AFQ_CHECK_LAUNCH();
   rc = exchange_fp64(s, W, M, Na, G, Gha, ra, rc_cplx, sc.gchunk, sc.part, e_exch, false);
   if (rc) return rc;
@@ -207,10 +214,10 @@ static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, i
 
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                    const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
-                   double* X, double* e_coul) {
+                   double* X, double* e_coul, const double* sign) {
   int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, X);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)X, (cplx*)e_coul);
+  hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)X, (cplx*)e_coul, sign);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

@@ -22,7 +22,7 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
                  double* cmf, int nsplit, const float* vpart = nullptr, bf16_t* Xp = nullptr,
-                 int Gp = 0);
+                 int Gp = 0, const double* sign = nullptr);
 int build_vhs_fp64(hipStream_t s, int W, int M, int G, const double* xs, const double* chol,
                    double dt, double* VHS);
 int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, const double* phi,
@@ -54,7 +54,7 @@ int accumulate_launch(hipStream_t s, int W, const double* weights, const double*
 // energy (afq_energy.hip)
 int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
                   const double* rchol, bool rchol_cplx, int gchunk, double* partial,
-                  double* e_exch, bool accumulate);
+                  double* e_exch, bool accumulate, const double* sign = nullptr);
 int e1b_half_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* hTa,
                     const double* hTb, const double* Gha, const double* Ghb, double enuc,
                     double* e1b);
@@ -65,6 +65,6 @@ int e1b_fullg_launch(hipStream_t s, int W, int M, const double* h1e, const doubl
                      const double* Gb, double enuc, double* e1b);
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                    const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
-                   double* X, double* e_coul);
+                   double* X, double* e_coul, const double* sign = nullptr);
 
 }  // namespace afq

@@ -480,7 +480,7 @@ FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, 
 }
 
 int fb_mfma_f64(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
-  if (W <= 0 || sp.n == 0) return AFQ_OK;
+  if (W <= 0 || G <= 0 || sp.n == 0) return AFQ_OK;
   const int ntm = cdiv_int(2L * W, TM), ntn = cdiv_int(G, TN);
   hipLaunchKernelGGL(k_fb_f64, dim3(ntm * ntn, sp.n), dim3(NTH), 0, s, W, G, sp, (cplx*)part, ntm,
                      ntn);

@@ -17,6 +17,11 @@ constexpr int NT = 256;
 // vbias comes as nsplit FP64 partials (vbias) or FP32 partials (vpart), summed
 // in FP64 in split order.  Optional outputs: xbar, xshifted (FP64) and the
 // split-bf16 planes of xshifted, Xp[2][2W][Gp] with rows (w, re/im).
+// sign (optional, (G,)): the electron-gas form (afqmc_pw_heg.py:895-922), HS
+// phase i for sign +1 and 1 for sign -1: xbar = -sqrt(dt) (phase vbias - mf)
+// and VHS = sqrt(dt) sum_g xs_g phase_g L_g, written here as the coefficient
+// xs_g phase_g / i of the i sqrt(dt) sum_g (.) L_g the VHS kernels form (the
+// xs output then holds that coefficient; cmf and cfb use xs itself).
 __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
                                               const cplx* __restrict__ vbias,  // nsplit x (W,G)
                                               const float2* __restrict__ vpart,
@@ -25,7 +30,8 @@ __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
                                               double sqdt, double fb, cplx* __restrict__ xbar_out,
                                               cplx* __restrict__ xs_out, cplx* __restrict__ cfb,
                                               cplx* __restrict__ cmf, bf16_t* __restrict__ Xp,
-                                              int Gp, long Xplane) {
+                                              int Gp, long Xplane,
+                                              const double* __restrict__ sign) {
   __shared__ double red[32];
   const long w = blockIdx.x;
   cplx s_mf = mkc(0, 0), s_xixb = mkc(0, 0), s_xb2 = mkc(0, 0);
@@ -42,19 +48,22 @@ __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
       for (int sp = 1; sp < nsplit; ++sp) v = cadd(v, vbias[sp * sstride + w * G + g]);
     }
     const cplx m = mf[g];
-    cplx xb = mkc(sqdt * (v.y + m.x), -sqdt * (v.x - m.y));
+    const bool btype = sign && sign[g] < 0.0;
+    cplx xb = btype ? mkc(-sqdt * (v.x - m.x), -sqdt * (v.y - m.y))
+                    : mkc(sqdt * (v.y + m.x), -sqdt * (v.x - m.y));
     const double a = hypot(xb.x, xb.y);
     if (a > fb) xb = mkc((xb.x * fb) / a, (xb.y * fb) / a);
     const double x = xi[w * G + g];
     const cplx xs = mkc(x - xb.x, -xb.y);
+    const cplx xv = btype ? mkc(xs.y, -xs.x) : xs;  // coefficient of i sqrt(dt) L_g
     if (xbar_out) xbar_out[w * G + g] = xb;
-    if (xs_out) xs_out[w * G + g] = xs;
+    if (xs_out) xs_out[w * G + g] = xv;
     if (Xp) {
       bf16_t h, l;
-      split2(xs.x, h, l);
+      split2(xv.x, h, l);
       Xp[2 * w * Gp + g] = h;
       Xp[Xplane + 2 * w * Gp + g] = l;
-      split2(xs.y, h, l);
+      split2(xv.y, h, l);
       Xp[(2 * w + 1) * Gp + g] = h;
       Xp[Xplane + (2 * w + 1) * Gp + g] = l;
     }
@@ -139,11 +148,12 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
 
 int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double* mf,
                  const double* xi, double dt, double fb, double* xbar, double* xs, double* cfb,
-                 double* cmf, int nsplit, const float* vpart, bf16_t* Xp, int Gp) {
+                 double* cmf, int nsplit, const float* vpart, bf16_t* Xp, int Gp,
+                 const double* sign) {
   if (W <= 0) return AFQ_OK;
   hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, nsplit, (long)W * G,
                      (const cplx*)vbiasT, (const float2*)vpart, (const cplx*)mf, xi, sqrt(dt), fb,
-                     (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf, Xp, Gp, 2L * W * Gp);
+                     (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf, Xp, Gp, 2L * W * Gp, sign);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

@@ -264,7 +264,7 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   phase_end(sys, PH_FORCEBIAS);
   phase_begin(sys, PH_SHIFT);
   if ((rc = shift_launch(s, W, G, vb, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr, xs, cfb,
-                         cmf, sp.n)))
+                         cmf, sp.n, nullptr, nullptr, 0, sys->chol_sign)))
     return rc;
   phase_end(sys, PH_SHIFT);
   // 4
@@ -339,15 +339,17 @@ int system_energy_fp64(afq_system* sys, int W, const double* phia, const double*
   int rc = energy_prologue(sys, W, phia, phib, h1e, enuc, nullptr, e_1b, &es);
   if (rc) return rc;
   if ((rc = coulomb_launch(s, W, M, Na, Nb, G, sys->rchol_a, sys->rchol_b, false, es.gha, es.ghb,
-                           es.X, es.ec)))
+                           es.X, es.ec, sys->chol_sign)))
     return rc;
   // T slab in the VHS region (W*M*M complex): gchunk with gchunk*N*W*N <= W*M*M
   const int N = std::max(Na, Nb);
   int gc = N > 0 ? std::max(1, (M * M) / (N * N)) : G;
   gc = std::min(gc, std::max(G, 1));
-  if ((rc = exchange_fp64(s, W, M, Na, G, es.gha, sys->rchol_a, false, gc, es.part, es.ex, false)))
+  if ((rc = exchange_fp64(s, W, M, Na, G, es.gha, sys->rchol_a, false, gc, es.part, es.ex, false,
+                          sys->chol_sign)))
     return rc;
-  if ((rc = exchange_fp64(s, W, M, Nb, G, es.ghb, sys->rchol_b, false, gc, es.part, es.ex, true)))
+  if ((rc = exchange_fp64(s, W, M, Nb, G, es.ghb, sys->rchol_b, false, gc, es.part, es.ex, true,
+                          sys->chol_sign)))
     return rc;
   return combine_launch(s, W, e_1b, es.ec, es.ex, e_tot, e_2b);
 }
@@ -366,7 +368,7 @@ int afq_system_create(afq_ctx* ctx, afq_system** out, int M, int Na, int Nb, int
   AFQ_REQUIRE(dt > 0 && nmax >= 0, "bad dt / nmax");
   AFQ_REQUIRE(precision == AFQ_FP64 || precision == AFQ_FP32 || precision == AFQ_BF16X3,
               "unknown precision %d", precision);
-  AFQ_REQUIRE(chol && expH1 && mf && tu && td && (Na == 0 || ra) && (Nb == 0 || rb),
+  AFQ_REQUIRE(chol && expH1 && mf && (Na == 0 || (tu && ra)) && (Nb == 0 || (td && rb)),
               "NULL system array");
   afq_system* s = new afq_system();
   s->ctx = ctx;
@@ -381,6 +383,27 @@ int afq_system_create(afq_ctx* ctx, afq_system** out, int M, int Na, int Nb, int
     return rc;
   }
   *out = s;
+  return AFQ_OK;
+}
+
+int afq_system_set_chol_sign(afq_system* s, const double* sign) {
+  AFQ_REQUIRE(s, "system is NULL");
+  if (!sign || s->G == 0) {
+    s->chol_sign = nullptr;
+    return AFQ_OK;
+  }
+  std::vector<double> h(s->G);
+  AFQ_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+  AFQ_CHECK_HIP(hipMemcpy(h.data(), sign, sizeof(double) * s->G, hipMemcpyDeviceToHost));
+  bool neg = false;
+  for (double v : h) {
+    AFQ_REQUIRE(v == 1.0 || v == -1.0, "chol_sign entries must be +1 or -1 (got %g)", v);
+    neg = neg || v < 0.0;
+  }
+  if (neg && (s->precision != AFQ_FP64 || s->ndet > 0))
+    return afq::fail(AFQ_EUNSUPPORTED,
+                     "Cholesky vectors with sign -1 need the FP64 single-determinant path");
+  s->chol_sign = neg ? sign : nullptr;
   return AFQ_OK;
 }
 
@@ -436,7 +459,8 @@ int afq_propagate_walkers(afq_system* sys, int W, double* phia, double* phib, do
   AFQ_REQUIRE(sys, "system is NULL");
   AFQ_REQUIRE(W >= 0, "bad walker count");
   if (W == 0) return AFQ_OK;
-  AFQ_REQUIRE(phia && phib && weights && e_hybrid && ovlp_new, "NULL walker array");
+  AFQ_REQUIRE((sys->Na == 0 || phia) && (sys->Nb == 0 || phib) && weights && e_hybrid && ovlp_new,
+              "NULL walker array");
   if (sys->ndet > 0)
     return afq::propagate_multidet(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
                                    ovlp_new);

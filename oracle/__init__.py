@@ -14,6 +14,8 @@ Modules
 ``afqmc``   single-determinant step, Green's functions, energies, QR, comb,
             driver loop, binning analysis.
 ``multidet`` explicit multi-determinant Green's function / overlap / energy.
+``heg``     electron-gas step and sign-weighted local energy
+            (``OmegaQMC/afqmc_pw_heg.py``).
 ``philox``  Philox4x32-10 counter-based generator (Salmon et al., SC'11) and
             the Box-Muller normal stream the GPU generator also implements.
 
