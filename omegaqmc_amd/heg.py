@@ -330,3 +330,98 @@ def get_afqmc_3deg_func(system, dt=0.005, include_coulomb=True, verbose=True,
     """afqmc_pw_heg.py:948-967: a driver for the given 3DEG system."""
     return HEGDriver(system, dt=dt, include_coulomb=include_coulomb, verbose=verbose,
                      precision=precision, fbbound=fbbound, exp_nmax=exp_nmax)
+
+
+# ---------------------------------------------------------------------------
+# twist-averaged boundary conditions (afqmc_pw_heg.py:970-1094)
+# ---------------------------------------------------------------------------
+
+def _radical_inverse(i, base):
+    r, f = 0.0, 1.0
+    while i > 0:
+        f /= base
+        r += f * (i % base)
+        i //= base
+    return r
+
+
+def generate_halton_twists_3d(n_twists):
+    """Twists kappa in [-0.5, 0.5)^3: the unscrambled Halton points (radical
+    inverses of 0, 1, 2, ... in bases 2, 3, 5) minus 1/2
+    (afqmc_pw_heg.py:970-981)."""
+    return np.array([[_radical_inverse(i, b) for b in (2, 3, 5)] for i in range(n_twists)],
+                    dtype=np.float64).reshape(n_twists, 3) - 0.5
+
+
+def twisted_system_3d(base_system, kappa):
+    """Copy of a system with k = (n + kappa) dk (afqmc_pw_heg.py:984-1001)."""
+    kappa = np.asarray(kappa, dtype=np.float64)
+    tw = dict(base_system)
+    tw['kvecs'] = (np.asarray(base_system['grid_int'], dtype=np.float64) + kappa[None, :]) \
+        * base_system['dk']
+    tw['kappa'] = kappa
+    return tw
+
+
+def run_twist_averaged_afqmc_3deg(base_system, n_twists=20, dt=0.005, num_walkers=100,
+                                  num_blocks=50, num_steps_per_block=25, num_blocks_equil=10,
+                                  verbose=True, precision="fp64"):
+    """One AFQMC run per Halton twist, energies averaged over the twists and
+    errors combined in quadrature / n_twists (afqmc_pw_heg.py:1004-1094)."""
+    twists = generate_halton_twists_3d(n_twists)
+    e, err, et = np.zeros(n_twists), np.zeros(n_twists), np.zeros(n_twists)
+    for i, kap in enumerate(twists):
+        d = HEGDriver(twisted_system_3d(base_system, kap), dt=dt, include_coulomb=True,
+                      verbose=False, precision=precision)
+        r = d(num_walkers=num_walkers, num_blocks=num_blocks,
+              num_steps_per_block=num_steps_per_block, num_blocks_equil=num_blocks_equil)
+        e[i], err[i], et[i] = r['energy_mean'], r['energy_err'], d.e_trial
+        if verbose:
+            print(f"twist {i}/{n_twists} kappa=({kap[0]:.4f}, {kap[1]:.4f}, {kap[2]:.4f}) "
+                  f"E={e[i]:.8f} +/- {err[i]:.8f} E_HF={et[i]:.8f}", file=sys.stderr)
+    n = base_system['N_elec']
+    mean, emean = float(np.mean(e)), float(np.sqrt(np.sum(err ** 2)) / n_twists)
+    etm = float(np.mean(et))
+    return {'twists': twists, 'e_twists': e, 'e_err_twists': err, 'e_trial_twists': et,
+            'energy_mean': mean, 'energy_err': emean, 'e_trial_mean': etm,
+            'e_per_elec_ha': mean / n, 'e_per_elec_ry': 2.0 * mean / n,
+            'e_trial_per_elec_ry': 2.0 * etm / n}
+
+
+# ---------------------------------------------------------------------------
+# infinite-gas reference energies (afqmc_pw_heg.py:1097-1169)
+# ---------------------------------------------------------------------------
+
+def _kf(rs, polarization):
+    """Fermi wave vector: (9 pi / 4)^(1/3) / rs unpolarized, (9 pi / 2)^(1/3)
+    / rs fully polarized."""
+    return ((9.0 * np.pi / (4.0 if polarization == 'unpolarized' else 2.0)) ** (1.0 / 3.0)) / rs
+
+
+def hf_kinetic_energy_per_elec_ry(rs, polarization='unpolarized'):
+    """Kinetic energy per electron of the infinite Fermi sea, 3/5 kF^2 / 2 Ha,
+    in Ry (afqmc_pw_heg.py:1097-1111)."""
+    return 0.6 * _kf(rs, polarization) ** 2
+
+
+def hf_exchange_energy_per_elec_ry(rs, polarization='unpolarized'):
+    """Exchange energy per electron of the infinite Fermi sea, -3 kF / (4 pi)
+    Ha, in Ry (afqmc_pw_heg.py:1114-1136)."""
+    return 2.0 * (-3.0 / (4.0 * np.pi)) * _kf(rs, polarization)
+
+
+# Perdew-Zunger (Phys. Rev. B 23, 5048 (1981), Table XII) fits of the
+# Ceperley-Alder correlation energy, Hartree per electron: (gamma, beta1,
+# beta2) for rs >= 1 and (A, B, C, D) for rs < 1.
+_PZ = {'unpolarized': ((-0.1423, 1.0529, 0.3334), (0.0311, -0.048, 0.0020, -0.0116)),
+       'polarized': ((-0.0843, 1.3981, 0.2611), (0.01555, -0.0269, 0.0007, -0.0048))}
+
+
+def pz_correlation_energy(rs, polarization='unpolarized'):
+    """Correlation energy per electron (Hartree) of the Perdew-Zunger fit
+    (afqmc_pw_heg.py:1139-1169)."""
+    (g, b1, b2), (a, b, c, d) = _PZ['unpolarized' if polarization == 'unpolarized'
+                                   else 'polarized']
+    if rs >= 1.0:
+        return g / (1.0 + b1 * np.sqrt(rs) + b2 * rs)
+    return a * np.log(rs) + b + c * rs * np.log(rs) + d * rs

@@ -138,3 +138,19 @@ def test_bf16x3_refuses_antisymmetric_vectors(afq):
     s = H.build_3deg_system(2.0, 2, 7)
     with pytest.raises(afq.AFQError):
         H.HEGDriver(s, dt=0.01, verbose=False, precision="bf16x3")
+
+
+def test_twist_averaged_run(afq):
+    from omegaqmc_amd import heg as H
+    base = H.build_3deg_system(2.0, 2, 7)
+    out = H.run_twist_averaged_afqmc_3deg(base, n_twists=2, dt=0.01, num_walkers=16,
+                                          num_blocks=2, num_steps_per_block=3,
+                                          num_blocks_equil=1, verbose=False)
+    for i, kap in enumerate(out['twists']):
+        ints = H.prepare_3deg_integrals(H.twisted_system_3d(base, kap))
+        tu, td = H.build_trial_3deg(ints['h1e'], 1, 1)
+        e_hf = H.hf_energy_3deg(ints['h1e'], ints['chol'], ints['chol_sign'], tu, td,
+                                ints['e_madelung'] * 2)
+        np.testing.assert_allclose(out['e_trial_twists'][i], e_hf, rtol=1e-12)
+    assert np.all(np.isfinite(out['e_twists']))
+    np.testing.assert_allclose(out['energy_mean'], np.mean(out['e_twists']), rtol=1e-14)

@@ -245,3 +245,36 @@ def test_step_without_coulomb_is_one_body(gas):
                                     np.zeros((0, 7, 19)), 0.0, np.zeros(0))
     occ = np.sum(up ** 2, axis=1) > 0.5
     np.testing.assert_allclose(et.real, 2.0 * np.sum(eps[occ]), rtol=1e-12)
+
+
+# -- twists and infinite-gas references (afqmc_pw_heg.py:970-1169) --------
+
+def test_halton_twists():
+    from scipy.stats.qmc import Halton
+    t = H.generate_halton_twists_3d(12)
+    np.testing.assert_allclose(t, Halton(d=3, scramble=False).random(n=12) - 0.5, atol=1e-15)
+    assert np.all((t >= -0.5) & (t < 0.5))
+    s = H.build_3deg_system(1.0, 14, 19)
+    tw = H.twisted_system_3d(s, t[3])
+    ref = H.build_3deg_system(1.0, 14, 19, kappa=t[3])
+    np.testing.assert_allclose(tw['kvecs'], ref['kvecs'], rtol=1e-15)
+
+
+def test_infinite_gas_references():
+    """3/5 E_F kinetic and -3 kF / (4 pi) exchange (kF from the density
+    n = 3 / (4 pi rs^3), per spin channel for the unpolarized gas), and the
+    Perdew-Zunger fit continuous at rs = 1 for both branches."""
+    for rs in (0.8, 2.0, 5.0):
+        n = 3.0 / (4.0 * np.pi * rs ** 3)
+        kf_u = (3.0 * np.pi ** 2 * n) ** (1.0 / 3.0)
+        kf_p = (6.0 * np.pi ** 2 * n) ** (1.0 / 3.0)
+        np.testing.assert_allclose(H.hf_kinetic_energy_per_elec_ry(rs), 2 * 0.3 * kf_u ** 2,
+                                   rtol=1e-13)
+        np.testing.assert_allclose(H.hf_kinetic_energy_per_elec_ry(rs, 'polarized'),
+                                   2 * 0.3 * kf_p ** 2, rtol=1e-13)
+        np.testing.assert_allclose(H.hf_exchange_energy_per_elec_ry(rs),
+                                   -2 * 3.0 * kf_u / (4 * np.pi), rtol=1e-13)
+    for pol in ('unpolarized', 'polarized'):
+        lo, hi = H.pz_correlation_energy(1.0 - 1e-9, pol), H.pz_correlation_energy(1.0, pol)
+        assert abs(lo - hi) < 2e-4 and hi < 0
+        assert H.pz_correlation_energy(2.0, pol) > H.pz_correlation_energy(0.5, pol)
