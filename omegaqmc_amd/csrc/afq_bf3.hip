@@ -1196,6 +1196,42 @@ size_t exch_lds_bytes() {
 
 }  // namespace
 
+// Lhalf planes of one determinant with occupation string occ: element
+// (g, i, q) = L_g[occ[i]][q] (real L, identity-column determinants), laid out
+// as k_prep_lh's ([plane][G][K], (i, q) at i ldq + q, zero padding)
+__global__ void k_prep_lh_occ(long n, int N, int M, int ldq, long K, long plane,
+                              const int* __restrict__ occ, const double* __restrict__ L,
+                              bf16_t* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long g = e / K;
+    const int c = (int)(e % K);
+    const int i = c / ldq, q = c % ldq;
+    const double v = (i < N && q < M) ? L[(g * M + occ[i]) * M + q] : 0.0;
+    bf16_t h, l;
+    split2(v, h, l);
+    out[e] = h;
+    out[plane + e] = l;
+  }
+}
+
+// Ghalf (W, N, M) complex -> split planes [2][2W][K], rows (w, re/im),
+// (i, q) at i ldq + q, zero padding (the layout of GhalfBf16)
+__global__ void k_prep_gh(long n, int N, int M, int ldq, long K, long plane,
+                          const double* __restrict__ Gh, bf16_t* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long r = e / K;
+    const int c = (int)(e % K);
+    const int i = c / ldq, q = c % ldq;
+    const long w = r >> 1;
+    const int ri = (int)(r & 1);
+    const double v = (i < N && q < M) ? Gh[((w * N + i) * M + q) * 2 + ri] : 0.0;
+    bf16_t h, l;
+    split2(v, h, l);
+    out[e] = h;
+    out[plane + e] = l;
+  }
+}
+
 bool bf3_exchange_supported(const Bf3System& b) { return b.Na <= XMAXN && b.Nb <= XMAXN; }
 
 size_t bf3_exchange_part_bytes(const Bf3System& b, int W) {
@@ -1254,6 +1290,50 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
                      ntrs[1], parts[1], ex);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
+}
+
+size_t bf3_md_exchange_bytes(const Bf3System& b, int W) {
+  return align256((size_t)2 * b.G * b.Ka * sizeof(bf16_t)) +
+         align256((size_t)2 * b.G * b.Kb * sizeof(bf16_t)) +
+         align256((size_t)2 * 2 * W * b.Ka * sizeof(bf16_t)) +
+         align256((size_t)2 * 2 * W * b.Kb * sizeof(bf16_t)) +
+         align256(bf3_exchange_part_bytes(b, W));
+}
+
+int bf3_md_exchange(hipStream_t s, const Bf3System& b, int W, const int* occa, const int* occb,
+                    const double* chol, const double* gha, const double* ghb, void* scratch,
+                    double* ex) {
+  if (W <= 0) return AFQ_OK;
+  char* p = (char*)scratch;
+  auto take = [&](size_t n) {
+    char* r = p;
+    p += align256(n);
+    return r;
+  };
+  Bf3System d = b;
+  d.Lha = (bf16_t*)take((size_t)2 * b.G * b.Ka * sizeof(bf16_t));
+  d.Lhb = (bf16_t*)take((size_t)2 * b.G * b.Kb * sizeof(bf16_t));
+  bf16_t* pa = (bf16_t*)take((size_t)2 * 2 * W * b.Ka * sizeof(bf16_t));
+  bf16_t* pb = (bf16_t*)take((size_t)2 * 2 * W * b.Kb * sizeof(bf16_t));
+  double* part = (double*)take(bf3_exchange_part_bytes(b, W));
+  const int Ns[2] = {b.Na, b.Nb};
+  const long Ks[2] = {b.Ka, b.Kb};
+  const int* occ[2] = {occa, occb};
+  const double* gh[2] = {gha, ghb};
+  bf16_t* lh[2] = {d.Lha, d.Lhb};
+  bf16_t* gp[2] = {pa, pb};
+  for (int sp = 0; sp < 2; ++sp) {
+    if (Ns[sp] == 0 || b.G == 0) continue;
+    const long nl = (long)b.G * Ks[sp];
+    hipLaunchKernelGGL(k_prep_lh_occ, dim3(2048), dim3(256), 0, s, nl, Ns[sp], b.M, b.ldq, Ks[sp],
+                       nl, occ[sp], chol, lh[sp]);
+    AFQ_CHECK_LAUNCH();
+    const long ng = 2L * W * Ks[sp];
+    hipLaunchKernelGGL(k_prep_gh, dim3(2048), dim3(256), 0, s, ng, Ns[sp], b.M, b.ldq, Ks[sp], ng,
+                       gh[sp], gp[sp]);
+    AFQ_CHECK_LAUNCH();
+  }
+  return bf3_exchange(s, d, W, pa, pb, part, ex);
 }
 
 int bf3_coulomb(hipStream_t s, int W, int G, int nsplit, const float* part, double* ec) {

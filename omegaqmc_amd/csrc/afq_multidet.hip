@@ -430,20 +430,28 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
 // FP64 multi-det local energy given the buffers of md_greens(MD_GHALF) with G.
 // scratch: E2 (nd W complex) | rchol_a | rchol_b | X (W G complex) | ec | ex
 //          | exchange slab (gc N W N complex)
-static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int* gc_out) {
+// lp: the split-BF16 operands of a BF16x3 system (its exchange then runs on
+// k_exch_bf3 per determinant; NULL: FP64 exchange)
+static bool md_bf3_exchange(const Bf3System* lp) { return lp && bf3_exchange_supported(*lp); }
+
+static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int* gc_out,
+                                const Bf3System* lp = nullptr) {
   const long N = std::max(Na, Nb);
   long gc = N > 0 ? std::max(1L, std::min((long)G, (256L << 20) / (16L * N * W * N))) : 1;
   if (gc_out) *gc_out = (int)gc;
+  const size_t slab = md_bf3_exchange(lp) ? a256(bf3_md_exchange_bytes(*lp, W))
+                                          : a256(16UL * gc * N * W * N);
   return a256(16UL * nd * W) + a256(8UL * G * Na * M) + a256(8UL * G * Nb * M) +
-         a256(16UL * W * G) + 2 * a256(16UL * W) + a256(16UL * gc * N * W * N);
+         a256(16UL * W * G) + 2 * a256(16UL * W) + slab;
 }
 
 static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd, int chunk,
                      const double* h1e, const double* chol, const int* occa, const int* occb,
                      double enuc, const MdBufs& b, void* scr, double* e_tot, double* e_1b,
-                     double* e_2b) {
+                     double* e_2b, const Bf3System* lp = nullptr) {
   int gc = 1;
-  (void)md_energy_scratch(W, M, Na, Nb, G, nd, &gc);
+  (void)md_energy_scratch(W, M, Na, Nb, G, nd, &gc, lp);
+  const bool bx = md_bf3_exchange(lp);
   char* p = (char*)scr;
   auto take = [&](size_t n) {
     char* r = p;
@@ -456,7 +464,8 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
   double* X = (double*)take(16UL * W * G);
   double* ec = (double*)take(16UL * W);
   double* ex = (double*)take(16UL * W);
-  double* slab = (double*)take(16UL * gc * std::max(Na, Nb) * W * std::max(Na, Nb));
+  double* slab = (double*)take(bx ? bf3_md_exchange_bytes(*lp, W)
+                                  : 16UL * gc * std::max(Na, Nb) * W * std::max(Na, Nb));
   int rc = e1b_fullg_launch(s, W, M, h1e, b.Ga, b.Gb, enuc, e_1b);
   if (rc) return rc;
   for (int I = 0; I < nd; ++I) {
@@ -475,8 +484,14 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
     const double* ga = b.gha + 2L * I * W * Na * M;
     const double* gb = b.ghb + 2L * I * W * Nb * M;
     if ((rc = coulomb_launch(s, W, M, Na, Nb, G, ra, rb, false, ga, gb, X, ec))) return rc;
-    if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
-    if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
+    if (bx) {
+      if ((rc = bf3_md_exchange(s, *lp, W, occa + (long)I * Na, occb + (long)I * Nb, chol, ga, gb,
+                                slab, ex)))
+        return rc;
+    } else {
+      if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
+      if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
+    }
     hipLaunchKernelGGL(k_md_e2b_store, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W,
                        (const cplx*)ec, (const cplx*)ex, (cplx*)E2 + (long)I * W);
     AFQ_CHECK_LAUNCH();
@@ -676,7 +691,9 @@ int system_energy_multidet(afq_system* sys, int W, const double* phia, const dou
   hipStream_t s = sys->ctx->stream;
   const int M = sys->M, Na = sys->Na, Nb = sys->Nb, G = sys->G, nd = sys->ndet;
   const size_t nb = md_bytes(W, M, Na, Nb, nd, MD_GHALF, true);
-  const size_t ne = md_energy_scratch(W, M, Na, Nb, G, nd, nullptr);
+  const Bf3System* lp =
+      sys->precision == AFQ_BF16X3 ? (const Bf3System*)sys->chol_lp : nullptr;
+  const size_t ne = md_energy_scratch(W, M, Na, Nb, G, nd, nullptr, lp);
   char* scr = (char*)scratch(sys->ctx, nb + ne);
   if (!scr) return AFQ_ENOMEM;
   const MdBufs b = md_carve(scr, W, M, Na, Nb, nd, MD_GHALF, true);
@@ -684,7 +701,7 @@ int system_energy_multidet(afq_system* sys, int W, const double* phia, const dou
                      sys->ci, MD_GHALF, false, b);
   if (rc) return rc;
   return md_energy(s, W, M, Na, Nb, G, nd, sys->det_chunk, h1e, sys->chol, sys->occ_a,
-                   sys->occ_b, enuc, b, scr + nb, e_tot, e_1b, e_2b);
+                   sys->occ_b, enuc, b, scr + nb, e_tot, e_1b, e_2b, lp);
 }
 
 }  // namespace afq

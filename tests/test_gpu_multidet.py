@@ -129,6 +129,58 @@ def test_multidet_system_energy(afq):
     assert rel_err(_n(et), rt) < 1e-10 and rel_err(_n(e2), r2) < 1e-10
 
 
+def _per_det_two_body(s, pa, pb, tu, td, ci, emulate):
+    """sum_I w_I (E_coul,I - E_exch,I) / ovlp as local_energy_multidet forms
+    it, and the bound scale sum_I |w_I| (|E_coul,I| + |E_exch,I|) / |ovlp|;
+    emulate: each determinant's exchange products as the split BF16 ones
+    (float64; the Coulomb term stays exact, as on the GPU)."""
+    from afq_cases import bf16x3_two_body
+    from oracle import afqmc as O
+    omax = MD.det_overlap_max(pa, pb, tu, td)
+    e2 = np.zeros(pa.shape[0], complex)
+    scale = np.zeros(pa.shape[0])
+    ovlp = np.zeros(pa.shape[0], complex)
+    for d in range(tu.shape[0]):
+        gha, oa = MD._gf_spin_single_det(pa, tu[d])
+        ghb, ob = MD._gf_spin_single_det(pb, td[d])
+        gha, ghb = MD.mask_small_overlap(gha[None], ghb[None], oa[None], ob[None], omax)
+        w = ci[d].conj() * oa * ob
+        ra = np.einsum('pi,gpq->giq', tu[d].conj(), s.chol).real
+        rb = np.einsum('pi,gpq->giq', td[d].conj(), s.chol).real
+        ec, ex = O.local_energy_2body(gha[0], ghb[0], ra, rb)
+        if emulate:
+            ex = bf16x3_two_body(gha[0], ghb[0], ra, rb)[1]
+        e2 += w * (ec - ex)
+        scale += np.abs(w) * (np.abs(ec) + np.abs(ex))
+        ovlp += w
+    return e2 / ovlp, scale / np.abs(ovlp)
+
+
+def test_multidet_system_energy_bf16x3(afq):
+    """BF16x3 system: every determinant's exchange on the split-BF16 kernel
+    (the determinant algebra, Coulomb and E_1b stay FP64).  Each
+    determinant's two-body energy carries at most 3e-5 of its component
+    scale (the single-determinant bound), so the estimator's error is bounded
+    by 3e-5 sum_I |w_I| (|E_coul,I| + |E_exch,I|) / |ovlp| -- the weights
+    cancel in the estimate but not in the error.  The error is the split's
+    own: a float64 emulation of the same hi/lo products lands within a third
+    of it (plus 2e-7 of the scale for the FP32 sums)."""
+    import omegaqmc_amd as A
+    s, ou, od, ci, tu, td, pa, pb = _case(40, 8, 7, 60, 11, 6, 2)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    ref = A.MultiDetSystem(propd, _t(s.chol), _t(ou), _t(od), _t(ci), det_chunk_size=5)
+    bf = A.MultiDetSystem(propd, _t(s.chol), _t(ou), _t(od), _t(ci), det_chunk_size=5,
+                          precision="bf16x3")
+    rt, r1, r2 = (_n(x) for x in ref.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc))
+    et, e1, e2 = (_n(x) for x in bf.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc))
+    assert rel_err(e1, r1) < 1e-12
+    o2, scale = _per_det_two_body(s, pa, pb, tu, td, ci, emulate=False)
+    assert rel_err(r2, o2) < 1e-10
+    assert np.all(np.abs(e2 - r2) < 3e-5 * scale)
+    emu, _ = _per_det_two_body(s, pa, pb, tu, td, ci, emulate=True)
+    assert np.all(np.abs(e2 - emu) < 0.34 * np.abs(emu - r2) + 2e-7 * scale)
+
+
 def test_multidet_step_at_full_size(afq):
     """M=150, N=20+18, Nchol=800, 32 dets, 128 walkers: 3 steps from the
     dominant det; overlaps and weights finite, the multi-det overlap equals
