@@ -79,8 +79,8 @@ KERNEL_OF_PHASE = {
     ("vhs", "bf16x3"): "k_bf3_gemm<0> (VHS build, BF16x3 MFMA)",
     ("force_bias", "fp64"): "k_fb_f64 (split-K force bias, FP64 MFMA)",
     ("force_bias", "bf16x3"): "k_bf3_gemm<1> (split-K force bias, BF16x3 MFMA)",
-    ("greens", "fp64"): "k_greens (wave LU + MFMA Ghalf)",
-    ("greens", "bf16x3"): "k_greens (wave LU + MFMA Ghalf)",
+    ("greens", "fp64"): "k_greens_gj (wave Gauss-Jordan) + k_ghalf_inv (FP64 MFMA Ghalf)",
+    ("greens", "bf16x3"): "k_greens_gj (wave Gauss-Jordan) + k_ghalf_inv (FP64 MFMA Ghalf)",
 }
 
 
@@ -89,7 +89,7 @@ KERNEL_OF_PHASE = {
 KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop_f64<3>", ("taylor", "bf16x3"): "k_prop_bf3<3>",
                  ("vhs", "fp64"): "k_vhs_f64<true>", ("vhs", "bf16x3"): "k_bf3_gemm<0>",
                  ("force_bias", "fp64"): "k_fb_f64", ("force_bias", "bf16x3"): "k_bf3_gemm<1>",
-                 ("greens", "fp64"): "k_greens_gj<40, true>", ("greens", "bf16x3"): "k_greens_gj<40, true>"}
+                 ("greens", "fp64"): "k_ghalf_inv<10>", ("greens", "bf16x3"): "k_ghalf_inv<10>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phaseless_step_M300_N40_G1800_W1024_hbm_traffic.md")
 
 
