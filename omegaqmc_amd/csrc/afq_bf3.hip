@@ -811,22 +811,34 @@ namespace {
 constexpr int XNW = 4, XNT = 64 * XNW, XWT = 80, XRT = 2 * XWT, XCT = 2 * XWT, XGR = 8;
 constexpr int XMAXN = XWT / 2;
 constexpr int XSLOT = (2 * XRT) * 4 / XNT;  // 16-byte staging slots per thread and operand
-// One wave's epilogue block: the transposed T of one g block as (re, im)
-// float2 pairs, rows (walker, j) of XS2 pairs (XMAXN rows), then a dummy area
-// that absorbs the writes of rows and columns outside the tile.
-constexpr int XS2 = XMAXN + 1;
-constexpr int XDUM = 2 * XMAXN * XS2;  // floats
-constexpr int XEPI = XDUM + 144;
+// One wave's epilogue block: the T planes of one g block, T_c[wi][i][j] at
+// ((2 wi + c) N + i) XP + j (c = 0 real, 1 imaginary; nw 2N <= XWT rows of XP
+// floats), then a dummy area of 256 floats for the lanes of columns outside
+// the tile.  XP = 4 mod 8 keeps both the column writes of 16 lanes and the
+// 16-byte row reads of 8 or 16 lanes on distinct banks (32 or 64 banks).
+constexpr int XP = 44;
+constexpr int XDUM = XWT * XP;  // floats
+constexpr int XEPI = XDUM + 256;
+static_assert(XP >= XMAXN && XP % 8 == 4 && XDUM % 4 == 0 && XEPI % 4 == 0, "epilogue layout");
 // one staging buffer: planes (A hi, A lo, B hi, B lo) of XRT rows x GK, rows
 // of 64 bytes in blocks of 16 rows written by one LDS-DMA instruction each
 constexpr int XSTAGE = 4 * XRT * GK;  // bf16
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 // buffer load of 16 bytes per lane straight into LDS: lane l's bytes land at
-// lds + 16 l (lds uniform across the wavefront)
-__device__ void dma16(i32x4 rsrc, __attribute__((address_space(3))) unsigned* lds, int size,
-                      int voffset, int soffset, int offset, int aux)
-    __asm("llvm.amdgcn.raw.buffer.load.lds");
+// lds + 16 l (lds uniform across the wavefront).  Written as inline asm so
+// that the compiler does not see an LDS store: for the intrinsic form it
+// waits for every outstanding DMA before the next LDS read, which would
+// serialise the prefetch of the next K step with the reads of this one.  The
+// caller orders the DMA with its own vmcnt wait and barrier.  No VGPR is
+// written; the operands are read when the instruction issues.  M0 holds the
+// LDS base and needs one wait state before the load.
+__device__ __forceinline__ void dma16(i32x4 rsrc, unsigned lds, int voffset, int soffset) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
+               :
+               : "s"(__builtin_amdgcn_readfirstlane(lds)), "v"(voffset), "s"(rsrc), "s"(soffset)
+               : "memory", "m0");
+}
 
 
 struct Bf3Exch {
@@ -856,6 +868,7 @@ __device__ __forceinline__ void xwave_sync() {
 // two LDS stages by buffer-load DMA (no staging registers): the next K step's
 // chunk is in flight while this one computes, one barrier per step.  The
 // epilogue reuses both stages.
+template <bool VEC>
 __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
   extern __shared__ __align__(16) bf16_t lds[];
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
@@ -925,21 +938,19 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
   // chunk at K offset k0 into stage st (rows are ldq long, a multiple of GK,
   // zero past M: chunks never run into the next row)
   auto fetch = [&](int st, int k0) {
-    __attribute__((address_space(3))) unsigned* base =
-        (__attribute__((address_space(3))) unsigned*)(lds + st * XSTAGE);
+    const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
       const int blk = wv + 4 * i;  // 16-row block, uniform
-      dma16(rsa, base + blk * 256, 16, oa[i], 2 * k0, 0, 0);
-      dma16(rsb, base + (2 * XRT / 16 + blk) * 256, 16, ob[i], 2 * k0, 0, 0);
+      dma16(rsa, base + blk * 1024, oa[i], 2 * k0);
+      dma16(rsb, base + (2 * XRT / 16 + blk) * 1024, ob[i], 2 * k0);
     }
   };
   auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
   const int nk = a.ldq / GK;
 
-  float* Rt = (float*)lds + wv * XEPI;
-  double* cs = (double*)((float*)lds + XNW * XEPI) + wv * 2 * XWT;  // [80][2]
-  double* wsum = (double*)((float*)lds + XNW * XEPI) + XNW * 2 * XWT;  // [XNW][nw][2]
+  float* Tw = (float*)lds + wv * XEPI;
+  double* wsum = (double*)((float*)lds + XNW * XEPI);  // [XNW][nw][2]
   // lane offset of (row li, chunk lq) inside a 16-row block, in bf16
   const int frag = li * GK + 8 * xswz(li, lq);
 
@@ -978,18 +989,19 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       __syncthreads();
     }
 
-    // Epilogue.  The lane holding T_c[i][j] (c = re/im) of walker wi writes
-    // it transposed into the wave's block, Rt2[wi][j][i].c as (re, im)
-    // pairs; after a wave barrier one 8-byte read at Rt2[wi][i][j] returns
-    // (T_re[j][i], T_im[j][i]) = (p, q) and the lane accumulates A += x p,
-    // B += x q.  sum_ij T_ij T_ji then takes (A, B) from re lanes and
-    // (-B, A) from im lanes.  Products and the lane's partial sums of one g
-    // block (at most 16 terms) are FP32, the sums over g blocks, lanes and
-    // tiles FP64.  Full 16-row tiles are straight-line code with compile-time
-    // LDS offsets; the tiles that straddle a g-block edge clamp their rows
-    // (zero weight) and send the writes of rows outside the block, like those
-    // of columns outside the tile, to the dummy area.
-    int wb2[5], rf2[5];
+    // Epilogue: E(g, w) = sum_ij T_ij T_ji for the g blocks and walkers of
+    // the wave tile.  Pass gi covers g block gi: the lanes holding its rows
+    // write them into the wave's planes T_c[wi][i][j] (a lane holds rows
+    // 4 lq .. 4 lq + 3 of one column), and after a wave barrier each reads
+    // the transposed partners T_c[wi][j][i .. i + 3] of its rows, one
+    // 16-byte load per plane (one load per element when N % 4 != 0, where a
+    // lane's rows can span two g blocks).  With T = Tr + i Ti,
+    //   Re E = sum Tr_ij Tr_ji - sum Ti_ij Ti_ji,   Im E = 2 sum Tr_ij Ti_ji,
+    // so every lane reads the partner in its own plane and the lanes of real
+    // columns also the imaginary one.  Products and a lane's partial sum over
+    // one g block (at most 16 terms) are FP32, the sums over g blocks, lanes
+    // and tiles FP64, all in a fixed order.
+    int wb[5], rb[5], wiv[5];
     bool cvv[5], cimv[5];
     {
       int lj = li;  // opaque per tile, as in slots()
@@ -1004,39 +1016,41 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
         const int j = cv ? rem - (cim ? N : 0) : 0;
         cvv[w] = cv;
         cimv[w] = cim;
-        wb2[w] = ((wi * N + j) * XS2 + 4 * lq) * 2 + (cim ? 1 : 0);  // floats, + 2 (i - 4 lq)
-        rf2[w] = (wi * N + 4 * lq) * XS2 + j;  // pairs, + (i - 4 lq) XS2
+        wiv[w] = wi;
+        wb[w] = cv ? (2 * wi + (cim ? 1 : 0)) * N * XP + j : XDUM + l;  // + i XP
+        rb[w] = cv ? ((2 * wi + (cim ? 1 : 0)) * N + j) * XP : XDUM + 4 * l;  // + i
       }
     }
-    const float2* Rt2 = (const float2*)Rt;
     double cA[5], cB[5];
 #pragma unroll
     for (int w = 0; w < 5; ++w) cA[w] = cB[w] = 0.0;
+    const int NPl = N * XP;  // imaginary plane - real plane
     const int gw0 = (tr * 2 + wr) * a.ng;
     for (int gi = 0; gi < a.ng; ++gi) {
       if (gw0 + gi >= a.G) break;  // wave-uniform
-      const int lo = gi * N;       // first row of this g block in the wave tile
-      // row of the block for (u, r): i = 16 u - lo + 4 lq + r
-      int wcur[5];
-#pragma unroll
-      for (int w = 0; w < 5; ++w) wcur[w] = cvv[w] ? wb2[w] - 2 * lo : XDUM;
+      const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
 #pragma unroll
       for (int u = 0; u < 5; ++u) {
-        const int rlo = 16 * u - lo;
-        if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
-        if (rlo >= 0 && rlo + 16 <= N) {
+        if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+        if (VEC) {
+          const int r0 = 16 * u + 4 * lq;
+          if (r0 >= lo && r0 < hi) {
+            const int i0 = r0 - lo;
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+            for (int w = 0; w < 5; ++w) {
+              float* d = Tw + wb[w] + (cvv[w] ? i0 * XP : 0);
 #pragma unroll
-            for (int w = 0; w < 5; ++w) Rt[wcur[w] + 2 * (16 * u + r)] = acc[u][w][r];
+              for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
+            }
+          }
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = rlo + 4 * lq + r;
-            const bool ok = i >= 0 && i < N;
+            const int row = 16 * u + 4 * lq + r;
+            const bool own = row >= lo && row < hi;
 #pragma unroll
             for (int w = 0; w < 5; ++w)
-              Rt[(ok ? wcur[w] : XDUM) + 2 * (16 * u + r)] = acc[u][w][r];
+              Tw[own && cvv[w] ? wb[w] + (row - lo) * XP : XDUM + l] = acc[u][w][r];
           }
         }
       }
@@ -1046,30 +1060,40 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
 #pragma unroll
       for (int u = 0; u < 5; ++u) {
-        const int rlo = 16 * u - lo;
-        if (rlo + 16 <= 0 || rlo >= N) continue;  // uniform
-        if (rlo >= 0 && rlo + 16 <= N) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+        if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+        if (VEC) {
+          const int r0 = 16 * u + 4 * lq;
+          if (r0 >= lo && r0 < hi) {
+            const int i0 = r0 - lo;
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
-              const float x = acc[u][w][r];
-              const float2 v = Rt2[rf2[w] - lo * XS2 + (16 * u + r) * XS2];
-              A[w] = fmaf(x, v.x, A[w]);
-              B[w] = fmaf(x, v.y, B[w]);
+              const float* q = Tw + rb[w] + (cvv[w] ? i0 : 0);
+              const float4 ps = *(const float4*)q;
+              float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (cvv[w] && !cimv[w]) pi = *(const float4*)(q + NPl);
+              const f4v x = acc[u][w];
+              A[w] = fmaf(x[0], ps.x, A[w]);
+              A[w] = fmaf(x[1], ps.y, A[w]);
+              A[w] = fmaf(x[2], ps.z, A[w]);
+              A[w] = fmaf(x[3], ps.w, A[w]);
+              B[w] = fmaf(x[0], pi.x, B[w]);
+              B[w] = fmaf(x[1], pi.y, B[w]);
+              B[w] = fmaf(x[2], pi.z, B[w]);
+              B[w] = fmaf(x[3], pi.w, B[w]);
             }
+          }
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = rlo + 4 * lq + r;
-            const bool ok = i >= 0 && i < N;
-            const int ic = min(max(i, 0), N - 1);
+            const int row = 16 * u + 4 * lq + r;
+            const bool own = row >= lo && row < hi;
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
+              const bool ok = own && cvv[w];
+              const float* q = Tw + (ok ? rb[w] + row - lo : XDUM + 4 * l);
               const float x = ok ? acc[u][w][r] : 0.f;
-              const float2 v = Rt2[rf2[w] + (ic - 4 * lq) * XS2];
-              A[w] = fmaf(x, v.x, A[w]);
-              B[w] = fmaf(x, v.y, B[w]);
+              A[w] = fmaf(x, q[0], A[w]);
+              if (ok && !cimv[w]) B[w] = fmaf(x, q[NPl], B[w]);
             }
           }
         }
@@ -1081,34 +1105,22 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       }
       xwave_sync();
     }
-    double cre[5], cimg[5];
-#pragma unroll
-    for (int w = 0; w < 5; ++w) {
-      cre[w] = cimv[w] ? -cB[w] : cA[w];
-      cimg[w] = cimv[w] ? cA[w] : cB[w];
-    }
-    // column sums over the four row groups of the lane layout, then per walker
-#pragma unroll
-    for (int w = 0; w < 5; ++w) {
-      double x = cre[w], y = cimg[w];
-      x += __shfl_xor(x, 16);
-      x += __shfl_xor(x, 32);
-      y += __shfl_xor(y, 16);
-      y += __shfl_xor(y, 32);
-      if (lq == 0) {
-        cs[2 * (16 * w + li)] = x;
-        cs[2 * (16 * w + li) + 1] = y;
-      }
-    }
-    xwave_sync();
-    if (l < a.nw) {
+    // per walker: the lane's columns in a fixed order, then the wave
+    for (int k = 0; k < a.nw; ++k) {  // uniform
       double x = 0.0, y = 0.0;
-      for (int c = 2 * N * l; c < 2 * N * (l + 1); ++c) {
-        x += cs[2 * c];
-        y += cs[2 * c + 1];
+#pragma unroll
+      for (int w = 0; w < 5; ++w) {
+        if (cvv[w] && wiv[w] == k) {
+          x += cimv[w] ? -cA[w] : cA[w];
+          y += cimv[w] ? 0.0 : 2.0 * cB[w];
+        }
       }
-      wsum[(wv * a.nw + l) * 2] = x;
-      wsum[(wv * a.nw + l) * 2 + 1] = y;
+      x = wave_sum(x);
+      y = wave_sum(y);
+      if (l == 0) {
+        wsum[(wv * a.nw + k) * 2] = x;
+        wsum[(wv * a.nw + k) * 2 + 1] = y;
+      }
     }
     __syncthreads();
     // the two row waves of a column hold disjoint g ranges of the same walkers
@@ -1189,8 +1201,7 @@ void exch_geom(int N, int G, int W, int* ng, int* nw, int* ntr, int* ntc) {
 
 size_t exch_lds_bytes() {
   const size_t stages = (size_t)2 * XSTAGE * sizeof(bf16_t);
-  const size_t epi = (size_t)XNW * XEPI * sizeof(float) + (size_t)XNW * 2 * XWT * sizeof(double) +
-                     (size_t)XNW * XWT * 2 * sizeof(double);
+  const size_t epi = (size_t)XNW * XEPI * sizeof(float) + (size_t)XNW * XWT * 2 * sizeof(double);
   return std::max(stages, epi);
 }
 
@@ -1282,8 +1293,11 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     if (a.ldq % GK) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: row stride %d", a.ldq);
     // persistent: two workgroups per CU, a multiple of 8 (one share per XCD)
     const long nwg = std::min<long>((long)num_cus() * 2, (nb + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_exch_bf3, dim3((unsigned)std::max<long>(8, nwg / 8 * 8)), dim3(XNT), sm,
-                       s, a);
+    const dim3 grid((unsigned)std::max<long>(8, nwg / 8 * 8));
+    if (a.N % 4 == 0)
+      hipLaunchKernelGGL(k_exch_bf3<true>, grid, dim3(XNT), sm, s, a);
+    else
+      hipLaunchKernelGGL(k_exch_bf3<false>, grid, dim3(XNT), sm, s, a);
     AFQ_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 64)), dim3(256), 0, s, W, ntrs[0], parts[0],
