@@ -937,6 +937,12 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
   };
   // chunk at K offset k0 into stage st (rows are ldq long, a multiple of GK,
   // zero past M: chunks never run into the next row)
+  auto fetch_slot = [&](int st, int k0, int i) {
+    const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
+    const int blk = wv + 4 * i;
+    dma16(rsa, base + blk * 1024, oa[i], 2 * k0);
+    dma16(rsb, base + (2 * XRT / 16 + blk) * 1024, ob[i], 2 * k0);
+  };
   auto fetch = [&](int st, int k0) {
     const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
 #pragma unroll
@@ -968,7 +974,6 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
 #pragma unroll
       for (int w = 0; w < 5; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
     for (int kt = 0; kt < nk; ++kt) {
-      if (kt + 1 < nk) fetch((kt + 1) & 1, (kt + 1) * GK);
       const bf16_t* st = lds + (kt & 1) * XSTAGE + frag;
       uint4 ah[5], al[5];
 #pragma unroll
@@ -982,6 +987,7 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
         const int c = XWT * wc + 16 * w;
         const uint4 bh = *(const uint4*)(st + (2 * XRT + c) * GK);
         const uint4 bl = *(const uint4*)(st + (3 * XRT + c) * GK);
+        if (kt + 1 < nk) fetch_slot((kt + 1) & 1, (kt + 1) * GK, w);
 #pragma unroll
         for (int u = 0; u < 5; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
       }
@@ -996,12 +1002,13 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
     // the transposed partners T_c[wi][j][i .. i + 3] of its rows, one
     // 16-byte load per plane (one load per element when N % 4 != 0, where a
     // lane's rows can span two g blocks).  With T = Tr + i Ti,
-    //   Re E = sum Tr_ij Tr_ji - sum Ti_ij Ti_ji,   Im E = 2 sum Tr_ij Ti_ji,
-    // so every lane reads the partner in its own plane and the lanes of real
-    // columns also the imaginary one.  Products and a lane's partial sum over
+    //   Re E = sum Tr_ij Tr_ji - sum Ti_ij Ti_ji,
+    //   Im E = sum Tr_ij Ti_ji + sum Ti_ij Tr_ji,
+    // so every lane reads the partner in its own plane (A) and in the other
+    // one (B).  Products and a lane's partial sum over
     // one g block (at most 16 terms) are FP32, the sums over g blocks, lanes
     // and tiles FP64, all in a fixed order.
-    int wb[5], rb[5], wiv[5];
+    int wb[5], rb[5], ro[5], wiv[5];
     bool cvv[5], cimv[5];
     {
       int lj = li;  // opaque per tile, as in slots()
@@ -1019,12 +1026,12 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
         wiv[w] = wi;
         wb[w] = cv ? (2 * wi + (cim ? 1 : 0)) * N * XP + j : XDUM + l;  // + i XP
         rb[w] = cv ? ((2 * wi + (cim ? 1 : 0)) * N + j) * XP : XDUM + 4 * l;  // + i
+        ro[w] = cv ? ((2 * wi + (cim ? 0 : 1)) * N + j) * XP : XDUM + 4 * l;
       }
     }
     double cA[5], cB[5];
 #pragma unroll
     for (int w = 0; w < 5; ++w) cA[w] = cB[w] = 0.0;
-    const int NPl = N * XP;  // imaginary plane - real plane
     const int gw0 = (tr * 2 + wr) * a.ng;
     for (int gi = 0; gi < a.ng; ++gi) {
       if (gw0 + gi >= a.G) break;  // wave-uniform
@@ -1067,10 +1074,9 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
             const int i0 = r0 - lo;
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
-              const float* q = Tw + rb[w] + (cvv[w] ? i0 : 0);
-              const float4 ps = *(const float4*)q;
-              float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-              if (cvv[w] && !cimv[w]) pi = *(const float4*)(q + NPl);
+              const int di = cvv[w] ? i0 : 0;
+              const float4 ps = *(const float4*)(Tw + rb[w] + di);
+              const float4 pi = *(const float4*)(Tw + ro[w] + di);
               const f4v x = acc[u][w];
               A[w] = fmaf(x[0], ps.x, A[w]);
               A[w] = fmaf(x[1], ps.y, A[w]);
@@ -1090,10 +1096,10 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
 #pragma unroll
             for (int w = 0; w < 5; ++w) {
               const bool ok = own && cvv[w];
-              const float* q = Tw + (ok ? rb[w] + row - lo : XDUM + 4 * l);
+              const int di = ok ? row - lo : 0;
               const float x = ok ? acc[u][w][r] : 0.f;
-              A[w] = fmaf(x, q[0], A[w]);
-              if (ok && !cimv[w]) B[w] = fmaf(x, q[NPl], B[w]);
+              A[w] = fmaf(x, Tw[rb[w] + di], A[w]);
+              B[w] = fmaf(x, Tw[ro[w] + di], B[w]);
             }
           }
         }
@@ -1112,7 +1118,7 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       for (int w = 0; w < 5; ++w) {
         if (cvv[w] && wiv[w] == k) {
           x += cimv[w] ? -cA[w] : cA[w];
-          y += cimv[w] ? 0.0 : 2.0 * cB[w];
+          y += cB[w];
         }
       }
       x = wave_sum(x);
