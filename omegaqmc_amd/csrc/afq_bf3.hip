@@ -128,6 +128,32 @@ __global__ void k_prep_lh(long n, int N, int M, int ldq, long K, long plane,
   }
 }
 
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+// buffer load of 16 bytes per lane straight into LDS: lane l's bytes land at
+// lds + 16 l (lds uniform across the wavefront).  Written as inline asm so
+// that the compiler does not see an LDS store: for the intrinsic form it
+// waits for every outstanding DMA before the next LDS read, which would
+// serialise the prefetch of the next K step with the reads of this one.  The
+// caller orders the DMA with its own vmcnt wait and barrier.  No VGPR is
+// written; the operands are read when the instruction issues.  M0 holds the
+// LDS base and needs one wait state before the load.
+__device__ __forceinline__ void dma16(i32x4 rsrc, unsigned lds, int voffset, int soffset) {
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
+               :
+               : "s"(__builtin_amdgcn_readfirstlane(lds)), "v"(voffset), "s"(rsrc), "s"(soffset)
+               : "memory", "m0");
+}
+
+// buffer resource over `bytes` bytes at p (raw buffer, 32-bit offsets)
+__device__ __forceinline__ i32x4 buf_rsrc(const void* p, unsigned bytes) {
+  const unsigned long long ad = (unsigned long long)p;
+  i32x4 r = {(int)(unsigned)ad, (int)((ad >> 32) & 0xffff), (int)bytes, 0x00020000};
+  r.x = __builtin_amdgcn_readfirstlane(r.x);
+  r.y = __builtin_amdgcn_readfirstlane(r.y);
+  r.z = __builtin_amdgcn_readfirstlane(r.z);
+  return r;
+}
+
 // ----------------------------------------------------------------------------
 // the GEMM core: 256 x 256 tile, K step 32, 8 waves of 64 x 128
 // ----------------------------------------------------------------------------
@@ -138,6 +164,7 @@ struct Bf3Split {
   const bf16_t* b[2];  // B planes (cols x ldb)
   long lda, ldb;
   int kbeg, kend;
+  unsigned abytes, bbytes;  // bytes of one A / B plane (< 2^32)
 };
 
 struct Bf3Gemm {
@@ -166,45 +193,40 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   const int row0 = tm * GB, col0 = tn * GB;
   const Bf3Split& S = a.sp[blockIdx.y];
 
-  // two stage buffers: the next K step is stored while this one is consumed;
-  // rows of GK with the 16-byte chunks XOR-swizzled within 16-row blocks
-  // (conflict-free MFMA operand reads)
+  // two stage buffers: the next K step lands in one while this one is
+  // consumed; rows of GK with the 16-byte chunks XOR-swizzled within 16-row
+  // blocks (conflict-free MFMA operand reads)
   constexpr int GSTAGE = 4 * GB * GK;
 
-  // staging slots (fixed over K): slot i covers plane i / 2, row
-  // ((t + GT i) & 1023) / 4, K chunk t % 4.  Row pointers are formed once so
-  // that the K loop issues plain global loads back to back (no pointer loads
-  // or selects in between, which would serialise them).  A row or column
-  // outside the matrix reads row 0 instead: it only feeds output rows or
-  // columns that the epilogue discards.
-  typedef unsigned v4u __attribute__((ext_vector_type(4)));
-  typedef __attribute__((address_space(1))) const v4u gv4u;
-  const gv4u* pa[4];
-  const gv4u* pb[4];
+  // Operand chunks go from global memory straight into LDS by buffer-load
+  // DMA (no staging registers): per stage 64 instructions of 16 rows each,
+  // 16 blocks of 16 rows in each of the planes A hi, A lo, B hi, B lo.  Lane l
+  // fetches row l / 4 of its block, K chunk xswz(l / 4, l % 4), so the chunk
+  // lands at its swizzled position l % 4.  A row or column outside the matrix
+  // reads row 0: it only feeds output rows or columns that the epilogue
+  // discards.
+  const i32x4 rs[4] = {buf_rsrc(S.a[0], S.abytes), buf_rsrc(S.a[1], S.abytes),
+                       buf_rsrc(S.b[0], S.bbytes), buf_rsrc(S.b[1], S.bbytes)};
+  const int lrow = l >> 2, lkc = xswz(lrow, l & 3);
+  unsigned off[8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int c = t + GT * i, r = (c & 1023) >> 2, kq = c & 3;
-    const int gr = row0 + r, gc = col0 + r;
-    pa[i] = (const gv4u*)(S.a[i >> 1] + (long)(gr < a.rows ? gr : 0) * S.lda + 8 * kq);
-    pb[i] = (const gv4u*)(S.b[i >> 1] + (long)(gc < a.cols ? gc : 0) * S.ldb + 8 * kq);
+  for (int i = 0; i < 8; ++i) {
+    // instruction i of this wave: operand plane q = i / 2 (A hi, A lo, B hi,
+    // B lo), 16-row block b = wv + 8 (i % 2)
+    const int q = i >> 1, b = wv + 8 * (i & 1);
+    const int r = 16 * b + lrow;
+    const bool isa = q < 2;
+    const int gr = (isa ? row0 : col0) + r;
+    const bool ok = gr < (isa ? a.rows : a.cols);
+    off[i] = (unsigned)((ok ? (long)gr : 0L) * (isa ? S.lda : S.ldb) * 2 + 16 * lkc);
   }
-  v4u ra[4], rb[4];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      ra[i] = pa[i][k0 >> 3];
-      rb[i] = pb[i][k0 >> 3];
-    }
+  auto fetch1 = [&](int buf, int k0, int i) {
+    const int q = i >> 1, b = wv + 8 * (i & 1);
+    const unsigned dst =
+        (unsigned)(uintptr_t)(lds + buf * GSTAGE + q * GB * GK + b * 16 * GK);
+    dma16(rs[q], dst, (int)off[i], 2 * k0);
   };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int c = t + GT * i, r = (c & 1023) >> 2, kq = c & 3;
-      const int pos = r * GK + 8 * xswz(r & 15, kq);
-      *(v4u*)(lds + buf * GSTAGE + (i >> 1) * GB * GK + pos) = ra[i];
-      *(v4u*)(lds + buf * GSTAGE + (2 + (i >> 1)) * GB * GK + pos) = rb[i];
-    }
-  };
+  auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
   const int frag = li * GK + 8 * xswz(li, lq);  // (row li, chunk lq) of a 16-row block
 
   f4v acc[4][8];
@@ -215,12 +237,12 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 
   const int nk = (S.kend - S.kbeg) / GK;
   if (nk > 0) {
-    load(S.kbeg);
-    store(0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) fetch1(0, S.kbeg, i);
+    dma_wait();
   }
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    if (kt + 1 < nk) load(S.kbeg + (kt + 1) * GK);
     const int buf = kt & 1;
     const bf16_t* st = lds + buf * GSTAGE + frag;
     uint4 ah[4], al[4];
@@ -235,11 +257,12 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
       const int c = 128 * wc + 16 * w;
       const uint4 bh = *(const uint4*)(st + (2 * GB + c) * GK);
       const uint4 bl = *(const uint4*)(st + (3 * GB + c) * GK);
+      // the other buffer was last read before the previous barrier
+      if (kt + 1 < nk) fetch1(buf ^ 1, S.kbeg + (kt + 1) * GK, w);
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
     }
-    // the other buffer was last read before the previous barrier
-    if (kt + 1 < nk) store(buf ^ 1);
+    dma_wait();
     __syncthreads();
   }
 
@@ -713,6 +736,11 @@ int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt
   a.sp[0].ldb = b.Gp;
   a.sp[0].kbeg = 0;
   a.sp[0].kend = b.Gp;
+  const size_t ab = (size_t)2 * W * b.Gp * 2, bb = (size_t)b.nblk * 256 * b.Gp * 2;
+  if (ab >= (1ull << 32) || bb >= (1ull << 32))
+    return fail(AFQ_EUNSUPPORTED, "bf16x3 VHS: operand plane of %zu / %zu bytes (limit 4 GiB)", ab, bb);
+  a.sp[0].abytes = (unsigned)ab;
+  a.sp[0].bbytes = (unsigned)bb;
   a.rows = 2 * W;
   a.cols = b.nblk * 256;
   a.ntm = cdiv_int(2L * W, GB);
@@ -751,6 +779,12 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
       S.ldb = K;
       S.kbeg = k0;
       S.kend = std::min(K, k0 + chunk);
+      const size_t ab = (size_t)2 * W * K * 2, bb = (size_t)b.G * K * 2;
+      if (ab >= (1ull << 32) || bb >= (1ull << 32))
+        return fail(AFQ_EUNSUPPORTED, "bf16x3 force bias: operand plane of %zu / %zu bytes (limit 4 GiB)",
+                    ab, bb);
+      S.abytes = (unsigned)ab;
+      S.bbytes = (unsigned)bb;
     }
   }
   *nsplit_out = n;
@@ -824,21 +858,6 @@ static_assert(XP >= XMAXN && XP % 8 == 4 && XDUM % 4 == 0 && XEPI % 4 == 0, "epi
 // of 64 bytes in blocks of 16 rows written by one LDS-DMA instruction each
 constexpr int XSTAGE = 4 * XRT * GK;  // bf16
 
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-// buffer load of 16 bytes per lane straight into LDS: lane l's bytes land at
-// lds + 16 l (lds uniform across the wavefront).  Written as inline asm so
-// that the compiler does not see an LDS store: for the intrinsic form it
-// waits for every outstanding DMA before the next LDS read, which would
-// serialise the prefetch of the next K step with the reads of this one.  The
-// caller orders the DMA with its own vmcnt wait and barrier.  No VGPR is
-// written; the operands are read when the instruction issues.  M0 holds the
-// LDS base and needs one wait state before the load.
-__device__ __forceinline__ void dma16(i32x4 rsrc, unsigned lds, int voffset, int soffset) {
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
-               :
-               : "s"(__builtin_amdgcn_readfirstlane(lds)), "v"(voffset), "s"(rsrc), "s"(soffset)
-               : "memory", "m0");
-}
 
 
 struct Bf3Exch {
@@ -895,15 +914,7 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
   int v = next_tile(v0 + jx);
   if (v >= v1) return;  // whole workgroup: before any barrier
 
-  auto rsrc = [](const void* p, int bytes) {
-    const unsigned long long ad = (unsigned long long)p;
-    i32x4 r = {(int)(unsigned)ad, (int)((ad >> 32) & 0xffff), bytes, 0x00020000};
-    r.x = __builtin_amdgcn_readfirstlane(r.x);
-    r.y = __builtin_amdgcn_readfirstlane(r.y);
-    r.z = __builtin_amdgcn_readfirstlane(r.z);
-    return r;
-  };
-  const i32x4 rsa = rsrc(a.a, 4 * a.aplane), rsb = rsrc(a.b, 4 * a.bplane);
+  const i32x4 rsa = buf_rsrc(a.a, 4u * a.aplane), rsb = buf_rsrc(a.b, 4u * a.bplane);
 
   // DMA slots: wave wv's instruction i fills 16-row block wv + 4 i of the A
   // planes (rows 0..2*XRT-1 = (plane, row)) and of the B planes; lane l
