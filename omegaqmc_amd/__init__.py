@@ -18,5 +18,6 @@ from .afqmc_gto import (  # noqa: F401
 from ._lib import AFQError  # noqa: F401
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
 from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401
+from .edga import load_edga_result, make_omegaqmc_trial, select_top_determinants  # noqa: F401
 
 __version__ = "0.1.0"
