@@ -42,6 +42,13 @@ int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, fl
                   double* phia, double* phib, int nmax);
 // Exchange energy e_x[w] = 1/2 sum_s sum_g sum_ij T_ij T_ji, T = Lhalf_s[g] Ghalf_s^T,
 // from the split-bf16 Ghalf planes (N_s <= 40); part: bf3_exchange_part_bytes.
+// rows x cols FP64 -> split planes [2][rows][cols_p] (zero padding)
+int bf3_split_rows(hipStream_t s, const double* A, int rows, int cols, int cols_p, bf16_t* out);
+// vbias partials [nsplit][W][G] (FP32 re, im) = sum_c L[g][c] Gpk[w][c] over
+// split planes of the packed triangle: gpl [2][2W][Kp] (rows (w, re/im)),
+// lpl [2][G][Kp]; K split in up to four chunks
+int bf3_packed_force_bias(hipStream_t s, int W, int G, int Kp, const bf16_t* gpl,
+                          const bf16_t* lpl, float* part, int* nsplit_out);
 bool bf3_exchange_supported(const Bf3System& b);
 size_t bf3_exchange_part_bytes(const Bf3System& b, int W);
 int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, const bf16_t* ghb,

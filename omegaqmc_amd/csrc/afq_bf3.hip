@@ -803,6 +803,55 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
   return AFQ_OK;
 }
 
+int bf3_split_rows(hipStream_t s, const double* A, int rows, int cols, int cols_p, bf16_t* out) {
+  const long n = (long)rows * cols_p;
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, s, n, rows, cols, cols_p, n, A, out);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+int bf3_packed_force_bias(hipStream_t s, int W, int G, int Kp, const bf16_t* gpl,
+                          const bf16_t* lpl, float* part, int* nsplit_out) {
+  *nsplit_out = 0;
+  if (W <= 0 || G <= 0 || Kp <= 0) return AFQ_OK;
+  if (Kp % GK) return fail(AFQ_EUNSUPPORTED, "bf16x3 packed force bias: K %d", Kp);
+  if (gemm_lds() > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 GEMM: LDS %zu", gemm_lds());
+  const size_t ab = (size_t)2 * W * Kp * 2, bb = (size_t)G * Kp * 2;
+  if (ab >= (1ull << 32) || bb >= (1ull << 32))
+    return fail(AFQ_EUNSUPPORTED, "bf16x3 packed force bias: operand plane of %zu / %zu bytes", ab,
+                bb);
+  Bf3Gemm a{};
+  constexpr int nsp = 4;
+  const int chunk = ((Kp / GK + nsp - 1) / nsp) * GK;
+  int n = 0;
+  for (int k0 = 0; k0 < Kp; k0 += chunk) {
+    Bf3Split& S = a.sp[n++];
+    S.a[0] = gpl;
+    S.a[1] = gpl + (size_t)2 * W * Kp;
+    S.lda = Kp;
+    S.b[0] = lpl;
+    S.b[1] = lpl + (size_t)G * Kp;
+    S.ldb = Kp;
+    S.kbeg = k0;
+    S.kend = std::min(Kp, k0 + chunk);
+    S.abytes = (unsigned)ab;
+    S.bbytes = (unsigned)bb;
+  }
+  a.nsplit = n;
+  a.rows = 2 * W;
+  a.cols = G;
+  a.ntm = cdiv_int(2L * W, GB);
+  a.ntn = cdiv_int(G, GB);
+  a.W = W;
+  a.G = G;
+  a.part = (float2*)part;
+  hipLaunchKernelGGL((k_bf3_gemm<1>), dim3(a.ntm * a.ntn, n), dim3(GT), gemm_lds(), s, a);
+  AFQ_CHECK_LAUNCH();
+  *nsplit_out = n;
+  return AFQ_OK;
+}
+
 size_t bf3_prop_scratch_bytes(const Bf3System& b, int W) {
   const long ncb = (b.Na + PC - 1) / PC + (b.Nb + PC - 1) / PC;
   return (size_t)W * ncb * b.RT * 3 * 64 * 2 * sizeof(float4);

@@ -137,6 +137,10 @@ struct afq_system {
   int md_cap = 0;
   void* md_work = nullptr;
   size_t md_bytes = 0;
+  // split-BF16 planes of chol_packed for the BF16x3 multi-det force bias
+  // ([2][G][md_kp], built on first use, owned)
+  bf16_t* md_lpk = nullptr;
+  int md_kp = 0;
 };
 
 // Phases of the fused step timed by afq_system_timing_*.

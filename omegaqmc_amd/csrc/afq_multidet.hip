@@ -251,6 +251,34 @@ __global__ void k_md_gpack(long n, int M, int P, const int2* __restrict__ colmap
   }
 }
 
+// The packed triangle of k_md_gpack as split-BF16 planes for the BF16x3
+// force bias: plane [2][2W][Kp], row (w, re/im), column c (zero for
+// P <= c < Kp)
+__global__ void k_md_gpack_bf3(long n, int M, int P, int Kp, long plane,
+                               const int2* __restrict__ colmap, const cplx* __restrict__ Ga,
+                               bf16_t* __restrict__ out) {
+  const long MM = (long)M * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long w = e / Kp;
+    const int c = (int)(e % Kp);
+    cplx v = mkc(0.0, 0.0);
+    if (c < P) {
+      const int2 cm = colmap[c];
+      v = Ga[w * MM + cm.x];
+      if (cm.y >= 0) v = cadd(v, Ga[w * MM + cm.y]);
+    }
+    const long rr = 2 * w * Kp + c, ri = rr + Kp;
+    bf16_t h, l;
+    split2(v.x, h, l);
+    out[rr] = h;
+    out[plane + rr] = l;
+    split2(v.y, h, l);
+    out[ri] = h;
+    out[plane + ri] = l;
+  }
+}
+
 // rchol_I[g][i][q] = L[g][occ_I[i]][q]
 __global__ void k_md_rchol(long n, int M, int N, const int* __restrict__ occ,
                            const double* __restrict__ L, double* __restrict__ out) {
@@ -593,11 +621,28 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp: over the packed triangle on the
-  //    FP64 MFMA force-bias kernel (split-K) when L is symmetric, else one
-  //    generic GEMM over (p, q)
+  //    FP64 MFMA force-bias kernel (split-K) when L is symmetric (the
+  //    split-BF16 GEMM, FP32 partials, in a BF16x3 system), else one generic
+  //    GEMM over (p, q)
   phase_begin(sys, PH_FORCEBIAS);
   int nsplit = 1;
-  if (sys->chol_packed) {
+  float* fb_part = nullptr;
+  if (sys->chol_packed && sys->precision == AFQ_BF16X3) {
+    const int P = sys->packed_P, Kp = (P + 31) / 32 * 32;
+    if (!sys->md_lpk) {
+      AFQ_CHECK_HIP(hipMalloc(&sys->md_lpk, (size_t)2 * G * Kp * sizeof(bf16_t)));
+      sys->md_kp = Kp;
+      if ((rc = bf3_split_rows(s, sys->chol_packed, G, P, Kp, sys->md_lpk))) return rc;
+    }
+    const long n = (long)W * Kp;
+    hipLaunchKernelGGL(k_md_gpack_bf3, dim3(4096), dim3(256), 0, s, n, M, P, Kp, 2 * n,
+                       sys->colmap, (const cplx*)st.b.Ga, (bf16_t*)st.Gt);
+    AFQ_CHECK_LAUNCH();
+    fb_part = (float*)st.vbias;
+    if ((rc = bf3_packed_force_bias(s, W, G, Kp, (const bf16_t*)st.Gt, sys->md_lpk, fb_part,
+                                    &nsplit)))
+      return rc;
+  } else if (sys->chol_packed) {
     const int P = sys->packed_P;
     const long n = (long)W * P;
     hipLaunchKernelGGL(k_md_gpack, dim3(4096), dim3(256), 0, s, n, M, P, sys->colmap,
@@ -638,8 +683,8 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     if ((rc = lowp_step_buffers(sys, W, &X, &vb, &t0))) return rc;
     const Bf3System& b = *(const Bf3System*)sys->chol_lp;
     phase_begin(sys, PH_SHIFT);
-    rc = shift_launch(s, W, G, st.vbias, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr,
-                      nullptr, st.cfb, st.cmf, nsplit, nullptr, X, b.Gp);
+    rc = shift_launch(s, W, G, fb_part ? nullptr : st.vbias, sys->mf_shift, xi, sys->dt,
+                      sys->fbbound, nullptr, nullptr, st.cfb, st.cmf, nsplit, fb_part, X, b.Gp);
     phase_end(sys, PH_SHIFT);
     if (rc) return rc;
     phase_begin(sys, PH_VHS);
