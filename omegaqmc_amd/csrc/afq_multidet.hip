@@ -232,50 +232,85 @@ __global__ void k_md_gt(long n, int M, const cplx* __restrict__ Ga, const cplx* 
   }
 }
 
-// Gpk[w][c] = Gs[p][q] + Gs[q][p] (p < q) or Gs[p][p], Gs = Ga + Gb (Ga alone
-// when Gb is NULL), over the
-// packed upper triangle (colmap[c] = (p M + q, q M + p or -1)): with L
-// symmetric, sum_pq L_pq Gs_qp = sum_c Lpacked_c Gpk_c.
-__global__ void k_md_gpack(long n, int M, int P, const int2* __restrict__ colmap,
-                           const cplx* __restrict__ Ga, const cplx* __restrict__ Gb,
-                           cplx* __restrict__ Gpk) {
-  const long MM = (long)M * M;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
-       e += (long)gridDim.x * blockDim.x) {
-    const long w = e / P;
-    const int c = (int)(e % P);
-    const int2 cm = colmap[c];
-    cplx v = Gb ? cadd(Ga[w * MM + cm.x], Gb[w * MM + cm.x]) : Ga[w * MM + cm.x];
-    if (cm.y >= 0) v = cadd(v, Gb ? cadd(Ga[w * MM + cm.y], Gb[w * MM + cm.y]) : Ga[w * MM + cm.y]);
-    Gpk[e] = v;
+// The packed triangle straight from B_s and phi_s (no M x M G): with
+// G = sum_s B_s phi_s^T, S = G + G^T = sum_s [B_s | phi_s] [phi_s | B_s]^T,
+// formed over the upper-triangle 16 x 16 blocks on FP64 MFMA; packed column
+// c(p, q) (row-major upper triangle, as colmap) gets S[p][q] for p < q and
+// S[p][p] / 2 = G[p][p].  One wavefront per (walker, block pair), four per
+// workgroup; consecutive k-steps alternate between two accumulator sets.
+// Output: FP64 Gpk[w][c] (gpk) or split-BF16 planes [2][2W][Kp], row
+// (w, re/im), zero for P <= c < Kp (gbf).
+__global__ __launch_bounds__(256) void k_md_gsym(int W, int M, int Na, int Nb, int RT, int P,
+                                                 int Kp, const cplx* __restrict__ Ba,
+                                                 const cplx* __restrict__ Bb,
+                                                 const cplx* __restrict__ phia,
+                                                 const cplx* __restrict__ phib,
+                                                 cplx* __restrict__ gpk, bf16_t* __restrict__ gbf) {
+  const int npair = RT * (RT + 1) / 2;
+  const long gw = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (gw >= (long)W * npair) return;  // whole wavefront; no barrier in this kernel
+  const long w = gw / npair;
+  const int pr = (int)(gw % npair);
+  const int l = threadIdx.x & 63, li = l & 15, lq = l >> 4;
+  const long plane = 2L * W * Kp;
+  if (gbf && pr == 0 && l < Kp - P) {
+    const long rr = 2 * w * Kp + P + l;
+    gbf[rr] = gbf[rr + Kp] = gbf[plane + rr] = gbf[plane + rr + Kp] = 0;
   }
-}
-
-// The packed triangle of k_md_gpack as split-BF16 planes for the BF16x3
-// force bias: plane [2][2W][Kp], row (w, re/im), column c (zero for
-// P <= c < Kp)
-__global__ void k_md_gpack_bf3(long n, int M, int P, int Kp, long plane,
-                               const int2* __restrict__ colmap, const cplx* __restrict__ Ga,
-                               bf16_t* __restrict__ out) {
-  const long MM = (long)M * M;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
-       e += (long)gridDim.x * blockDim.x) {
-    const long w = e / Kp;
-    const int c = (int)(e % Kp);
-    cplx v = mkc(0.0, 0.0);
-    if (c < P) {
-      const int2 cm = colmap[c];
-      v = Ga[w * MM + cm.x];
-      if (cm.y >= 0) v = cadd(v, Ga[w * MM + cm.y]);
+  int PB = 0, rem = pr;  // pair -> block row PB <= block column QB
+  while (rem >= RT - PB) {
+    rem -= RT - PB;
+    ++PB;
+  }
+  const int QB = PB + rem;
+  const int pa = 16 * PB + li, qa = 16 * QB + li;  // A row (lane i), B column (lane j)
+  gj::d4v re[2], im[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) re[t] = im[t] = gj::d4v{0.0, 0.0, 0.0, 0.0};
+  for (int sp = 0; sp < 2; ++sp) {
+    const int N = sp ? Nb : Na;
+    if (N == 0) continue;
+    const cplx* Bs = (sp ? Bb : Ba) + w * M * N;
+    const cplx* ph = (sp ? phib : phia) + w * M * N;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const cplx* X = (h ? ph : Bs) + (long)min(pa, M - 1) * N;  // row of block PB
+      const cplx* Y = (h ? Bs : ph) + (long)min(qa, M - 1) * N;  // row of block QB
+      for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int j = k0 + 4 * t + lq;
+          const bool ok = j < N;
+          const cplx a = ok && pa < M ? X[j] : mkc(0.0, 0.0);
+          const cplx b = ok && qa < M ? Y[j] : mkc(0.0, 0.0);
+          re[t] = gj::mfma64(a.x, b.x, re[t]);
+          re[t] = gj::mfma64(-a.y, b.y, re[t]);
+          im[t] = gj::mfma64(a.x, b.y, im[t]);
+          im[t] = gj::mfma64(a.y, b.x, im[t]);
+        }
+      }
     }
-    const long rr = 2 * w * Kp + c, ri = rr + Kp;
-    bf16_t h, l;
-    split2(v.x, h, l);
-    out[rr] = h;
-    out[plane + rr] = l;
-    split2(v.y, h, l);
-    out[ri] = h;
-    out[plane + ri] = l;
+  }
+  // D[i][j] in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int p = 16 * PB + 4 * r + lq, q = 16 * QB + li;
+    if (p >= M || q >= M || p > q) continue;
+    const double f = p == q ? 0.5 : 1.0;
+    const cplx v = mkc(f * (re[0][r] + re[1][r]), f * (im[0][r] + im[1][r]));
+    const long c = (long)p * M - (long)p * (p - 1) / 2 + (q - p);
+    if (gbf) {
+      const long rr = 2 * w * Kp + c, ri = rr + Kp;
+      bf16_t hh, lo;
+      split2(v.x, hh, lo);
+      gbf[rr] = hh;
+      gbf[plane + rr] = lo;
+      split2(v.y, hh, lo);
+      gbf[ri] = hh;
+      gbf[plane + ri] = lo;
+    } else {
+      gpk[w * P + c] = v;
+    }
   }
 }
 
@@ -616,8 +651,10 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   }
   // 1: multi-det G (Ga + Gb into st.b.Ga) and overlap
   phase_begin(sys, PH_GREENS);
+  MdBufs gb = st.b;
+  if (sys->chol_packed) gb.Ga = nullptr;  // the packed triangle comes from B_s and phi_s
   rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 MD_INVERSE, true, st.b);
+                 MD_INVERSE, true, gb);
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp: over the packed triangle on the
@@ -634,9 +671,12 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
       sys->md_kp = Kp;
       if ((rc = bf3_split_rows(s, sys->chol_packed, G, P, Kp, sys->md_lpk))) return rc;
     }
-    const long n = (long)W * Kp;
-    hipLaunchKernelGGL(k_md_gpack_bf3, dim3(4096), dim3(256), 0, s, n, M, P, Kp, 2 * n,
-                       sys->colmap, (const cplx*)st.b.Ga, (bf16_t*)st.Gt);
+    const int RT = (M + 15) / 16;
+    const long nwv = (long)W * (RT * (RT + 1) / 2);
+    hipLaunchKernelGGL(k_md_gsym, dim3((unsigned)cdiv_int(nwv, 4)), dim3(256), 0, s, W, M, Na,
+                       Nb, RT, P, Kp,
+                       (const cplx*)st.b.Ba, (const cplx*)st.b.Bb, (const cplx*)phia,
+                       (const cplx*)phib, (cplx*)nullptr, (bf16_t*)st.Gt);
     AFQ_CHECK_LAUNCH();
     fb_part = (float*)st.vbias;
     if ((rc = bf3_packed_force_bias(s, W, G, Kp, (const bf16_t*)st.Gt, sys->md_lpk, fb_part,
@@ -644,9 +684,12 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
       return rc;
   } else if (sys->chol_packed) {
     const int P = sys->packed_P;
-    const long n = (long)W * P;
-    hipLaunchKernelGGL(k_md_gpack, dim3(4096), dim3(256), 0, s, n, M, P, sys->colmap,
-                       (const cplx*)st.b.Ga, (const cplx*)nullptr, (cplx*)st.Gt);
+    const int RT = (M + 15) / 16;
+    const long nwv = (long)W * (RT * (RT + 1) / 2);
+    hipLaunchKernelGGL(k_md_gsym, dim3((unsigned)cdiv_int(nwv, 4)), dim3(256), 0, s, W, M, Na,
+                       Nb, RT, P, 0,
+                       (const cplx*)st.b.Ba, (const cplx*)st.b.Bb, (const cplx*)phia,
+                       (const cplx*)phib, (cplx*)st.Gt, (bf16_t*)nullptr);
     AFQ_CHECK_LAUNCH();
     FbSplits sp{};
     const int nsp = 4;
