@@ -64,8 +64,13 @@ constexpr int GJ_LD = GJ_NMAX + 1;
 // reaches the other lanes by readlane (scalar operands of the update), not
 // through LDS: one-lane LDS writes and broadcast reads cost a full LDS
 // instruction each, and the eight wavefronts of a CU would queue on them.
+//
+// DET: the determinant only (LU form): step k updates just the N - k - 1
+// columns right of the pivot.  The rows still to be pivoted see exactly the
+// operations of the full elimination, so the pivots and the determinant are
+// bit-identical to it; the other rows and columns are left stale.
 // ----------------------------------------------------------------------------
-template <int NB>
+template <int NB, bool DET = false>
 __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, int* order,
                                         int& mystep, cplx* ov) {
   const int l = threadIdx.x & 63;
@@ -128,11 +133,13 @@ __device__ __forceinline__ void wave_gj(cplx (&a)[NB], int N, int* order,
     const cplx col = me ? id : mkc(-g.x, -g.y);
 #pragma unroll
     for (int j = 0; j + 1 < NB; ++j) {
-      cplx x = a[j + 1];
-      cfms(x, g, rlc(a[j + 1], p));
-      a[j] = x;
+      if (!DET || j + 1 < N - k) {  // uniform; the loop stays unrolled
+        cplx x = a[j + 1];
+        cfms(x, g, rlc(a[j + 1], p));
+        a[j] = x;
+      }
     }
-    a[NB - 1] = col;
+    if (!DET) a[NB - 1] = col;
   }
   wave_sync();  // order[] for gj_store_inverse
   if (l == 0) {

@@ -266,8 +266,9 @@ size_t gj_wave_lds(int mode, int N, bool planes) {
 
 // One walker per workgroup of two wavefronts, wave s = spin s; identity trials.
 // GH: the Ghalf modes (a separate instantiation, so that the overlap-only
-// kernel keeps the small register footprint of the elimination alone).
-template <int NB, bool GH>
+// kernel keeps the small register footprint of the elimination alone); DET:
+// the overlap alone (no inverse), by the determinant-only elimination.
+template <int NB, bool GH, bool DET = false>
 __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int Nb, int wlds,
                                                    const cplx* __restrict__ phia,
                                                    const cplx* __restrict__ phib,
@@ -293,8 +294,8 @@ __global__ __launch_bounds__(128) void k_greens_gj(int mode, int M, int Na, int 
 #pragma unroll
     for (int j = 0; j < NB; ++j) a[j] = (l < N && j < N) ? phi[(long)j * N + l] : mkc(0.0, 0.0);
     int mystep = 0;
-    gj::wave_gj<NB>(a, N, order, mystep, &ov[wv]);
-    if (!GH && inv_out)
+    gj::wave_gj<NB, DET>(a, N, order, mystep, &ov[wv]);
+    if (!GH && !DET && inv_out)
       gj::gj_store_inverse<NB>(a, N, mystep, order,
                                inv_out + w * ((long)Na * Na + (long)Nb * Nb) + (wv ? (long)Na * Na : 0),
                                N);
@@ -389,10 +390,14 @@ int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const d
     hipLaunchKernelGGL((k_greens_gj<NB, true>), dim3(W), dim3(128), sm2, s, gmode, M, Na, Nb,   \
                        (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb,   \
                        (cplx*)ovlp, g, nullptr);                                                \
-  else                                                                                          \
+  else if (two)                                                                                 \
     hipLaunchKernelGGL((k_greens_gj<NB, false>), dim3(W), dim3(128), sm2, s, gmode, M, Na, Nb,  \
                        (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha, (cplx*)Ghb,   \
-                       (cplx*)ovlp, g, two ? (cplx*)inv : nullptr);                             \
+                       (cplx*)ovlp, g, (cplx*)inv);                                             \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_greens_gj<NB, false, true>), dim3(W), dim3(128), sm2, s, gmode, M, Na, \
+                       Nb, (int)wl, (const cplx*)phia, (const cplx*)phib, (cplx*)Gha,           \
+                       (cplx*)Ghb, (cplx*)ovlp, g, nullptr);                                    \
   if (two) {                                                                                    \
     AFQ_CHECK_LAUNCH();                                                                         \
     hipLaunchKernelGGL((k_ghalf_inv<(NB + 3) / 4>), dim3(W, 2), dim3(64 * GHW), 0, s, M, Na, Nb, \

@@ -53,7 +53,7 @@ size_t md_wave_lds(int mode, int NB, int N) {
 
 // Grid W * nd (det fastest), two wavefronts: wave s = spin s.  mode 0: the
 // per-det overlaps; 1: also Ghalf_I = S_I^-1 phi^T; 2: also S_I^-1 itself.
-template <int NB>
+template <int NB, bool DET>
 __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, int Nb, int nd,
                                                int wlds, const cplx* __restrict__ phia,
                                                const cplx* __restrict__ phib,
@@ -82,7 +82,8 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
   for (int j = 0; j < NB; ++j)
     a[j] = (l < N && j < N) ? phi[(long)occ[j] * N + l] : mkc(0.0, 0.0);
   int mystep = 0;
-  gj::wave_gj<NB>(a, N, order, mystep, od);
+  gj::wave_gj<NB, DET>(a, N, order, mystep, od);
+  if (DET) return;
   if (mode == 1) {
     gj::gj_store_inverse<NB>(a, N, mystep, order, Inv);
     cplx* G = (wv ? ghb : gha) + ((long)I * W + w) * N * M;
@@ -438,10 +439,17 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "multi-det: LDS %zu", sm);
   if ((long)W * nd > 0x7fffffffL) return fail(AFQ_EUNSUPPORTED, "multi-det: grid too large");
   const dim3 grid((unsigned)((long)W * nd));
-#define AFQ_MDGJ(NBV)                                                                        \
-  hipLaunchKernelGGL((k_md_gj<NBV>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd, (int)wl, \
-                     (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda,           \
-                     (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb, (cplx*)b.inva, (cplx*)b.invb)
+#define AFQ_MDGJ(NBV)                                                                          \
+  do {                                                                                            \
+    if (mode == 0)                                                                                \
+      hipLaunchKernelGGL((k_md_gj<NBV, true>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd,    \
+                         (int)wl, (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda, \
+                         (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb, (cplx*)b.inva, (cplx*)b.invb); \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_md_gj<NBV, false>), grid, dim3(128), sm, s, mode, W, M, Na, Nb, nd,   \
+                         (int)wl, (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda, \
+                         (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb, (cplx*)b.inva, (cplx*)b.invb); \
+  } while (0)
   if (NB == 8) AFQ_MDGJ(8);
   else if (NB == 16) AFQ_MDGJ(16);
   else if (NB == 24) AFQ_MDGJ(24);
