@@ -10,10 +10,14 @@
 //                     16x16 blocks of the symmetric result; each workgroup
 //                     owns one block for 128 walkers and writes it and its
 //                     transpose in the block-major layout the propagator reads.
-//   k_bf3_gemm<FB>    split-K force bias, FP32 partials summed in FP64 later.
+//   k_bf3_gemm<FB>    split-K force bias, FP32 partials summed in FP64 later
+//                     (Lhalf . Ghalf, or packed L . packed G for multi-det
+//                     trials).  Both GEMMs stage their operands by LDS DMA.
 //   k_prop_bf3        expH1 . sum_n VHS^n/n! . expH1 (Horner form) on a 48-column
 //                     block of one spin of one walker, the block kept in LDS as
 //                     bf16 planes.
+//   k_exch_bf3        exchange energy sum_ij T_ij T_ji, T = Lhalf_g . Ghalf^T,
+//                     T kept in registers and LDS only.
 //
 // MFMA layout (16x16x32 bf16, checked on the device): A[i][k] in lane
 // i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8); D[i][j] in lane
