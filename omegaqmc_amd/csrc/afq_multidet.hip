@@ -433,7 +433,9 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
   if (Nmax > GJ_NMAX) return fail(AFQ_EUNSUPPORTED, "multi-det: nocc %d > %d", Nmax, GJ_NMAX);
   if (gsum && kind == MD_GHALF) return fail(AFQ_EINVAL, "multi-det: summed G needs MD_INVERSE");
   const int mode = kind == MD_GHALF ? 1 : kind == MD_INVERSE ? 2 : 0;
-  const int NB = Nmax <= 8 ? 8 : (Nmax + 7) / 8 * 8;
+  // register row length: N rounded up to a multiple of 4 (every padding
+  // column is a full update per elimination step)
+  const int NB = Nmax <= 8 ? 8 : (Nmax + 3) / 4 * 4;
   const size_t wl = md_wave_lds(mode, NB, Nmax);
   const size_t sm = 2 * wl;
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "multi-det: LDS %zu", sm);
@@ -450,11 +452,17 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
                          (int)wl, (const cplx*)phia, (const cplx*)phib, occa, occb, (cplx*)b.oda, \
                          (cplx*)b.odb, (cplx*)b.gha, (cplx*)b.ghb, (cplx*)b.inva, (cplx*)b.invb); \
   } while (0)
-  if (NB == 8) AFQ_MDGJ(8);
-  else if (NB == 16) AFQ_MDGJ(16);
-  else if (NB == 24) AFQ_MDGJ(24);
-  else if (NB == 32) AFQ_MDGJ(32);
-  else AFQ_MDGJ(40);
+  switch (NB) {
+    case 8: AFQ_MDGJ(8); break;
+    case 12: AFQ_MDGJ(12); break;
+    case 16: AFQ_MDGJ(16); break;
+    case 20: AFQ_MDGJ(20); break;
+    case 24: AFQ_MDGJ(24); break;
+    case 28: AFQ_MDGJ(28); break;
+    case 32: AFQ_MDGJ(32); break;
+    case 36: AFQ_MDGJ(36); break;
+    default: AFQ_MDGJ(40); break;
+  }
 #undef AFQ_MDGJ
   AFQ_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_md_weights, dim3(cdiv_int(W, 128)), dim3(128), 0, s, W, nd,
