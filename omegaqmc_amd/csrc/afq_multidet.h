@@ -20,7 +20,7 @@ struct MdBufs {
   double* invb = nullptr;  // [I][w][Nb][Nb]
   double* Ba = nullptr;    // B_a: [w][M][Na] complex
   double* Bb = nullptr;    // [w][M][Nb]
-  int* ioa = nullptr;      // row of orbital p in det I, or -1: [I][M]
+  int* ioa = nullptr;      // dets on orbital p: [M][nd] (I << 6 | row), then counts [M]
   int* iob = nullptr;
   double* oda = nullptr;   // per-det overlaps [w][I] complex
   double* odb = nullptr;

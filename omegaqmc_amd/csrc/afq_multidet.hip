@@ -94,20 +94,22 @@ __global__ __launch_bounds__(128) void k_md_gj(int mode, int W, int M, int Na, i
   }
 }
 
-// io[I][p] = i with occ_I[i] = p, or -1
-__global__ void k_md_invocc(int nd, int N, int M, const int* __restrict__ occ,
-                            int* __restrict__ io) {
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= (long)nd * M) return;
-  const int I = (int)(e / M), p = (int)(e % M);
-  int r = -1;
-  for (int i = 0; i < N; ++i)
-    if (occ[(long)I * N + i] == p) r = i;
-  io[e] = r;
+// The dets occupying orbital p, in det order: io[p nd + k] = I << 6 | i
+// (occ_I[i] = p) for k < io[M nd + p].  One thread per orbital.
+__global__ void k_md_occlist(int nd, int N, int M, const int* __restrict__ occ,
+                             int* __restrict__ io) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M) return;
+  int k = 0;
+  for (int I = 0; I < nd; ++I)
+    for (int i = 0; i < N; ++i)
+      if (occ[(long)I * N + i] == p) io[(long)p * nd + k++] = I << 6 | i;
+  io[(long)M * nd + p] = k;
 }
 
 // B_s[w][p][j] = sum_I z_I S_I^-1[i][j] (p = occ_I[i], kept dets only) /
-// overlap, the dets summed per chunk and the chunks in order.  Then
+// overlap, the dets summed per chunk and the chunks in order (a chunk with
+// no det on p adds nothing: the sums are those of the loop over all dets).  Then
 // G_s = T^* Ghalf summed over dets = B_s phi_s^T.  One thread per element of
 // both spins' B, grid-stride over W M (Na + Nb).
 __global__ __launch_bounds__(256) void k_md_bmat(long n, int W, int M, int Na, int Nb, int nd,
@@ -130,17 +132,22 @@ __global__ __launch_bounds__(256) void k_md_bmat(long n, int W, int M, int Na, i
     const int p = (int)(r / N), j = (int)(r % N);
     const int* io = sb ? iob : ioa;
     const cplx* inv = sb ? invb : inva;
-    cplx acc = mkc(0.0, 0.0);
-    for (int c0 = 0; c0 < nd; c0 += chunk) {
-      cplx cs = mkc(0.0, 0.0);
-      const int c1 = min(nd, c0 + chunk);
-      for (int I = c0; I < c1; ++I) {
-        const int i = io[(long)I * M + p];
-        if (i < 0 || !keep[w * nd + I]) continue;
-        cfma(cs, z[w * nd + I], inv[(((long)I * W + w) * N + i) * N + j]);
+    const int* el = io + (long)p * nd;
+    const int ne = io[(long)M * nd + p];
+    cplx acc = mkc(0.0, 0.0), cs = mkc(0.0, 0.0);
+    int cur = -1;
+    for (int k = 0; k < ne; ++k) {
+      const int I = el[k] >> 6, i = el[k] & 63;
+      const int ch = I / chunk;
+      if (ch != cur) {
+        if (cur >= 0) acc = cadd(acc, cs);
+        cs = mkc(0.0, 0.0);
+        cur = ch;
       }
-      acc = cadd(acc, cs);
+      if (!keep[w * nd + I]) continue;
+      cfma(cs, z[w * nd + I], inv[(((long)I * W + w) * N + i) * N + j]);
     }
+    if (cur >= 0) acc = cadd(acc, cs);
     (sb ? Bb : Ba)[(w * M + p) * N + j] = cdiv(acc, ovlp[w]);
   }
 }
@@ -366,7 +373,7 @@ size_t md_bytes(int W, int M, int Na, int Nb, int nd, int kind, bool gfull) {
   if (kind == MD_INVERSE) {
     o += a256(16UL * nd * W * Na * Na) + a256(16UL * nd * W * Nb * Nb);
     o += a256(16UL * W * M * Na) + a256(16UL * W * M * Nb);
-    o += 2 * a256(4UL * nd * M);
+    o += 2 * a256(4UL * (nd + 1) * M);
   }
   o += 2 * a256(16UL * W * nd);        // oda, odb
   o += a256(16UL * W * nd);            // z
@@ -393,8 +400,8 @@ MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, int kind, bool
     b.invb = (double*)take(16UL * nd * W * Nb * Nb);
     b.Ba = (double*)take(16UL * W * M * Na);
     b.Bb = (double*)take(16UL * W * M * Nb);
-    b.ioa = (int*)take(4UL * nd * M);
-    b.iob = (int*)take(4UL * nd * M);
+    b.ioa = (int*)take(4UL * (nd + 1) * M);
+    b.iob = (int*)take(4UL * (nd + 1) * M);
   }
   b.oda = (double*)take(16UL * W * nd);
   b.odb = (double*)take(16UL * W * nd);
@@ -475,10 +482,10 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
                        (cplx*)b.Ga, (cplx*)b.Gb, b.Ga ? 1 : 0);
     AFQ_CHECK_LAUNCH();
   } else if (kind == MD_INVERSE) {
-    const int nio = cdiv_int((long)nd * M, 256);
-    hipLaunchKernelGGL(k_md_invocc, dim3(nio), dim3(256), 0, s, nd, Na, M, occa, b.ioa);
+    const int nio = cdiv_int(M, 256);
+    hipLaunchKernelGGL(k_md_occlist, dim3(nio), dim3(256), 0, s, nd, Na, M, occa, b.ioa);
     AFQ_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_md_invocc, dim3(nio), dim3(256), 0, s, nd, Nb, M, occb, b.iob);
+    hipLaunchKernelGGL(k_md_occlist, dim3(nio), dim3(256), 0, s, nd, Nb, M, occb, b.iob);
     AFQ_CHECK_LAUNCH();
     const long n = (long)W * M * (Na + Nb);
     if (n > 0) {
