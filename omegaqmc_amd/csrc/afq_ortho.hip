@@ -18,6 +18,7 @@
 // lane j+16(i%4) register i/4); R^-1 comes from one wavefront: for N <= 40 by
 // an LDL^H elimination held in registers (one lane per row), else by the
 // column-by-column Cholesky factor and its inverse in LDS.
+#include "afq_gj.h"
 #include "afq_kernels.h"
 
 namespace afq {
@@ -87,9 +88,10 @@ __device__ void gram(const cplx* __restrict__ phi, int M, int N, int LD, cplx* G
 // R = D^1/2 L^H and R^-1 = L^-H D^-1/2 is written to Ri (column-major).
 // lg = sum_k log sqrt(d_k) (every lane).  Returns false (uniform) if a pivot
 // is not positive and finite, as the column-by-column factorisation would.
+// The pivot row reaches the other lanes by readlane (scalar operands of the
+// update), not through LDS.
 template <int NB>
-__device__ bool wave_ldl_rinv(const cplx* Gm, int LD, int N, double shift, cplx* bro, cplx* Ri,
-                              double& lg) {
+__device__ bool wave_ldl_rinv(const cplx* Gm, int LD, int N, double shift, cplx* Ri, double& lg) {
   const int l = threadIdx.x & 63;
   cplx a[NB];
   // G[l][j] = conj(G[j][l]), read along the contiguous column l of Gm
@@ -103,12 +105,7 @@ __device__ bool wave_ldl_rinv(const cplx* Gm, int LD, int N, double shift, cplx*
   bool ok = true;
   double myd = 1.0;
   for (int k = 0; k < N; ++k) {
-    if (l == k) {
-#pragma unroll
-      for (int j = 0; j < NB; ++j) bro[j] = a[j];
-    }
-    wave_sync();
-    const double d = bro[0].x;
+    const double d = gj::rl64(a[0].x, k);
     ok = ok && d > 0.0 && d <= 1.7976931348623157e308;
     const bool below = l > k && l < N;
     const double id = 1.0 / d;
@@ -116,12 +113,11 @@ __device__ bool wave_ldl_rinv(const cplx* Gm, int LD, int N, double shift, cplx*
 #pragma unroll
     for (int j = 0; j + 1 < NB; ++j) {
       cplx x = a[j + 1];
-      cfms(x, f, bro[j + 1]);
+      cfms(x, f, gj::rlc(a[j + 1], k));
       a[j] = x;
     }
     a[NB - 1] = l == k ? mkc(1.0, 0.0) : (below ? mkc(-f.x, -f.y) : mkc(0.0, 0.0));
     if (l == k) myd = d;
-    wave_sync();
   }
   if (l < N) {
     const double sc = 1.0 / sqrt(myd);
@@ -312,7 +308,6 @@ __global__ __launch_bounds__(64) void k_qr_factor(int pass, int M, int Na, int N
                                                   const cplx* __restrict__ Gbuf,
                                                   cplx* __restrict__ Ribuf, int* __restrict__ state,
                                                   double* __restrict__ ldr2) {
-  __shared__ cplx bro[NB];
   const long w = blockIdx.x;
   const int spin = blockIdx.y;
   const int N = spin ? Nb : Na;
@@ -329,13 +324,13 @@ __global__ __launch_bounds__(64) void k_qr_factor(int pass, int M, int Na, int N
   const cplx* G = Gbuf + id * NN;
   cplx* Ri = Ribuf + id * NN;
   double lg = 0.0;
-  bool ok = wave_ldl_rinv<NB>(G, N, N, 0.0, bro, Ri, lg);
+  bool ok = wave_ldl_rinv<NB>(G, N, N, 0.0, Ri, lg);
   if (!ok && pass == 0) {
     // s = 11 (M N + N (N+1)) u ||phi||_F^2
     double f = 0.0;
     for (int i = 0; i < N; ++i) f += G[i * N + i].x;
     const double sft = 11.0 * ((double)M * N + (double)N * (N + 1)) * 0x1.0p-53 * f;
-    ok = wave_ldl_rinv<NB>(G, N, N, sft, bro, Ri, lg);
+    ok = wave_ldl_rinv<NB>(G, N, N, sft, Ri, lg);
     st = ok ? 1 : 2;
   } else if (!ok) {
     st = 2;
