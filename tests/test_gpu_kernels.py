@@ -294,3 +294,4 @@ def test_propagate_one_step(afq, M, Na, Nb, G, W):
                                 _t(rb), _t(tu), _t(td), -1.0, None, xi=_t(xi))
     for x, y in zip(out[:5], ref):
         assert rel_err(_n(x), y) < TOL
+

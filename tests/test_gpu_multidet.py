@@ -69,6 +69,9 @@ def test_greens_function_multidet(afq, M, Na, Nb, G, nd, W, seed):
     assert rel_err(_n(ov), rov) < 1e-10
     ov2 = afq.greens_function_multidet_overlap_only(_t(pa), _t(pb), _t(tu), _t(td), _t(ci), 3)
     assert rel_err(_n(ov2), MD.greens_function_multidet_overlap_only(pa, pb, tu, td, ci, 3)) < 1e-10
+    # the overlap-only launch runs the determinant-only elimination: its
+    # pivots are those of the full one, so the overlaps agree bit for bit
+    assert np.array_equal(_n(ov), _n(ov2))
 
 
 def test_masked_dets_copies_of_det0(afq):
