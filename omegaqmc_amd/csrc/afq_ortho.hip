@@ -199,7 +199,9 @@ __device__ void apply_rinv(cplx* phi, int M, int N, int LD, const cplx* Ri) {
       const cplx a = (row < M && k < N) ? phi[(long)row * N + k] : mkc(0.0, 0.0);
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        if (c >= RTn) continue;
+        // R^-1 is upper triangular: column tile c has no k >= 16 (c + 1)
+        // (the skipped products are exact zeros)
+        if (c >= RTn || k0 >= 16 * (c + 1)) continue;  // uniform
         const int j = 16 * c + li;
         const cplx b = (j < N && k < N) ? Ri[j * LD + k] : mkc(0.0, 0.0);
         cr[c] = mfma64(a.x, b.x, cr[c]);
