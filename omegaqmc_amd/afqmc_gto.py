@@ -484,20 +484,38 @@ class System:
             pass
 
 
-_system_cache: dict = {}
+SYSTEM_CACHE_SIZE = 4
+
+
+def _cached_system(propagator, arrays, params, make):
+    """The prepared system of ``propagator`` for these input arrays.
+
+    Entries are keyed by the identity of the caller's arrays and keep strong
+    references to them, so an id cannot be recycled by another array while
+    its entry lives; at most SYSTEM_CACHE_SIZE entries (least recently used
+    first out) are kept per propagator, each holding device operands."""
+    if not isinstance(propagator, dict):
+        return make()
+    key = tuple(id(a) for a in arrays) + tuple(params)
+    cache = propagator.setdefault('_systems', {})
+    hit = cache.pop(key, None)
+    if hit is not None and all(x is y for x, y in zip(hit[0], arrays)):
+        cache[key] = hit  # most recent last
+        return hit[1]
+    sysobj = make()
+    cache[key] = (tuple(arrays), sysobj)
+    while len(cache) > SYSTEM_CACHE_SIZE:
+        cache.pop(next(iter(cache)))
+    return sysobj
 
 
 def _system_for(propagator, chol, rchol_a, rchol_b, trial_up, trial_dn, fbbound, exp_nmax,
                 precision="fp64"):
-    key = (id(propagator), id(chol), id(rchol_a), id(rchol_b), id(trial_up), id(trial_dn),
-           fbbound, exp_nmax, precision)
-    sysobj = propagator.get('_systems', {}).get(key) if isinstance(propagator, dict) else None
-    if sysobj is None:
-        sysobj = System(propagator, chol, rchol_a, rchol_b, trial_up, trial_dn, fbbound,
-                        exp_nmax, precision)
-        if isinstance(propagator, dict):
-            propagator.setdefault('_systems', {})[key] = sysobj
-    return sysobj
+    return _cached_system(
+        propagator, (chol, rchol_a, rchol_b, trial_up, trial_dn),
+        (fbbound, exp_nmax, precision),
+        lambda: System(propagator, chol, rchol_a, rchol_b, trial_up, trial_dn, fbbound,
+                       exp_nmax, precision))
 
 
 def propagate_walkers(phia, phib, weights, overlap, e_hybrid, propagator, chol, rchol_a, rchol_b,
@@ -645,14 +663,11 @@ def propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagato
                                exp_nmax=6, walker_chunk_size=None, chol_chunk_g=None,
                                det_chunk_size=5, xi=None, precision="fp64"):
     """afqmc_gto.py:728-839 -> (phia, phib, weights_new, overlap_new, e_hybrid_new, rng_key)."""
-    key = (id(propagator), id(chol), id(trials_up), id(trials_dn), id(ci_coeffs), fbbound,
-           exp_nmax, det_chunk_size, precision, "md")
-    sysobj = propagator.get('_systems', {}).get(key) if isinstance(propagator, dict) else None
-    if sysobj is None:
-        sysobj = MultiDetSystem(propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound,
-                                exp_nmax, det_chunk_size, precision)
-        if isinstance(propagator, dict):
-            propagator.setdefault('_systems', {})[key] = sysobj
+    sysobj = _cached_system(
+        propagator, (chol, trials_up, trials_dn, ci_coeffs),
+        (fbbound, exp_nmax, det_chunk_size, precision, "md"),
+        lambda: MultiDetSystem(propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound,
+                               exp_nmax, det_chunk_size, precision))
     pa, pb = _c(phia).clone(), _c(phib).clone()
     w = _r(weights).clone()
     eh = _c(e_hybrid).clone()

@@ -690,9 +690,17 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   if (sys->chol_packed && sys->precision == AFQ_BF16X3) {
     const int P = sys->packed_P, Kp = (P + 31) / 32 * 32;
     if (!sys->md_lpk) {
-      AFQ_CHECK_HIP(hipMalloc(&sys->md_lpk, (size_t)2 * G * Kp * sizeof(bf16_t)));
+      // published only once the split has been launched without error, so a
+      // failed first step never leaves a plane buffer that later steps trust
+      bf16_t* lpk = nullptr;
+      AFQ_CHECK_HIP(hipMalloc(&lpk, (size_t)2 * G * Kp * sizeof(bf16_t)));
+      if ((rc = bf3_split_rows(s, sys->chol_packed, G, P, Kp, lpk))) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(lpk);
+        return rc;
+      }
+      sys->md_lpk = lpk;
       sys->md_kp = Kp;
-      if ((rc = bf3_split_rows(s, sys->chol_packed, G, P, Kp, sys->md_lpk))) return rc;
     }
     const int RT = (M + 15) / 16;
     const long nwv = (long)W * (RT * (RT + 1) / 2);

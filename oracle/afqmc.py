@@ -103,14 +103,19 @@ def local_energy_1body(h1e, Ga, Gb, enuc):
 
 
 def local_energy_2body(Ghalfa, Ghalfb, rchol_a, rchol_b):
-    """energy.py:51-91 -> (e_coul, e_exch); exchange is sum_ij T_ij T_ji."""
-    Xa = np.einsum('giq,wiq->gw', rchol_a, Ghalfa)
-    Xb = np.einsum('giq,wiq->gw', rchol_b, Ghalfb)
-    e_coul = 0.5 * np.sum((Xa + Xb) ** 2, axis=0)
-    Ta = np.einsum('giq,wjq->gwij', rchol_a, Ghalfa)
-    Tb = np.einsum('giq,wjq->gwij', rchol_b, Ghalfb)
-    e_exch = (0.5 * np.sum(Ta * Ta.transpose(0, 1, 3, 2), axis=(0, 2, 3))
-              + 0.5 * np.sum(Tb * Tb.transpose(0, 1, 3, 2), axis=(0, 2, 3)))
+    """energy.py:51-91 -> (e_coul, e_exch); exchange is sum_ij T_ij T_ji.
+
+    X_g = einsum('giq,wiq->gw') and T = einsum('giq,wjq->gwij') are formed
+    as tensordot (BLAS) products; the sums over (g, i, j) follow."""
+    def coul(rc, gh):
+        return np.tensordot(rc, gh, axes=([1, 2], [1, 2]))           # (G, W)
+
+    def exch(rc, gh):
+        T = np.tensordot(rc, gh, axes=([2], [2])).transpose(0, 2, 1, 3)  # (G, W, i, j)
+        return 0.5 * np.sum(T * T.transpose(0, 1, 3, 2), axis=(0, 2, 3))
+
+    e_coul = 0.5 * np.sum((coul(rchol_a, Ghalfa) + coul(rchol_b, Ghalfb)) ** 2, axis=0)
+    e_exch = exch(rchol_a, Ghalfa) + exch(rchol_b, Ghalfb)
     return e_coul, e_exch
 
 

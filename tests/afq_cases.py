@@ -58,15 +58,17 @@ def bf16x3_two_body(Ghalfa, Ghalfb, rchol_a, rchol_b):
     """(e_coul, e_exch) of energy.py:51-91 with every product a*b replaced by
     ah*bh + ah*bl + al*bh of the split operands, summed in float64: the error
     the split-BF16 kernels are designed to have (without FP32 accumulation)."""
-    def prod(sub, L, G):
+    def prod(axes, L, G):
         Lh, Ll = bf16x3_split(L)
         Gh, Gl = bf16x3_split(G)
-        f = lambda a, b: np.einsum(sub, a, b)  # noqa: E731
+        f = lambda a, b: np.tensordot(a, b, axes=axes)  # noqa: E731
         return f(Lh, Gh) + f(Lh, Gl) + f(Ll, Gh)
     e_exch = 0.0
     X = 0.0
+    tq, tiq = ([2], [2]), ([1, 2], [1, 2])
     for L, Gh in ((rchol_a.real, Ghalfa), (rchol_b.real, Ghalfb)):
-        T = prod('giq,wjq->gwij', L, Gh.real) + 1j * prod('giq,wjq->gwij', L, Gh.imag)
+        # T[g, w, i, j] = sum_q L[g, i, q] Gh[w, j, q]
+        T = (prod(tq, L, Gh.real) + 1j * prod(tq, L, Gh.imag)).transpose(0, 2, 1, 3)
         e_exch = e_exch + 0.5 * np.sum(T * T.transpose(0, 1, 3, 2), axis=(0, 2, 3))
-        X = X + prod('giq,wiq->gw', L, Gh.real) + 1j * prod('giq,wiq->gw', L, Gh.imag)
+        X = X + prod(tiq, L, Gh.real) + 1j * prod(tiq, L, Gh.imag)
     return 0.5 * np.sum(X ** 2, axis=0), e_exch
