@@ -169,6 +169,21 @@ int afq_population_control_comb(afq_ctx* ctx, int nwalkers, int nbasis, int nup,
                                 double* weights_new, double* phia_new, double* phib_new,
                                 double* e_hybrid_new, int32_t* idx);
 
+/* No counterpart in the reference: afq_population_control_comb for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results; the running sums of W > ~20000 walkers (beyond LDS) go to a
+ * fixed device array of AFQ_COMB_NOALLOC_MAX_WALKERS doubles (8 MiB, held
+ * by every device the library runs on until the process ends) instead of
+ * the context scratch.  W above the limit returns AFQ_EINVAL.  Calls that
+ * overlap on one device share that array and race. */
+#define AFQ_COMB_NOALLOC_MAX_WALKERS (1 << 20)
+int afq_population_control_comb_noalloc(afq_ctx* ctx, int nwalkers, int nbasis, int nup,
+                                        int ndown, const double* weights, const double* phia,
+                                        const double* phib, double r, const double* e_hybrid,
+                                        double* weights_new, double* phia_new,
+                                        double* phib_new, double* e_hybrid_new, int32_t* idx);
+
 /* afqmc_gto.py:229-238 (second half of population_control_comb): gather the
  * walkers and e_hybrid by given survivor indices idx (W,) int32 and reset the
  * weights to 1.  Lets a caller replay the population-control decisions of

@@ -51,6 +51,8 @@ SIGNATURES = {
     "afq_update_weights_phaseless": [_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P],
     "afq_orthogonalize_walkers": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
     "afq_population_control_comb": [_P, _I, _I, _I, _I, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P],
+    "afq_population_control_comb_noalloc": [_P, _I, _I, _I, _I, _P, _P, _P, _D, _P, _P, _P, _P,
+                                            _P, _P],
     "afq_population_control_apply": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "afq_clip_weights": [_P, _I, _P, _D],
     "afq_accumulate_hybrid": [_P, _I, _P, _P, _P],

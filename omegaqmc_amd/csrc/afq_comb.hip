@@ -6,9 +6,32 @@
 // The total and the running sum are accumulated left to right by a single
 // lane in FP64, so the indices are bit-for-bit those of a sequential host
 // loop; the searches and the walker gather are parallel.
+#include <mutex>
+
 #include "afq_kernels.h"
 
+// Running sums of afq_population_control_comb_noalloc when W does not fit in
+// LDS: a fixed array in device memory (each device holds its own copy for
+// the life of the process), so that entry point never allocates.  Host code
+// reaches it only through the address hipGetSymbolAddress returns.
+static __device__ __align__(256) double scratch[AFQ_COMB_NOALLOC_MAX_WALKERS];
+
 namespace afq {
+
+int comb_static_scratch(int device, double** out) {
+  static std::mutex mu;
+  static double* addr[64] = {};
+  if (device < 0 || device >= 64) return fail(AFQ_EINVAL, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    // the caller (afq_ctx_create) has made `device` current
+    AFQ_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    addr[device] = (double*)p;
+  }
+  *out = addr[device];
+  return AFQ_OK;
+}
 
 namespace {
 
@@ -235,6 +258,27 @@ int afq_population_control_comb(afq_ctx* ctx, int W, int M, int Na, int Nb, cons
   }
   return afq::comb_launch(ctx->stream, W, M, Na, Nb, weights, phia, phib, r, eh, w_new, phia_new,
                           phib_new, eh_new, idx, cum);
+}
+
+int afq_population_control_comb_noalloc(afq_ctx* ctx, int W, int M, int Na, int Nb,
+                                        const double* weights, const double* phia,
+                                        const double* phib, double r, const double* eh,
+                                        double* w_new, double* phia_new, double* phib_new,
+                                        double* eh_new, int32_t* idx) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0, "bad shape");
+  AFQ_REQUIRE(W <= AFQ_COMB_NOALLOC_MAX_WALKERS,
+              "comb without allocation: %d walkers exceed the static limit of %d", W,
+              AFQ_COMB_NOALLOC_MAX_WALKERS);
+  AFQ_REQUIRE(r >= 0.0 && r < 1.0, "comb offset r=%g outside [0,1)", r);
+  AFQ_REQUIRE((Na == 0 || phia_new != phia) && (Nb == 0 || phib_new != phib),
+              "comb outputs must not alias inputs");
+  AFQ_REQUIRE(idx != nullptr && w_new != nullptr, "idx and weights_new are required");
+  AFQ_REQUIRE(eh == nullptr || eh_new != nullptr, "e_hybrid_new is required with e_hybrid");
+  AFQ_REQUIRE(ctx->comb_static, "context has no static comb buffer");
+  // every running sum is written before it is read: nothing to clear
+  return afq::comb_launch(ctx->stream, W, M, Na, Nb, weights, phia, phib, r, eh, w_new, phia_new,
+                          phib_new, eh_new, idx, ctx->comb_static);
 }
 
 int afq_clip_weights(afq_ctx* ctx, int W, double* weights, double frac) {

@@ -92,6 +92,9 @@ struct afq_ctx {
   hipStream_t stream = nullptr;
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // device address of the comb's static running-sum array on this device
+  // (afq_population_control_comb_noalloc), looked up at context creation
+  double* comb_static = nullptr;
 };
 
 struct afq_system {
@@ -156,6 +159,9 @@ int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 // Ensure the context scratch holds at least `bytes`; returns pointer or null.
 void* scratch(afq_ctx* ctx, size_t bytes);
+// Device address of the comb's file-scope running-sum array on the current
+// device (hipGetSymbolAddress once per device, kept for the process).
+int comb_static_scratch(int device, double** out);
 // Largest dynamic LDS allocation a workgroup may request on the current device.
 size_t max_lds_bytes();
 // compute units of the current device

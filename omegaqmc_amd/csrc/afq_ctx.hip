@@ -83,9 +83,13 @@ int afq_ctx_create(afq_ctx** out, int device, void* stream) {
   AFQ_CHECK_HIP(hipGetDeviceCount(&ndev));
   AFQ_REQUIRE(device >= 0 && device < ndev, "afq_ctx_create: device %d of %d", device, ndev);
   AFQ_CHECK_HIP(hipSetDevice(device));
+  double* comb_static = nullptr;
+  int rc = afq::comb_static_scratch(device, &comb_static);
+  if (rc) return rc;
   afq_ctx* c = new afq_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;
+  c->comb_static = comb_static;
   *out = c;
   return AFQ_OK;
 }
