@@ -15,7 +15,9 @@
 //                     trials).  Both GEMMs stage their operands by LDS DMA.
 //   k_prop_bf3        expH1 . sum_n VHS^n/n! . expH1 (Horner form) on a 48-column
 //                     block of one spin of one walker, the block kept in LDS as
-//                     bf16 planes.
+//                     bf16 planes; each one-body half step as y + (expH1 - I) y
+//                     with y in the FP32 accumulators, so the fixed rounding of
+//                     expH1's near-1 diagonal does not compound step after step.
 //   k_exch_bf3        exchange energy sum_ij T_ij T_ji, T = Lhalf_g . Ghalf^T,
 //                     T kept in registers and LDS only.
 //
@@ -103,13 +105,15 @@ __global__ void k_prep_L(long n, int M, int G, int Gp, long NC, const int2* __re
   }
 }
 
-// rows x cols -> [plane][rows_p][cols_p] with zero padding
+// rows x cols -> [plane][rows_p][cols_p] with zero padding; diag_sub is
+// taken off the diagonal first (A - diag_sub I)
 __global__ void k_prep_mat(long n, int rows, int cols, int cols_p, long plane,
-                           const double* __restrict__ A, bf16_t* __restrict__ out) {
+                           const double* __restrict__ A, double diag_sub,
+                           bf16_t* __restrict__ out) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const long r = e / cols_p;
     const int c = (int)(e % cols_p);
-    const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
+    const double v = (r < rows && c < cols) ? A[r * cols + c] - (r == c ? diag_sub : 0.0) : 0.0;
     bf16_t h, l;
     split2(v, h, l);
     out[e] = h;
@@ -334,6 +338,7 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 // ----------------------------------------------------------------------------
 // The Taylor sum runs in Horner form, y <- T0 + V y / n for n = nmax..1 with
 // y = T0 = expH1 phi to start (the same polynomial as afqmc_gto.py:451-469):
+// T0 = phi + D phi with D = expH1 - I.
 // only T0 has to be kept besides the current y, and it waits in a global
 // scratch in the accumulators' own lane order (t0s, RT * 3 * 64 * 8 floats per
 // workgroup), so the registers hold one accumulator set and two waves fit
@@ -458,10 +463,32 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       }
     }
   };
-  // k-steps in pairs with two static fragment buffers: the next k-step's
-  // expH1 rows are in flight while this one runs
+  // acc <- phi (FP64 -> FP32) on the owned tiles, zero outside M x N: the
+  // identity part of the first one-body half step
+  auto load_phi = [&]() {
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+      const int r0 = 16 * min(wv + PW * u, RT - 1) + 4 * lq;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const int c = c0 + 16 * x + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const bool ok = owns(u) && r0 + r < M && c < N;
+          const long i = ok ? ((long)w * M + r0 + r) * N + c : 0;
+          const double2 z = ok ? *(const double2*)(phi + 2 * i) : make_double2(0.0, 0.0);
+          are[u][x][r] = (float)z.x;
+          aim[u][x][r] = (float)z.y;
+        }
+      }
+    }
+  };
+  // acc += D . T with D = expH1 - I, so that a one-body half step is
+  // acc = y + D y when acc enters holding y (FP32): the identity part never
+  // passes through the BF16 split.  k-steps in pairs with two static
+  // fragment buffers: the next k-step's D rows are in flight while this one
+  // runs.
   auto onebody = [&]() {
-    zero();
     const int nks = Kp / 32;
     uint4 e0h[MT], e0l[MT], e1h[MT], e1l[MT];
     ldE(e0h, e0l, 0);
@@ -598,6 +625,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     }
   };
 
+  load_phi();
   onebody();
   __syncthreads();
   to_lds(are, aim);
@@ -631,7 +659,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     to_lds(are, aim);
     __syncthreads();
   }
-  onebody();
+  onebody();  // acc holds y_1: phi_out = y_1 + D y_1
 #pragma unroll
   for (int u = 0; u < MT; ++u) {
     if (!owns(u)) continue;
@@ -697,7 +725,11 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
   }
   {
     const long rows = 16L * b->RT, n = rows * b->Kp;
-    hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, st, n, M, M, b->Kp, n, s->expH1, b->E);
+    // the one-body half steps run as phi + (expH1 - I) phi: only the O(dt)
+    // part D = expH1 - I goes through the split-BF16 products (exact in FP64:
+    // the diagonal of expH1 lies within a factor 2 of 1)
+    hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, st, n, M, M, b->Kp, n, s->expH1, 1.0,
+                       b->E);
     AFQ_CHECK_LAUNCH();
   }
   if (G > 0 && s->Na > 0 && s->rchol_a) {
@@ -810,7 +842,7 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
 int bf3_split_rows(hipStream_t s, const double* A, int rows, int cols, int cols_p, bf16_t* out) {
   const long n = (long)rows * cols_p;
   if (n <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, s, n, rows, cols, cols_p, n, A, out);
+  hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, s, n, rows, cols, cols_p, n, A, 0.0, out);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

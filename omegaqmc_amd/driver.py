@@ -213,6 +213,78 @@ class AFQMCDriver:
         return out
 
 
+def lockstep_energy_error(ref, low, num_walkers, num_steps, seed=42, every=10,
+                          num_steps_per_block=25, stabilize_freq=5, pop_control_freq=10):
+    """Block-energy error of driver ``low`` against driver ``ref`` (same
+    system, typically FP64 and a reduced precision) on one trajectory.
+
+    The protocol is the reference driver's (afqmc_gto.py:1383-1502): blocks
+    of ``num_steps_per_block`` steps, eshift = the hybrid-energy average of
+    the block (afqmc_gto.py:1497-1502) taken from ``ref``'s accumulators and
+    used by both runs.  Both runs draw the same auxiliary fields (same Philox
+    stream); at every comb ``low`` replays ``ref``'s survivor indices, so the
+    populations stay walker-for-walker comparable and the difference is the
+    reduced-precision propagation plus ``low``'s own energy estimator (a
+    BF16x3 system evaluates its exchange and Coulomb in split BF16).  Every
+    ``every`` steps and at every block end both mixed estimators are
+    evaluated, uncapped and with the reference's local_energy_cap (real part
+    clipped to E_ref +/- sqrt(2/dt), E_ref = E_HF in block 0, then eshift +
+    e_const; afqmc_gto.py:1475-1485).
+
+    Returns a dict: 'trace' [(step, |dE| raw, |dE| capped, n_outside,
+    sigma)] in Ha, n_outside = walkers of ``ref`` beyond the cap, sigma = the
+    statistical error of ``ref``'s block energy (the |w|-weighted standard
+    deviation of the local energies over the square root of the Kish size
+    (sum |w|)^2 / sum w^2); 'block_ends' the same at block ends; 'max_raw',
+    'max_capped' over all checkpoints; 'max_block_raw', 'max_block_capped'
+    over block ends."""
+    W = int(num_walkers)
+    sr, sl = ref.init_walkers(W), low.init_walkers(W)
+    eshift = ref.ehf - ref.e_const
+    ebound = float(np.sqrt(2.0 / ref.dt))
+    trace, ends = [], []
+    iblock = 0
+    for step in range(1, int(num_steps) + 1):
+        if (step - 1) % num_steps_per_block == 0:
+            sr['acc'].zero_()
+            sl['acc'].zero_()
+        ref.step(sr, step, seed, eshift, stabilize_freq, pop_control_freq)
+        comb = sr['idx'] if step % pop_control_freq == 0 else None
+        low.step(sl, step, seed, eshift, stabilize_freq, pop_control_freq, comb_idx=comb)
+        end = step % num_steps_per_block == 0
+        if step % every == 0 or end:
+            e_ref = ref.ehf if iblock == 0 else eshift + ref.e_const
+            vals = []
+            nout, sigma = 0, 0.0
+            for drv, st in ((ref, sr), (low, sl)):
+                e = drv.block_energy(st).real
+                w = torch.abs(st['weights'])
+                raw = float(torch.sum(w * e) / torch.sum(w))
+                ec = torch.clamp(e, e_ref - ebound, e_ref + ebound)
+                cap = float(torch.sum(w * ec) / torch.sum(w))
+                vals.append((raw, cap))
+                if drv is ref:
+                    nout = int(torch.sum(torch.abs(e - e_ref) > ebound))
+                    var = float(torch.sum(w * (e - raw) ** 2) / torch.sum(w))
+                    kish = float(torch.sum(w) ** 2 / torch.sum(w * w))
+                    sigma = (var / kish) ** 0.5
+            rec = (step, abs(vals[1][0] - vals[0][0]), abs(vals[1][1] - vals[0][1]), nout, sigma)
+            trace.append(rec)
+            if end:
+                ends.append(rec)
+        if end:
+            acc = sr['acc'].cpu().numpy()
+            if acc[0] > 1e-10:
+                eshift = float(acc[1] / acc[0])
+            iblock += 1
+    out = {'trace': trace, 'block_ends': ends,
+           'max_raw': max((r[1] for r in trace), default=0.0),
+           'max_capped': max((r[2] for r in trace), default=0.0),
+           'max_block_raw': max((r[1] for r in ends), default=0.0),
+           'max_block_capped': max((r[2] for r in ends), default=0.0)}
+    return out
+
+
 def get_afqmc_func_from_integrals(h1, chol, ecore, nelec, mo_coeff=None, e_hf=None, dt=0.005,
                                   trial=None, eshift0=None, verbose=True, chol_chunk_g=128,
                                   det_chunk_size=5, multidet_method='explicit',
