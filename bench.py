@@ -359,6 +359,8 @@ def main():
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-multidet", action="store_true",
                     help="skip the 32-determinant M=150 workload")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the M=120 FP64 / FP32 / BF16x3 workload")
     ap.add_argument("--acc-steps", type=int, default=100,
                     help="steps of the same-trajectory energy comparison")
     args = ap.parse_args()
@@ -453,6 +455,37 @@ def main():
     bw["note"] = ("W=1024 arrays of a few KB: launch-latency bound, far below the 8000 GB/s "
                   "HBM roof; the comb gather moves the 393 MB walker set")
     extra["bandwidth_kernels"] = bw
+
+    if not args.no_config2:
+        # the FP64 vs FP32-MFMA workload (the BF16x3 step beside them)
+        M2, N2, G2, W2, S2 = 120, 16, 600, 2048, 200
+        c2 = {"workload": f"phaseless AFQMC step, M={M2} N={N2}+{N2} Nchol={G2} W={W2} "
+                          f"dt=0.005, {S2} steps for the energy error (size-normalised L_g)"}
+        print(f"[bench] M={M2} workload", file=sys.stderr, flush=True)
+        sys2 = make_system(M2, N2, N2, G2, seed=42, normalise=True)
+        pf2 = phase_flops(M2, N2, N2, G2, W2)
+        for prec in ("fp64", "fp32", "bf16x3"):
+            print(f"[bench] M={M2} {prec} GPU step", file=sys.stderr, flush=True)
+            r = run_gpu(sys2, W2, args.steps, args.warmup, prec)
+            ph = {k: round(v["ms_total"] / args.steps, 4) for k, v in r["phases"].items()}
+            tay = r["phases"].get("taylor")
+            peak = {"fp64": PEAK["fp64"], "fp32": PEAK["fp32"], "bf16x3": bf16x3_taylor_peak()}[prec]
+            tfl = (pf2["taylor"] / (tay["ms_total"] / tay["intervals"] * 1e-3) / 1e12) if tay else None
+            c2[prec] = {"walker_steps_per_s": round(W2 / (r["ms_per_step"] * 1e-3), 2),
+                        "ms_per_step": round(r["ms_per_step"], 4), "phases_ms_per_step": ph,
+                        "block_energy_ms": round(r["energy_ms"], 2),
+                        "propagator_tflops": round(tfl, 2) if tfl else None,
+                        "propagator_frac_of_peak": round(tfl / peak, 4) if tfl else None}
+            del r["drv"], r["state"]
+            torch.cuda.empty_cache()
+        for prec in ("fp32", "bf16x3"):
+            print(f"[bench] M={M2} {prec} vs FP64 energy, same trajectory ({S2} steps)",
+                  file=sys.stderr, flush=True)
+            c2[prec]["energy_error"] = lockstep_energy_error(sys2, W2, S2, prec, every=25)
+            c2[prec]["speedup_vs_fp64"] = round(c2[prec]["walker_steps_per_s"]
+                                                / c2["fp64"]["walker_steps_per_s"], 3)
+        del sys2
+        extra["config_m120"] = c2
 
     if not args.no_multidet:
         md = {"workload": "multi-det trial: M=150 N=20+18 Nchol=800 32 dets W=1024 dt=0.005 "

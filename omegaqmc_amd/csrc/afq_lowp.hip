@@ -92,14 +92,23 @@ int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs, float** 
 
 int system_prepare_lowp(afq_system* sys) {
   if (sys->precision == AFQ_FP64) return AFQ_OK;
+  if (sys->precision == AFQ_FP32) {
+    if (sys->chol_sign)
+      return fail(AFQ_EUNSUPPORTED, "signed Cholesky vectors need the FP64 path");
+    return f32_prepare(sys);
+  }
   if (sys->precision != AFQ_BF16X3)
-    return fail(AFQ_EUNSUPPORTED, "precision %d is not built (use FP64 or BF16X3)", sys->precision);
+    return fail(AFQ_EUNSUPPORTED, "precision %d is not built", sys->precision);
   Bf3System* b = new Bf3System();
   sys->chol_lp = b;
   return bf3_prepare(sys, b);
 }
 
 void system_release_lowp(afq_system* sys) {
+  if (sys->precision == AFQ_FP32) {
+    f32_release(sys);
+    return;
+  }
   if (sys->chol_lp) {
     bf3_release((Bf3System*)sys->chol_lp);
     delete (Bf3System*)sys->chol_lp;

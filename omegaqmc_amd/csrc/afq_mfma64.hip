@@ -1,20 +1,26 @@
-// FP64 MFMA kernels (v_mfma_f64_16x16x4f64) for the three dense contractions
-// of the phaseless step (OmegaQMC/afqmc_gto.py:451-482, 682-683):
+// FP64 (v_mfma_f64_16x16x4f64) and FP32 (v_mfma_f32_16x16x4f32) MFMA kernels
+// for the three dense contractions of the phaseless step
+// (OmegaQMC/afqmc_gto.py:451-482, 682-683), templated on the scalar type T:
 //
-//   k_vhs_f64   VHS[w] = i sqrt(dt) sum_g x[w,g] L_g, as a real GEMM whose rows
+//   k_vhs<T>    VHS[w] = i sqrt(dt) sum_g x[w,g] L_g, as a real GEMM whose rows
 //               are (walker, re/im) pairs and whose columns are the packed upper
 //               triangle of the symmetric L_g (both triangles of VHS written).
-//   k_fb_f64    vbias[w,g] = sum_k Ghalf[w,k] Lhalf[g,k], split over K with
-//               per-split partial sums reduced in a fixed order afterwards.
-//   k_prop_f64  phi <- expH1 . sum_{n<=nmax} VHS^n/n! . expH1 . phi, fused: one
+//   k_fb<T>     vbias[w,g] = sum_k Ghalf[w,k] Lhalf[g,k], split over K with
+//               per-split partial sums (FP64) reduced in a fixed order afterwards.
+//   k_prop<T>   phi <- expH1 . sum_{n<=nmax} VHS^n/n! . expH1 . phi, fused: one
 //               workgroup owns a 16-column block of one walker for the whole
 //               chain, keeping that block in LDS between the 2 + nmax products.
+//               The FP32 form applies the one-body half steps as y + D y with
+//               D = expH1 - I, y in the accumulators.
 //
-// MFMA operand layout (gfx950, checked on the device): A[i][k] in lane i + 16k,
-// B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i % 4), register i / 4.
-// So a lane holds rows q, q+4, q+8, q+12 of a 16-row tile (q = lane / 16).
-// GEMM rows that pair (walker re, walker im) are therefore ordered so that a
-// lane's four rows are (w, re), (w, im), (w+4, re), (w+4, im).
+// Walkers, Ghalf and the shifted fields stay FP64 in HBM; the FP32 kernels
+// convert them on load and write FP64 walkers.
+//
+// MFMA operand layout (gfx950, both checked on the device): A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k; D[i][j] in lane j + 16q, register r with
+// i = q + 4r (FP64) or i = 4q + r (FP32).  GEMM rows that pair (walker re,
+// walker im) are ordered (Mf<T>::prow) so that a lane's four output registers
+// are (w, re), (w, im), (w', re), (w', im).
 //
 // Workgroups are renumbered so that the hardware's round-robin assignment of
 // consecutive workgroups to the 8 XCDs puts tiles that share an operand
@@ -26,10 +32,36 @@ namespace afq {
 namespace {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+template <class T> struct Mf;
+template <> struct Mf<double> {
+  typedef d4v V4;
+  typedef cplx C;
+  static constexpr bool DFORM = false;
+  __device__ static V4 mma(double a, double b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // tile row held by lane group q in register r
+  __device__ static int orow(int q, int r) { return q + 4 * r; }
+  // row of component c (0 re, 1 im) of walker wi (0..7) of a 16-row tile
+  __device__ static int prow(int wi, int c) { return (wi & 3) + 8 * (wi >> 2) + 4 * c; }
+  // walker (0..7 of the tile) in registers 2h, 2h + 1 of lane group q
+  __device__ static int wlane(int q, int h) { return q + 4 * h; }
+  __device__ static C mk(double r, double i) { return mkc(r, i); }
+};
+template <> struct Mf<float> {
+  typedef f4v V4;
+  typedef float2 C;
+  static constexpr bool DFORM = true;
+  __device__ static V4 mma(float a, float b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int orow(int q, int r) { return 4 * q + r; }
+  __device__ static int prow(int wi, int c) { return 2 * wi + c; }
+  __device__ static int wlane(int q, int h) { return 2 * q + h; }
+  __device__ static C mk(double r, double i) { return make_float2((float)r, (float)i); }
+};
 
 // Virtual block index such that consecutive virtual indices run on one XCD.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -38,23 +70,22 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return (x < rem) ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
-// Row of the 16-row tile that holds component c (0 re, 1 im) of walker wi (0..7).
-__device__ __forceinline__ int pair_row(int wi, int c) { return (wi & 3) + 8 * (wi >> 2) + 4 * c; }
-
 constexpr int TM = 128, TN = 128, TK = 16, NTH = 256;
 constexpr int PNW = 8, PTH = 64 * PNW;  // waves / threads of the fused propagator
 
 // --------------------------------------------------------------------------
 // VHS build
 // --------------------------------------------------------------------------
-template <bool PACKED>
-__global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
-                                                 const cplx* __restrict__ xs,
-                                                 const double* __restrict__ L,
-                                                 const int2* __restrict__ colmap, double sqdt,
-                                                 cplx* __restrict__ VHS, int ntm, int ntn) {
-  __shared__ double As[TK][TM + 4];
-  __shared__ double Bs[TK][TN + 4];
+template <class T, bool PACKED>
+__global__ __launch_bounds__(NTH) void k_vhs(int W, int G, int P, long MM,
+                                             const cplx* __restrict__ xs,
+                                             const T* __restrict__ L,
+                                             const int2* __restrict__ colmap, double sqdt,
+                                             typename Mf<T>::C* __restrict__ VHS, int ntm,
+                                             int ntn) {
+  typedef Mf<T> F;
+  __shared__ T As[TK][TM + 4];
+  __shared__ T Bs[TK][TN + 4];
   const int v = xcd_remap(blockIdx.x, ntm * ntn);
   const int tm = v % ntm, tn = v / ntm;
   const int w0 = tm * (TM / 2), n0 = tn * TN;
@@ -62,7 +93,7 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
   const int wr = wv >> 1, wc = wv & 1;
 
   cplx ra[4];
-  double rb[8];
+  T rb[8];
   auto load = [&](int g0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -74,16 +105,15 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
     for (int i = 0; i < 8; ++i) {
       const int e = t + NTH * i, kk = e >> 7, nn = e & 127;
       const int g = g0 + kk, n = n0 + nn;
-      rb[i] = (g < G && n < P) ? L[(long)g * P + n] : 0.0;
+      rb[i] = (g < G && n < P) ? L[(long)g * P + n] : T(0);
     }
   };
   auto store = [&]() {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = t + NTH * i, wl = e >> 4, gk = e & 15;
-      const int row = 16 * (wl >> 3) + pair_row(wl & 7, 0);
-      As[gk][row] = ra[i].x;
-      As[gk][row + 4] = ra[i].y;
+      As[gk][16 * (wl >> 3) + F::prow(wl & 7, 0)] = (T)ra[i].x;
+      As[gk][16 * (wl >> 3) + F::prow(wl & 7, 1)] = (T)ra[i].y;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -92,11 +122,11 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
     }
   };
 
-  d4v acc[4][4];
+  typename F::V4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) acc[i][j] = typename F::V4{0, 0, 0, 0};
 
   const int nk = (G + TK - 1) / TK;
   load(0);
@@ -107,7 +137,7 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
 #pragma unroll
     for (int ks = 0; ks < TK / 4; ++ks) {
       const int kl = 4 * ks + (l >> 4);
-      double a[4], b[4];
+      T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[kl][64 * wr + 16 * i + (l & 15)];
 #pragma unroll
@@ -115,7 +145,7 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = F::mma(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
     if (kt + 1 < nk) {
@@ -128,7 +158,7 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int tile = 4 * wr + i;
-    const int wA = w0 + 8 * tile + q, wB = wA + 4;
+    const int wA = w0 + 8 * tile + F::wlane(q, 0), wB = w0 + 8 * tile + F::wlane(q, 1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n = n0 + 64 * wc + 16 * j + (l & 15);
@@ -139,14 +169,14 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
         o1 = cm.x;
         o2 = cm.y;
       }
-      const d4v c = acc[i][j];
+      const auto c = acc[i][j];
       if (wA < W) {
-        const cplx val = mkc(-sqdt * c[1], sqdt * c[0]);
+        const auto val = F::mk(-sqdt * c[1], sqdt * c[0]);
         VHS[wA * MM + o1] = val;
         if (o2 >= 0) VHS[wA * MM + o2] = val;
       }
       if (wB < W) {
-        const cplx val = mkc(-sqdt * c[3], sqdt * c[2]);
+        const auto val = F::mk(-sqdt * c[3], sqdt * c[2]);
         VHS[wB * MM + o1] = val;
         if (o2 >= 0) VHS[wB * MM + o2] = val;
       }
@@ -157,10 +187,12 @@ __global__ __launch_bounds__(NTH) void k_vhs_f64(int W, int G, int P, long MM,
 // --------------------------------------------------------------------------
 // force bias, split K
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx* __restrict__ part,
-                                                int ntm, int ntn) {
-  __shared__ double As[TK][TM + 4];
-  __shared__ double Bs[TK][TN + 4];
+template <class T>
+__global__ __launch_bounds__(NTH) void k_fb(int W, int G, FbSplits sp, cplx* __restrict__ part,
+                                            int ntm, int ntn) {
+  typedef Mf<T> F;
+  __shared__ T As[TK][TM + 4];
+  __shared__ T Bs[TK][TN + 4];
   const int s = blockIdx.y;
   const int v = xcd_remap(blockIdx.x, ntm * ntn);
   const int tm = v % ntm, tn = v / ntm;
@@ -168,12 +200,12 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
   const int wr = wv >> 1, wc = wv & 1;
   const cplx* __restrict__ Gh = sp.gh[s];
-  const double* __restrict__ Lh = sp.lh[s];
+  const T* __restrict__ Lh = (const T*)sp.lh[s];
   const long K = sp.K[s];
   const int kb = sp.kbeg[s], ke = sp.kend[s];
 
   cplx ra[4];
-  double rb[8];
+  T rb[8];
   auto load = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -185,16 +217,15 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
     for (int i = 0; i < 8; ++i) {
       const int e = t + NTH * i, gl = e >> 4, kk = e & 15;
       const int g = n0 + gl, k = k0 + kk;
-      rb[i] = (g < G && k < ke) ? Lh[(long)g * K + k] : 0.0;
+      rb[i] = (g < G && k < ke) ? Lh[(long)g * K + k] : T(0);
     }
   };
   auto store = [&]() {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = t + NTH * i, wl = e >> 4, kk = e & 15;
-      const int row = 16 * (wl >> 3) + pair_row(wl & 7, 0);
-      As[kk][row] = ra[i].x;
-      As[kk][row + 4] = ra[i].y;
+      As[kk][16 * (wl >> 3) + F::prow(wl & 7, 0)] = (T)ra[i].x;
+      As[kk][16 * (wl >> 3) + F::prow(wl & 7, 1)] = (T)ra[i].y;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -203,11 +234,11 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
     }
   };
 
-  d4v acc[4][4];
+  typename F::V4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) acc[i][j] = typename F::V4{0, 0, 0, 0};
 
   const int nk = (ke - kb + TK - 1) / TK;
   if (nk > 0) {
@@ -220,7 +251,7 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
 #pragma unroll
     for (int ks = 0; ks < TK / 4; ++ks) {
       const int kl = 4 * ks + (l >> 4);
-      double a[4], b[4];
+      T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[kl][64 * wr + 16 * i + (l & 15)];
 #pragma unroll
@@ -228,7 +259,7 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = F::mma(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
     if (kt + 1 < nk) {
@@ -241,12 +272,12 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int tile = 4 * wr + i;
-    const int wA = w0 + 8 * tile + q, wB = wA + 4;
+    const int wA = w0 + 8 * tile + F::wlane(q, 0), wB = w0 + 8 * tile + F::wlane(q, 1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int g = n0 + 64 * wc + 16 * j + (l & 15);
       if (g >= G) continue;
-      const d4v c = acc[i][j];
+      const auto c = acc[i][j];
       if (wA < W) part[((long)s * W + wA) * G + g] = mkc(c[0], c[1]);
       if (wB < W) part[((long)s * W + wB) * G + g] = mkc(c[2], c[3]);
     }
@@ -258,15 +289,19 @@ __global__ __launch_bounds__(NTH) void k_fb_f64(int W, int G, FbSplits sp, cplx*
 // --------------------------------------------------------------------------
 // E == nullptr skips both one-body products (plain _apply_exp_vhs).  The
 // output may alias the input: a workgroup reads and writes only its own
-// columns, reading all of them before writing any.
-template <int MT>
-__global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, int ncb,
-                                                  const double* __restrict__ E,
-                                                  const cplx* __restrict__ V,
-                                                  const cplx* pa_in, const cplx* pb_in,
-                                                  cplx* pa_out, cplx* pb_out, int nmax) {
+// columns, reading all of them before writing any.  With Mf<T>::DFORM the
+// matrix E is expH1 - I and each one-body product is acc = y + E y.
+template <class T, int MT>
+__global__ __launch_bounds__(PTH) void k_prop(int W, int M, int Na, int Nb, int ncb,
+                                              const T* __restrict__ E,
+                                              const typename Mf<T>::C* __restrict__ V,
+                                              const cplx* pa_in, const cplx* pb_in,
+                                              cplx* pa_out, cplx* pb_out, int nmax) {
+  typedef Mf<T> F;
+  typedef typename F::C C;
+  typedef typename F::V4 V4;
   extern __shared__ __align__(16) char smem_raw[];
-  cplx* Ts = (cplx*)smem_raw;  // [Mpad][16]
+  C* Ts = (C*)smem_raw;  // [Mpad][16]
   const int RT = (M + 15) >> 4, Mpad = RT * 16;
   const int vb = xcd_remap(blockIdx.x, W * ncb);
   const int w = vb / ncb, cb = vb % ncb;
@@ -274,7 +309,7 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
   const int li = l & 15, lq = l >> 4;
   const int Nt = Na + Nb;
   const long MM = (long)M * M;
-  const cplx* __restrict__ Vw = V + (long)w * MM;
+  const C* __restrict__ Vw = V + (long)w * MM;
 
   // element offset of (row, column c of this block) in the spin array, or -1
   auto coloff = [&](int c, long row, bool* spin_b) -> long {
@@ -290,36 +325,51 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
     const int q = e >> 4, c = e & 15;
     bool sb = false;
     const long o = (q < M) ? coloff(c, q, &sb) : -1;
-    Ts[e] = (o >= 0) ? (sb ? pb_in[o] : pa_in[o]) : mkc(0.0, 0.0);
+    const cplx x = (o >= 0) ? (sb ? pb_in[o] : pa_in[o]) : mkc(0.0, 0.0);
+    Ts[e] = F::mk(x.x, x.y);
   }
   __syncthreads();
 
-  d4v are[MT], aim[MT], rre[MT], rim[MT];
+  V4 are[MT], aim[MT], rre[MT], rim[MT];
   auto owns = [&](int u) { return wv + PNW * u < RT; };
   auto arow = [&](int u) { return 16 * (wv + PNW * u) + li; };
+  auto orow = [&](int u, int r) { return 16 * (wv + PNW * u) + F::orow(lq, r); };
 
-  // acc = E (real, M x M) . Ts; the A fragments of the next k-step are
-  // loaded before the MFMAs of the current one.
+  // acc = E (real, M x M) . Ts (DFORM: acc = Ts + E . Ts); the A fragments
+  // of the next k-step are loaded before the MFMAs of the current one.
   auto onebody = [&]() {
 #pragma unroll
-    for (int u = 0; u < MT; ++u) { are[u] = d4v{0, 0, 0, 0}; aim[u] = d4v{0, 0, 0, 0}; }
-    double a0[MT], a1[MT];
-    auto ld = [&](double* a, int k0) {
+    for (int u = 0; u < MT; ++u) {
+      if (F::DFORM) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = orow(u, r);
+          const C x = (owns(u) && row < M) ? Ts[row * 16 + li] : F::mk(0.0, 0.0);
+          are[u][r] = x.x;
+          aim[u][r] = x.y;
+        }
+      } else {
+        are[u] = V4{0, 0, 0, 0};
+        aim[u] = V4{0, 0, 0, 0};
+      }
+    }
+    T a0[MT], a1[MT];
+    auto ld = [&](T* a, int k0) {
       const int k = k0 + lq;
 #pragma unroll
       for (int u = 0; u < MT; ++u) {
         const int row = arow(u);
-        a[u] = (row < M && k < M) ? E[(long)row * M + k] : 0.0;
+        a[u] = (row < M && k < M) ? E[(long)row * M + k] : T(0);
       }
     };
-    auto mm = [&](const double* a, int k0) {
-      const cplx b = Ts[(k0 + lq) * 16 + li];
+    auto mm = [&](const T* a, int k0) {
+      const C b = Ts[(k0 + lq) * 16 + li];
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) are[u] = mfma64(a[u], b.x, are[u]);
+        if (owns(u)) are[u] = F::mma(a[u], b.x, are[u]);
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) aim[u] = mfma64(a[u], b.y, aim[u]);
+        if (owns(u)) aim[u] = F::mma(a[u], b.y, aim[u]);
     };
     ld(a0, 0);
     for (int k0 = 0; k0 < Mpad; k0 += 8) {
@@ -332,30 +382,30 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
   // acc = V_w (complex) . Ts, same prefetch scheme
   auto taylor_term = [&]() {
 #pragma unroll
-    for (int u = 0; u < MT; ++u) { are[u] = d4v{0, 0, 0, 0}; aim[u] = d4v{0, 0, 0, 0}; }
-    cplx a0[MT], a1[MT];
-    auto ld = [&](cplx* a, int k0) {
+    for (int u = 0; u < MT; ++u) { are[u] = V4{0, 0, 0, 0}; aim[u] = V4{0, 0, 0, 0}; }
+    C a0[MT], a1[MT];
+    auto ld = [&](C* a, int k0) {
       const int k = k0 + lq;
 #pragma unroll
       for (int u = 0; u < MT; ++u) {
         const int row = arow(u);
-        a[u] = (row < M && k < M) ? Vw[(long)row * M + k] : mkc(0.0, 0.0);
+        a[u] = (row < M && k < M) ? Vw[(long)row * M + k] : F::mk(0.0, 0.0);
       }
     };
-    auto mm = [&](const cplx* a, int k0) {
-      const cplx b = Ts[(k0 + lq) * 16 + li];
+    auto mm = [&](const C* a, int k0) {
+      const C b = Ts[(k0 + lq) * 16 + li];
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) are[u] = mfma64(a[u].x, b.x, are[u]);
+        if (owns(u)) are[u] = F::mma(a[u].x, b.x, are[u]);
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) aim[u] = mfma64(a[u].x, b.y, aim[u]);
+        if (owns(u)) aim[u] = F::mma(a[u].x, b.y, aim[u]);
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) are[u] = mfma64(-a[u].y, b.y, are[u]);
+        if (owns(u)) are[u] = F::mma(-a[u].y, b.y, are[u]);
 #pragma unroll
       for (int u = 0; u < MT; ++u)
-        if (owns(u)) aim[u] = mfma64(a[u].y, b.x, aim[u]);
+        if (owns(u)) aim[u] = F::mma(a[u].y, b.x, aim[u]);
     };
     ld(a0, 0);
     for (int k0 = 0; k0 < Mpad; k0 += 8) {
@@ -366,15 +416,14 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
     }
   };
   // write acc rows (< M) into Ts
-  auto to_lds = [&](const d4v* xr, const d4v* xi) {
+  auto to_lds = [&](const V4* xr, const V4* xi) {
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
       if (!owns(u)) continue;
-      const int tile = wv + PNW * u;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 16 * tile + lq + 4 * r;
-        if (row < M) Ts[row * 16 + li] = mkc(xr[u][r], xi[u][r]);
+        const int row = orow(u, r);
+        if (row < M) Ts[row * 16 + li] = C{xr[u][r], xi[u][r]};
       }
     }
   };
@@ -387,11 +436,10 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
     // the result starts as the input block itself
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
-      const int tile = wv + PNW * u;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 16 * tile + lq + 4 * r;
-        const cplx x = owns(u) ? Ts[row * 16 + li] : mkc(0.0, 0.0);
+        const int row = orow(u, r);
+        const C x = owns(u) ? Ts[row * 16 + li] : F::mk(0.0, 0.0);
         are[u][r] = x.x;
         aim[u][r] = x.y;
       }
@@ -402,7 +450,7 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
   __syncthreads();
   for (int n = 1; n <= nmax; ++n) {
     taylor_term();
-    const double dn = (double)n;
+    const T dn = (T)n;
 #pragma unroll
     for (int u = 0; u < MT; ++u)
 #pragma unroll
@@ -427,10 +475,9 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
 #pragma unroll
   for (int u = 0; u < MT; ++u) {
     if (!owns(u)) continue;
-    const int tile = wv + PNW * u;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = 16 * tile + lq + 4 * r;
+      const int row = orow(u, r);
       bool sb = false;
       const long o = (row < M) ? coloff(li, row, &sb) : -1;
       if (o >= 0) (sb ? pb_out : pa_out)[o] = mkc(are[u][r], aim[u][r]);
@@ -440,28 +487,40 @@ __global__ __launch_bounds__(PTH) void k_prop_f64(int W, int M, int Na, int Nb, 
 
 }  // namespace
 
-int vhs_mfma_f64(hipStream_t s, int W, int M, int G, const double* xs, const double* L, int P,
-                 const int2* colmap, double dt, double* VHS) {
+template <class T>
+static int vhs_launch(hipStream_t s, int W, int M, int G, const double* xs, const T* L, int P,
+                      const int2* colmap, double dt, void* VHS) {
   if (W <= 0) return AFQ_OK;
   const int ntm = cdiv_int(2L * W, TM), ntn = cdiv_int(P, TN);
   const int nb = ntm * ntn;
+  typedef typename Mf<T>::C C;
   if (colmap)
-    hipLaunchKernelGGL((k_vhs_f64<true>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
-                       (const cplx*)xs, L, colmap, sqrt(dt), (cplx*)VHS, ntm, ntn);
+    hipLaunchKernelGGL((k_vhs<T, true>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
+                       (const cplx*)xs, L, colmap, sqrt(dt), (C*)VHS, ntm, ntn);
   else
-    hipLaunchKernelGGL((k_vhs_f64<false>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
-                       (const cplx*)xs, L, colmap, sqrt(dt), (cplx*)VHS, ntm, ntn);
+    hipLaunchKernelGGL((k_vhs<T, false>), dim3(nb), dim3(NTH), 0, s, W, G, P, (long)M * M,
+                       (const cplx*)xs, L, colmap, sqrt(dt), (C*)VHS, ntm, ntn);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
 
-FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, const double* Lha,
-                   const double* Lhb, int per_spin) {
+int vhs_mfma_f64(hipStream_t s, int W, int M, int G, const double* xs, const double* L, int P,
+                 const int2* colmap, double dt, double* VHS) {
+  return vhs_launch<double>(s, W, M, G, xs, L, P, colmap, dt, VHS);
+}
+
+int vhs_mfma_f32(hipStream_t s, int W, int M, int G, const double* xs, const float* L, int P,
+                 const int2* colmap, double dt, float* VHS) {
+  return vhs_launch<float>(s, W, M, G, xs, L, P, colmap, dt, VHS);
+}
+
+FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, const void* Lha,
+                   const void* Lhb, int per_spin) {
   FbSplits sp{};
   int n = 0;
   const int Ns[2] = {Na, Nb};
   const double* gh[2] = {Gha, Ghb};
-  const double* lh[2] = {Lha, Lhb};
+  const void* lh[2] = {Lha, Lhb};
   for (int s = 0; s < 2; ++s) {
     const long K = (long)Ns[s] * M;
     if (K == 0) continue;
@@ -479,32 +538,43 @@ FbSplits fb_splits(int M, int Na, int Nb, const double* Gha, const double* Ghb, 
   return sp;
 }
 
-int fb_mfma_f64(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
+template <class T>
+static int fb_launch(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
   if (W <= 0 || G <= 0 || sp.n == 0) return AFQ_OK;
   const int ntm = cdiv_int(2L * W, TM), ntn = cdiv_int(G, TN);
-  hipLaunchKernelGGL(k_fb_f64, dim3(ntm * ntn, sp.n), dim3(NTH), 0, s, W, G, sp, (cplx*)part, ntm,
-                     ntn);
+  hipLaunchKernelGGL(k_fb<T>, dim3(ntm * ntn, sp.n), dim3(NTH), 0, s, W, G, sp, (cplx*)part,
+                     ntm, ntn);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
 
+int fb_mfma_f64(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
+  return fb_launch<double>(s, W, G, sp, part);
+}
+
+int fb_mfma_f32(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
+  return fb_launch<float>(s, W, G, sp, part);
+}
+
 size_t prop_f64_smem(int M) { return (size_t)((M + 15) / 16) * 16 * 16 * sizeof(cplx); }
 
-int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
-                  const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
-                  double* pb_out, int nmax) {
+template <class T>
+static int prop_launch(hipStream_t s, int W, int M, int Na, int Nb, const T* E, const void* VHS,
+                       const double* pa_in, const double* pb_in, double* pa_out, double* pb_out,
+                       int nmax) {
   const int Nt = Na + Nb;
   if (W <= 0 || Nt == 0) return AFQ_OK;
+  typedef typename Mf<T>::C C;
   const int RT = (M + 15) / 16;
   const int ncb = (Nt + 15) / 16;
-  const size_t sm = prop_f64_smem(M);
+  const size_t sm = (size_t)RT * 16 * 16 * sizeof(C);
   if (sm > max_lds_bytes())
     return fail(AFQ_EUNSUPPORTED, "fused propagator: M=%d needs %zu bytes of LDS", M, sm);
   const int nb = W * ncb;
   const dim3 grid(nb), blk(PTH);
-#define AFQ_LAUNCH_PROP(MT)                                                                   \
-  hipLaunchKernelGGL((k_prop_f64<MT>), grid, blk, sm, s, W, M, Na, Nb, ncb, expH1,           \
-                     (const cplx*)VHS, (const cplx*)pa_in, (const cplx*)pb_in, (cplx*)pa_out, \
+#define AFQ_LAUNCH_PROP(MT)                                                                 \
+  hipLaunchKernelGGL((k_prop<T, MT>), grid, blk, sm, s, W, M, Na, Nb, ncb, E,              \
+                     (const C*)VHS, (const cplx*)pa_in, (const cplx*)pb_in, (cplx*)pa_out, \
                      (cplx*)pb_out, nmax)
   if (RT <= PNW) AFQ_LAUNCH_PROP(1);
   else if (RT <= 2 * PNW) AFQ_LAUNCH_PROP(2);
@@ -514,6 +584,19 @@ int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* exp
 #undef AFQ_LAUNCH_PROP
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
+}
+
+int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
+                  const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
+                  double* pb_out, int nmax) {
+  return prop_launch<double>(s, W, M, Na, Nb, expH1, VHS, pa_in, pb_in, pa_out, pb_out, nmax);
+}
+
+int prop_mfma_f32(hipStream_t s, int W, int M, int Na, int Nb, const float* expH1_minus_I,
+                  const float* VHS, const double* pa_in, const double* pb_in, double* pa_out,
+                  double* pb_out, int nmax) {
+  return prop_launch<float>(s, W, M, Na, Nb, expH1_minus_I, VHS, pa_in, pb_in, pa_out, pb_out,
+                            nmax);
 }
 
 }  // namespace afq

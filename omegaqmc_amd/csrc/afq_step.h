@@ -28,6 +28,20 @@ int energy_prologue(afq_system* sys, int W, const double* phia, const double* ph
                     const double* h1e, double enuc, const GhalfBf16* gbf, double* e_1b,
                     EnergySlots* es);
 
+// The FP64 step workspace's slots (sized for W walkers; the VHS slot holds
+// W M M complex128 values, enough for any precision's VHS).
+struct StepSlots {
+  double *gha, *ghb, *oo, *vb, *xs, *cfb, *cmf, *xi, *vhs, *inv;
+};
+int step_slots(afq_system* sys, int W, StepSlots* out);
+
+// FP32-MFMA path (afq_fp32.hip)
+int f32_prepare(afq_system* sys);
+void f32_release(afq_system* sys);
+int propagate_fp32(afq_system* sys, int W, double* phia, double* phib, double* weights,
+                   double* e_hybrid, double eshift, const double* xi, uint64_t seed, int64_t step,
+                   double* ovlp_new);
+
 // reduced-precision path (afq_lowp.hip)
 int system_prepare_lowp(afq_system* sys);
 void system_release_lowp(afq_system* sys);

@@ -218,6 +218,22 @@ double* work_slot(afq_system* s, int W, int slot) {
   return (double*)((char*)s->work + L.off[slot]);
 }
 
+int step_slots(afq_system* sys, int W, StepSlots* o) {
+  int rc = ensure_work(sys, W);
+  if (rc) return rc;
+  o->gha = work_slot(sys, W, GHA);
+  o->ghb = work_slot(sys, W, GHB);
+  o->oo = work_slot(sys, W, OVLP_OLD);
+  o->vb = work_slot(sys, W, VBIAS);
+  o->xs = work_slot(sys, W, XS);
+  o->cfb = work_slot(sys, W, CFB);
+  o->cmf = work_slot(sys, W, CMF);
+  o->xi = work_slot(sys, W, XI);
+  o->vhs = work_slot(sys, W, VHS);
+  o->inv = work_slot(sys, W, INV);
+  return AFQ_OK;
+}
+
 int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* weights,
                    double* e_hybrid, double eshift, const double* xi_in, uint64_t seed,
                    int64_t step, double* ovlp_new) {
@@ -468,6 +484,9 @@ int afq_propagate_walkers(afq_system* sys, int W, double* phia, double* phib, do
   if (sys->precision == AFQ_FP64)
     return afq::propagate_fp64(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
                                ovlp_new);
+  if (sys->precision == AFQ_FP32)
+    return afq::propagate_fp32(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
+                               ovlp_new);
   return afq::propagate_lowp(sys, W, phia, phib, weights, e_hybrid, eshift, xi, seed, step,
                              ovlp_new);
 }
@@ -481,7 +500,7 @@ int afq_system_local_energy(afq_system* sys, int W, const double* phia, const do
   if (W == 0) return AFQ_OK;
   if (sys->ndet > 0)
     return afq::system_energy_multidet(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
-  if (sys->precision == AFQ_FP64)
+  if (sys->precision == AFQ_FP64 || sys->precision == AFQ_FP32)
     return afq::system_energy_fp64(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
   return afq::system_energy_lowp(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
 }
