@@ -352,22 +352,31 @@ constexpr int PW = 8, PT = 64 * PW, PC = 48;
 // products of the first stage.
 constexpr int TAYLOR_BF16_FROM = 5;
 
+// A walker's columns are the concatenation of its two spins (column cg < Na
+// of phia, then Nb of phib), cut into blocks of PC = 48: with N = 40 + 40 the
+// blocks hold 48 and 32 columns, with 16 + 16 one block of 32.  A block's
+// 16-column tiles beyond the last column (x >= nxl) are skipped, a uniform
+// branch per workgroup.
 template <int MT>
 __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, int Na, int Nb,
-                                                 int ncba, int ncbb, const bf16_t* __restrict__ E,
+                                                 int nblk, const bf16_t* __restrict__ E,
                                                  const bf16_t* __restrict__ Vb,
                                                  float4* __restrict__ t0s, double* phia,
                                                  double* phib, int nmax) {
   extern __shared__ __align__(16) bf16_t Ts[];  // [4][PC][Kp + 8]
   const int KR = Kp + 8;
-  const int nb = W * (ncba + ncbb);
-  const int vb = xcd_remap(blockIdx.x, nb);
-  const int per = ncba + ncbb;
-  const int w = vb / per, cbx = vb % per;
-  const bool spin_b = cbx >= ncba;
-  const int N = spin_b ? Nb : Na;
-  const int c0 = (spin_b ? cbx - ncba : cbx) * PC;
-  double* phi = spin_b ? phib : phia;
+  const int vb = xcd_remap(blockIdx.x, W * nblk);
+  const int w = vb / nblk, cbx = vb % nblk;
+  const int c0 = cbx * PC;
+  const int Nt = Na + Nb;
+  const int nxl = min(3, (Nt - c0 + 15) >> 4);  // live 16-column tiles (uniform)
+  // (spin array, column, row stride) of global column cg; nullptr past the last
+  auto colmap = [&](int cg, int* col, int* ld) -> double* {
+    if (cg < Na) { *col = cg; *ld = Na; return phia; }
+    if (cg < Nt) { *col = cg - Na; *ld = Nb; return phib; }
+    *col = 0; *ld = 1;
+    return nullptr;
+  };
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
   const int li = l & 15, lq = l >> 4;
   const long MM = (long)M * M;
@@ -385,19 +394,20 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     typedef double d2v __attribute__((ext_vector_type(2)));
     constexpr int SB = 5;
     const int np = PC * (Kp / 2);
-    const d2v* pw = (const d2v*)phi + (long)w * M * N;
     for (int e0 = t; e0 < np; e0 += PT * SB) {
       d2v v[SB][2];
 #pragma unroll
       for (int k = 0; k < SB; ++k) {
         const int e = e0 + PT * k;
         const int c = e % PC, rp = e / PC;
-        const int cc = min(c0 + c, N - 1);
+        int col, ld;
+        const double* base = colmap(min(c0 + c, Nt - 1), &col, &ld);
+        const d2v* pw = (const d2v*)base + (long)w * M * ld + col;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = 2 * rp + h;
-          const d2v x = pw[(long)min(r, M - 1) * N + cc];
-          const bool ok = e < np && r < M && c0 + c < N;
+          const d2v x = pw[(long)min(r, M - 1) * ld];
+          const bool ok = e < np && r < M && c0 + c < Nt;
           v[k][h] = ok ? x : d2v{0.0, 0.0};
         }
       }
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   __syncthreads();
 
   f4v are[MT][3], aim[MT][3];
-  float4* T0 = t0s + (long)(w * per + cbx) * RT * 3 * 64 * 2;
+  float4* T0 = t0s + (long)(w * nblk + cbx) * RT * 3 * 64 * 2;
   auto t0_at = [&](int u, int x) { return T0 + ((long)((wv + PW * u) * 3 + x) * 64 + l) * 2; };
   auto owns = [&](int u) { return wv + PW * u < RT; };
   auto zero = [&]() {
@@ -431,6 +441,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       for (int x = 0; x < 3; ++x) { are[u][x] = f4v{0, 0, 0, 0}; aim[u][x] = f4v{0, 0, 0, 0}; }
   };
   auto bfrag = [&](int x, int ks, uint4* b) {
+    if (x >= nxl) return;
 #pragma unroll
     for (int p = 0; p < 4; ++p) b[p] = *(const uint4*)(T(p, 16 * x + li) + 32 * ks + 8 * lq);
   };
@@ -453,6 +464,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   auto estep = [&](const uint4* eh, const uint4* el, int ks) {
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
+      if (x >= nxl) continue;
       uint4 b[4];
       bfrag(x, ks, b);
 #pragma unroll
@@ -471,12 +483,13 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       const int r0 = 16 * min(wv + PW * u, RT - 1) + 4 * lq;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        const int c = c0 + 16 * x + li;
+        int col, ld;
+        const double* base = colmap(c0 + 16 * x + li, &col, &ld);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const bool ok = owns(u) && r0 + r < M && c < N;
-          const long i = ok ? ((long)w * M + r0 + r) * N + c : 0;
-          const double2 z = ok ? *(const double2*)(phi + 2 * i) : make_double2(0.0, 0.0);
+          const bool ok = owns(u) && r0 + r < M && base != nullptr;
+          const long i = ok ? ((long)w * M + r0 + r) * ld + col : 0;
+          const double2 z = ok ? *(const double2*)(base + 2 * i) : make_double2(0.0, 0.0);
           are[u][x][r] = (float)z.x;
           aim[u][x][r] = (float)z.y;
         }
@@ -519,6 +532,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     v[2] = gld(blk + 512);
   };
   auto bfrag_hi = [&](int x, int ks, uint4* b) {
+    if (x >= nxl) return;
     b[0] = *(const uint4*)(T(0, 16 * x + li) + 32 * ks + 8 * lq);
     b[2] = *(const uint4*)(T(2, 16 * x + li) + 32 * ks + 8 * lq);
   };
@@ -527,6 +541,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       const uint4 ni = neg8(vc[2]);
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
+        if (x >= nxl) continue;
         are[u][x] = mfma_bf(vc[0], b[x][0], are[u][x]);
         are[u][x] = mfma_bf(ni, b[x][2], are[u][x]);
         aim[u][x] = mfma_bf(vc[0], b[x][2], aim[u][x]);
@@ -539,6 +554,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
       const uint4 ni = neg8(vc[2]), nl = neg8(vc[3]);
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
+        if (x >= nxl) continue;
         are[u][x] = mfma3(vc[0], vc[1], b[x][0], b[x][1], are[u][x]);
         are[u][x] = mfma3(ni, nl, b[x][2], b[x][3], are[u][x]);
         aim[u][x] = mfma3(vc[0], vc[1], b[x][2], b[x][3], aim[u][x]);
@@ -666,12 +682,13 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     const int r0 = 16 * (wv + PW * u) + 4 * lq;
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      const int c = c0 + 16 * x + li;
-      if (c >= N) continue;
+      int col, ld;
+      double* base = colmap(c0 + 16 * x + li, &col, &ld);
+      if (!base) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (r0 + r >= M) continue;
-        double* z = phi + 2 * (((long)w * M + r0 + r) * N + c);
+        double* z = base + 2 * (((long)w * M + r0 + r) * ld + col);
         *(double2*)z = make_double2((double)are[u][x][r], (double)aim[u][x][r]);
       }
     }
@@ -889,22 +906,22 @@ int bf3_packed_force_bias(hipStream_t s, int W, int G, int Kp, const bf16_t* gpl
 }
 
 size_t bf3_prop_scratch_bytes(const Bf3System& b, int W) {
-  const long ncb = (b.Na + PC - 1) / PC + (b.Nb + PC - 1) / PC;
+  const long ncb = (b.Na + b.Nb + PC - 1) / PC;
   return (size_t)W * ncb * b.RT * 3 * 64 * 2 * sizeof(float4);
 }
 
 int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, float* t0,
                   double* phia, double* phib, int nmax) {
-  const int ncba = (b.Na + PC - 1) / PC, ncbb = (b.Nb + PC - 1) / PC;
-  if (W <= 0 || ncba + ncbb == 0) return AFQ_OK;
+  const int nblk = (b.Na + b.Nb + PC - 1) / PC;
+  if (W <= 0 || nblk == 0) return AFQ_OK;
   const size_t sm = (size_t)4 * PC * (b.Kp + 8) * sizeof(bf16_t);
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: LDS %zu", sm);
   if (b.RT > 4 * PW) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: M=%d", b.M);
   if (!t0) return fail(AFQ_EINVAL, "bf16x3 propagator: no scratch");
-  const dim3 grid(W * (ncba + ncbb)), blk(PT);
+  const dim3 grid(W * nblk), blk(PT);
 #define AFQ_PROP_BF3(MT)                                                                      \
-  hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, ncba, \
-                     ncbb, b.E, Vb, (float4*)t0, phia, phib, nmax)
+  hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, nblk, \
+                     b.E, Vb, (float4*)t0, phia, phib, nmax)
   if (b.RT <= PW) AFQ_PROP_BF3(1);
   else if (b.RT <= 2 * PW) AFQ_PROP_BF3(2);
   else if (b.RT <= 3 * PW) AFQ_PROP_BF3(3);
