@@ -309,6 +309,81 @@ def run_multidet(s, trial, W, steps, warmup, precision, seed=42):
             "phases_ms_per_step": phases, "e_block": float(st['stats'][0].item())}
 
 
+def run_uneven_batch(steps=50, nsys=64, precision="bf16x3", seed=7, tune_cache=None):
+    """64 independent synthetic systems (M ~ U{40..200}, N = M // 8 per spin,
+    Nchol = 5 M, walkers from {128, 256, 512, 1024}), ``steps`` driver steps
+    each: the per-system Python driver loop (one driver after another, as the
+    reference runs them), the batch on one stream, on eager streams, and as
+    hipGraph cycles with the autotuned tiles and stream assignment.
+    Aggregate walker-steps/s = sum_i W_i x steps / wall time."""
+    import torch
+    from omegaqmc_amd.batch import Batch, autotune
+    from omegaqmc_amd.driver import AFQMCDriver
+    from omegaqmc_amd.synth import make_system, uneven_batch_specs
+    specs = uneven_batch_specs(nsys, seed=seed)
+    dev = torch.device("cuda", 0)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    drivers = []
+    for k, sp in enumerate(specs):
+        s = make_system(sp['nbasis'], sp['nup'], sp['ndown'], sp['naux'], seed=1000 + k,
+                        rotate_device="cuda", normalise=True)
+        integ = {'h1e': t(s.h1e), 'h1e_mod': t(s.h1e_mod), 'chol': t(s.chol), 'enuc': s.enuc,
+                 'nbasis': s.nbasis, 'nup': s.nup, 'ndown': s.ndown}
+        drivers.append(AFQMCDriver(integ, verbose=False, precision=precision))
+        del s
+    W = [sp['walkers'] for sp in specs]
+    seeds = [seed + k for k in range(nsys)]
+    eshift = [d.ehf - d.e_const for d in drivers]
+    systems = [d.system for d in drivers]
+    total_w = float(sum(W))
+    out = {"workload": f"{nsys} systems, M~U[40,200], N=M/8 per spin, Nchol=5M, walkers from "
+                       f"{{128,256,512,1024}} (total {int(total_w)}), {steps} steps each, "
+                       f"{precision}, size-normalised L_g",
+           "M": [sp['nbasis'] for sp in specs], "walkers": W}
+
+    def timed(fn, nsteps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return {"wall_s": round(dt, 4), "walker_steps_per_s": round(total_w * nsteps / dt, 1),
+                "ms_per_step": round(dt / nsteps * 1e3, 4)}
+
+    # the reference's way: one driver per system, stepped one after another
+    states = [d.init_walkers(w) for d, w in zip(drivers, W)]
+
+    def drivers_loop(n0, n):
+        for step in range(n0 + 1, n0 + n + 1):
+            for d, st, sd, es in zip(drivers, states, seeds, eshift):
+                d.step(st, step, sd, es, 5, 10)
+    drivers_loop(0, 10)
+    out["per_system_driver"] = timed(lambda: drivers_loop(10, steps), steps)
+    del states
+    torch.cuda.empty_cache()
+
+    print("[bench] uneven batch: autotune", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tune = autotune(systems, W, [precision] * nsys, cache=tune_cache or os.path.join(
+        ROOT, "gpurun_out", "tune.json"), nsteps=10)
+    out["autotune"] = {"seconds": round(time.perf_counter() - t0, 1), "nstreams": tune["nstreams"],
+                       "fb_splits": tune["fb_splits"]}
+    for mode, ns, so in (("sequential", 1, None), ("streams", tune["nstreams"], tune["stream_of"]),
+                         ("graph", tune["nstreams"], tune["stream_of"])):
+        print(f"[bench] uneven batch: {mode}", file=sys.stderr, flush=True)
+        b = Batch(systems, W, seeds=seeds, nstreams=ns, stream_of=so)
+        b.set_eshift(eshift)
+        b.run(20, mode)  # warm-up: sizes workspaces (and captures the graphs)
+        out[f"batch_{mode}"] = timed(lambda: b.run(steps, mode), steps)
+        del b
+        torch.cuda.empty_cache()
+    out["graph_speedup_vs_per_system_driver"] = round(
+        out["batch_graph"]["walker_steps_per_s"] / out["per_system_driver"]["walker_steps_per_s"], 3)
+    del drivers, systems
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling HBM byte counters."""
     import torch
@@ -359,6 +434,8 @@ def main():
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--no-multidet", action="store_true",
                     help="skip the 32-determinant M=150 workload")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-system uneven batch")
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the M=120 FP64 / FP32 / BF16x3 workload")
     ap.add_argument("--acc-steps", type=int, default=100,
@@ -455,6 +532,10 @@ def main():
     bw["note"] = ("W=1024 arrays of a few KB: launch-latency bound, far below the 8000 GB/s "
                   "HBM roof; the comb gather moves the 393 MB walker set")
     extra["bandwidth_kernels"] = bw
+
+    if not args.no_batch:
+        print("[bench] uneven batch of 64 systems", file=sys.stderr, flush=True)
+        extra["uneven_batch"] = run_uneven_batch(steps=50)
 
     if not args.no_config2:
         # the FP64 vs FP32-MFMA workload (the BF16x3 step beside them)

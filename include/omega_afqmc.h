@@ -285,6 +285,11 @@ int afq_system_destroy(afq_system* sys);
  * BF16 VHS assumes symmetric L_g). */
 int afq_system_set_chol_sign(afq_system* sys, const double* chol_sign);
 
+/* K-splits of the force-bias contraction (no counterpart in the reference,
+ * which contracts it in one einsum): the tile choice of afq_batch's
+ * autotuner.  nsplit in {2, 4, 8} (BF16X3: {2, 4}); default 4. */
+int afq_system_set_force_bias_splits(afq_system* sys, int nsplit);
+
 /* afqmc_gto.py:630-725 propagate_walkers(...) for one step, in place on
  * phia/phib/weights/e_hybrid; overlap_new (W,) complex is written.  The
  * reference ignores its `overlap` argument, so there is none here.  xi (W,G)
@@ -311,6 +316,46 @@ int afq_system_local_energy(afq_system* sys, int nwalkers, const double* phia,
 int afq_system_timing_enable(afq_system* sys, int on);
 int afq_system_timing_read(afq_system* sys, double* total_ms, int64_t* launches, int nphase,
                            int reset);
+
+/* ---- uneven batches of independent systems ----------------------------- */
+/* The driver step of afqmc_gto.py:1383-1451 (QR every stabilize_freq steps
+ * before propagating, the fused step, weight clip from step 2 on, comb every
+ * pop_control_freq steps, hybrid-energy accumulators), run for many
+ * single-determinant systems of any sizes, precisions and walker counts at
+ * once.  The reference runs one system per driver (afqmc_gto.py:1257-1546);
+ * here every system keeps its own walkers (allocated by the batch and
+ * initialised to its trial, afqmc_gto.py:76-123), RNG seed (fields from the
+ * Philox stream, comb offsets from afq_comb_offset) and energy shift, and
+ * the systems share HIP streams: system i runs on stream stream_of[i] (NULL:
+ * i mod nstreams) of the batch's nstreams (1..16).  While a batch lives, its
+ * systems must not be stepped elsewhere with more walkers (the captured
+ * steps point into their workspaces). */
+typedef struct afq_batch afq_batch;
+int afq_batch_create(afq_ctx* ctx, afq_batch** out, int nsys, afq_system* const* systems,
+                     const int* nwalkers, const uint64_t* seeds, int stabilize_freq,
+                     int pop_control_freq, int nstreams, const int* stream_of);
+/* Energy shift of each system (nsys host doubles, the hybrid-energy level of
+ * afqmc_gto.py:1357-1359, 1497-1502); read by the steps from device memory. */
+int afq_batch_set_eshift(afq_batch* b, const double* eshift);
+/* Advance every system by nsteps driver steps, ordered after the work
+ * already on the context's stream (which waits for the batch in turn).
+ * mode 0: the systems one after another on one stream (one driver at a
+ * time, as the reference); 1: eager launches on the batch's streams; 2: one
+ * hipGraph per driver cycle (lcm of the two frequencies) of all systems,
+ * captured after one eager cycle, partial cycles eager.  All modes give the
+ * same results. */
+int afq_batch_run(afq_batch* b, int nsteps, int mode);
+/* Device pointers of system i's current walker state (any may be NULL):
+ * phia (W,M,Na), phib (W,M,Nb), weights (W,), e_hybrid (W,), overlap (W,)
+ * of the last step, acc[2] the accumulators of afq_accumulate_hybrid; and
+ * the number of steps run. */
+int afq_batch_state(afq_batch* b, int i, double** phia, double** phib, double** weights,
+                    double** e_hybrid, double** overlap, double** acc, int64_t* step);
+/* Copy system i's current state into caller buffers (any may be NULL), on
+ * the context's stream. */
+int afq_batch_copy_state(afq_batch* b, int i, double* phia, double* phib, double* weights,
+                         double* e_hybrid, double* overlap, double* acc);
+int afq_batch_destroy(afq_batch* b);
 
 #ifdef __cplusplus
 }

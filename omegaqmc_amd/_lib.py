@@ -72,6 +72,13 @@ SIGNATURES = {
                                    _P, _P, _D, _D, _I, _I, _I],
     "afq_propagate_walkers": [_P, _I, _P, _P, _P, _P, _D, _P, _U64, _I64, _P],
     "afq_system_local_energy": [_P, _I, _P, _P, _P, _D, _P, _P, _P],
+    "afq_batch_create": [_P, ctypes.POINTER(_P), _I, _P, _P, _P, _I, _I, _I, _P],
+    "afq_batch_set_eshift": [_P, _P],
+    "afq_batch_run": [_P, _I, _I],
+    "afq_batch_state": [_P, _I, _P, _P, _P, _P, _P, _P, _P],
+    "afq_batch_copy_state": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "afq_batch_destroy": [_P],
+    "afq_system_set_force_bias_splits": [_P, _I],
     "afq_system_timing_enable": [_P, _I],
     "afq_system_timing_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64), _I, _I],
 }

@@ -33,8 +33,9 @@ void bf3_release(Bf3System* b);
 // (re_hi, re_lo, im_hi, im_lo), both (P,Q) and (Q,P) blocks written.
 int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt, bf16_t* Vb);
 // Ghalf planes [2][2W][K_s] for both spins -> vbias partials (float2) part[s][W][G].
+// per_spin K splits per spin (1 or 2: at most four FP32 partial sets)
 int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, const bf16_t* Ghb,
-                   float* part, int* nsplit_out);
+                   float* part, int* nsplit_out, int per_spin = 2);
 // phi <- expH1 . Taylor(VHS) . expH1 . phi in place, FP64 walkers; t0 is a
 // scratch of bf3_prop_scratch_bytes(b, W).
 size_t bf3_prop_scratch_bytes(const Bf3System& b, int W);

@@ -810,10 +810,10 @@ int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt
 }
 
 int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, const bf16_t* Ghb,
-                   float* part, int* nsplit_out) {
+                   float* part, int* nsplit_out, int per_spin) {
   Bf3Gemm a{};
   int n = 0;
-  const int per_spin = 2;
+  per_spin = std::min(2, std::max(1, per_spin));
   const int Ks[2] = {b.Ka, b.Kb};
   const int Nrow[2] = {b.Na, b.Nb};
   const bf16_t* gh[2] = {Gha, Ghb};

@@ -37,11 +37,13 @@ namespace {
 
 constexpr int NT = 1024;
 
-__global__ __launch_bounds__(NT) void k_comb_index(int W, const double* __restrict__ w, double r,
+__global__ __launch_bounds__(NT) void k_comb_index(int W, const double* __restrict__ w, double r_arg,
+                                                   const double* __restrict__ r_dev,
                                                    double* __restrict__ cum_g,
                                                    int32_t* __restrict__ idx,
                                                    double* __restrict__ w_new) {
   extern __shared__ double cum_s[];
+  const double r = r_dev ? *r_dev : r_arg;
   const bool in_lds = (cum_g == nullptr);
   double* cum = in_lds ? cum_s : cum_g;
   for (int i = threadIdx.x; i < W; i += NT) cum[i] = w[i];
@@ -98,17 +100,20 @@ __global__ __launch_bounds__(256) void k_comb_gather(int W, long na, long nb,
 }
 
 __global__ __launch_bounds__(1024) void k_sum_abs(int W, const double* __restrict__ w,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ out,
+                                                  const int64_t* __restrict__ gate) {
   __shared__ double red[32];
+  if (gate && *gate <= 1) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < W; i += 1024) s += fabs(w[i]);
   s = block_sum(s, red);
   if (threadIdx.x == 0) *out = s;
 }
 
-__global__ void k_clip(int W, double* __restrict__ w, const double* __restrict__ total, double frac) {
+__global__ void k_clip(int W, double* __restrict__ w, const double* __restrict__ total, double frac,
+                       const int64_t* __restrict__ gate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W) return;
+  if (i >= W || (gate && *gate <= 1)) return;
   const double b = *total * frac;
   w[i] = jmin(jmax(w[i], 0.0), b);
 }
@@ -184,13 +189,14 @@ int gather_launch(hipStream_t s, int W, int M, int Na, int Nb, const int32_t* id
 int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
                 const double* phia, const double* phib, double r, const double* eh,
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
-                double* cum_scratch) {
+                double* cum_scratch, const double* r_dev) {
   if (W <= 0) return AFQ_OK;
   const bool lds = sizeof(double) * (size_t)W + 1024 <= max_lds_bytes();
   if (!lds && !cum_scratch) return fail(AFQ_EINVAL, "comb: W=%d needs a scratch buffer", W);
   const size_t sm = lds ? sizeof(double) * W : 0;
   if (lds) cum_scratch = nullptr;
-  hipLaunchKernelGGL(k_comb_index, dim3(1), dim3(NT), sm, s, W, weights, r, cum_scratch, idx, w_new);
+  hipLaunchKernelGGL(k_comb_index, dim3(1), dim3(NT), sm, s, W, weights, r, r_dev, cum_scratch, idx,
+                     w_new);
   AFQ_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_comb_gather, dim3(W), dim3(256), 0, s, W, (long)M * Na, (long)M * Nb, idx,
                      (const double4*)phia, (const double4*)phib, (const cplx*)eh,
@@ -199,11 +205,12 @@ int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weigh
   return AFQ_OK;
 }
 
-int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp) {
+int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp,
+                        const int64_t* gate) {
   if (W <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(1024), 0, s, W, weights, tmp);
+  hipLaunchKernelGGL(k_sum_abs, dim3(1), dim3(1024), 0, s, W, weights, tmp, gate);
   AFQ_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_clip, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, weights, tmp, frac);
+  hipLaunchKernelGGL(k_clip, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, weights, tmp, frac, gate);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

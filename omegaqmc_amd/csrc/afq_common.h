@@ -140,6 +140,11 @@ struct afq_system {
   int md_cap = 0;
   void* md_work = nullptr;
   size_t md_bytes = 0;
+  // device-resident step inputs of a captured step (afq_batch): when set,
+  // the auxiliary fields use the step number *step_dev and the weight update
+  // the energy shift *eshift_dev instead of the call's arguments
+  const int64_t* step_dev = nullptr;
+  const double* eshift_dev = nullptr;
   // split-BF16 planes of chol_packed for the BF16x3 multi-det force bias
   // ([2][G][md_kp], built on first use, owned)
   bf16_t* md_lpk = nullptr;

@@ -92,7 +92,7 @@ int propagate_fp32(afq_system* sys, int W, double* phia, double* phib, double* w
   const double* xi = xi_in;
   if (!xi) {
     phase_begin(sys, PH_RNG);
-    rc = aux_fields_launch(s, seed, step, W, G, w.xi);
+    rc = aux_fields_launch(s, seed, step, W, G, w.xi, sys->step_dev);
     phase_end(sys, PH_RNG);
     if (rc) return rc;
     xi = w.xi;
@@ -128,7 +128,7 @@ int propagate_fp32(afq_system* sys, int W, double* phia, double* phib, double* w
   phase_end(sys, PH_OVERLAP);
   phase_begin(sys, PH_WEIGHTS);
   rc = weights_launch(s, W, weights, w.oo, ovlp_new, w.cfb, w.cmf, e_hybrid, eshift, sys->dt,
-                      weights, e_hybrid);
+                      weights, e_hybrid, sys->eshift_dev);
   phase_end(sys, PH_WEIGHTS);
   return rc;
 }

@@ -29,25 +29,34 @@ int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, co
                        int nmax, double* phi_out, double* tmp0, double* tmp1);
 int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const double* phi,
                   double* out);
+// eshift_dev (optional): device energy shift used instead of eshift
 int weights_launch(hipStream_t s, int W, const double* w, const double* oo, const double* on,
                    const double* cfb, const double* cmf, const double* eh, double eshift, double dt,
-                   double* w_out, double* eh_out);
+                   double* w_out, double* eh_out, const double* eshift_dev = nullptr);
 
 // rng (afq_rng.hip)
-int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, double* xi);
+// step_dev (optional): device step number used instead of step
+int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, double* xi,
+                      const int64_t* step_dev = nullptr);
+// r[i] = comb offset of seeds[i] at the step *step_dev
+int comb_offsets_dev_launch(hipStream_t s, int n, const uint64_t* seeds, const int64_t* step_dev,
+                            double* r);
 
 // stabilisation / population control (afq_ortho.hip, afq_comb.hip)
 int ortho_launch(hipStream_t s, int W, int M, int Na, int Nb, double* phia, double* phib,
                  double* log_detR, double* sign_detR, void* scratch);
 size_t ortho_scratch_bytes(int W, int Na, int Nb);
+// r_dev (optional): device comb offset used instead of r
 int comb_launch(hipStream_t s, int W, int M, int Na, int Nb, const double* weights,
                 const double* phia, const double* phib, double r, const double* eh,
                 double* w_new, double* phia_new, double* phib_new, double* eh_new, int32_t* idx,
-                double* cum_scratch);
+                double* cum_scratch, const double* r_dev = nullptr);
 int gather_launch(hipStream_t s, int W, int M, int Na, int Nb, const int32_t* idx,
                   const double* phia, const double* phib, const double* eh, double* w_new,
                   double* phia_new, double* phib_new, double* eh_new);
-int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp);
+// gate (optional): no clip while *gate <= 1 (the first driver step)
+int clip_weights_launch(hipStream_t s, int W, double* weights, double frac, double* tmp,
+                        const int64_t* gate = nullptr);
 int block_stats_launch(hipStream_t s, int W, const double* weights, const double* e, double* out);
 int accumulate_launch(hipStream_t s, int W, const double* weights, const double* eh, double* acc);
 

@@ -143,7 +143,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   if (!xi) {
     double* xg = (double*)(base + L->xi);
     phase_begin(sys, PH_RNG);
-    rc = aux_fields_launch(s, seed, step, W, G, xg);
+    rc = aux_fields_launch(s, seed, step, W, G, xg, sys->step_dev);
     phase_end(sys, PH_RNG);
     if (rc) return rc;
     xi = xg;
@@ -158,7 +158,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   // 3: force bias (split-K, FP32 partials), shift in FP64
   int nsplit = 0;
   phase_begin(sys, PH_FORCEBIAS);
-  rc = bf3_force_bias(s, b, W, gha, ghb, part, &nsplit);
+  rc = bf3_force_bias(s, b, W, gha, ghb, part, &nsplit, sys->fb_nsplit / 2);
   phase_end(sys, PH_FORCEBIAS);
   if (rc) return rc;
   if (nsplit > FB_SPLITS_MAX) return fail(AFQ_EUNSUPPORTED, "force-bias splits %d", nsplit);
@@ -189,7 +189,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   if (rc) return rc;
   phase_begin(sys, PH_WEIGHTS);
   rc = weights_launch(s, W, weights, oo, ovlp_new, cfb, cmf, e_hybrid, eshift, sys->dt, weights,
-                      e_hybrid);
+                      e_hybrid, sys->eshift_dev);
   phase_end(sys, PH_WEIGHTS);
   return rc;
 }
@@ -216,7 +216,7 @@ int system_energy_lowp(afq_system* sys, int W, const double* phia, const double*
   if (bf3_exchange_part_bytes(b, W) > es.part_bytes)
     return fail(AFQ_EUNSUPPORTED, "exchange partials exceed the workspace");
   int nsplit = 0;
-  if ((rc = bf3_force_bias(s, b, W, gha, ghb, part, &nsplit))) return rc;
+  if ((rc = bf3_force_bias(s, b, W, gha, ghb, part, &nsplit, sys->fb_nsplit / 2))) return rc;
   if (nsplit > FB_SPLITS_MAX) return fail(AFQ_EUNSUPPORTED, "force-bias splits %d", nsplit);
   if ((rc = bf3_coulomb(s, W, sys->G, nsplit, part, es.ec))) return rc;
   if ((rc = bf3_exchange(s, b, W, gha, ghb, es.part, es.ex))) return rc;

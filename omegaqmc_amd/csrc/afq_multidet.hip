@@ -667,7 +667,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   const double* xi = xi_in;
   if (!xi) {
     phase_begin(sys, PH_RNG);
-    rc = aux_fields_launch(s, seed, step, W, G, st.xi);
+    rc = aux_fields_launch(s, seed, step, W, G, st.xi, sys->step_dev);
     phase_end(sys, PH_RNG);
     if (rc) return rc;
     xi = st.xi;
@@ -799,7 +799,7 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   // 8
   phase_begin(sys, PH_WEIGHTS);
   rc = weights_launch(s, W, weights, st.b.ovlp, ovlp_new, st.cfb, st.cmf, e_hybrid, eshift,
-                      sys->dt, weights, e_hybrid);
+                      sys->dt, weights, e_hybrid, sys->eshift_dev);
   phase_end(sys, PH_WEIGHTS);
   return rc;
 }

@@ -84,10 +84,12 @@ __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
 __global__ void k_weights(int W, const double* __restrict__ w_in, const cplx* __restrict__ o_old,
                           const cplx* __restrict__ o_new, const cplx* __restrict__ cfb,
                           const cplx* __restrict__ cmf, const cplx* __restrict__ eh_old,
-                          double eshift, double dt, double* __restrict__ w_out,
+                          double eshift_arg, const double* __restrict__ eshift_dev, double dt,
+                          double* __restrict__ w_out,
                           cplx* __restrict__ eh_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W) return;
+  const double eshift = eshift_dev ? *eshift_dev : eshift_arg;
   const cplx r = cdiv(o_new[i], o_old[i]);
   const cplx lr = mkc(log(hypot(r.x, r.y)), atan2(r.y, r.x));
   const cplx f = cfb[i], m = cmf[i];
@@ -208,11 +210,11 @@ int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const
 
 int weights_launch(hipStream_t s, int W, const double* w, const double* oo, const double* on,
                    const double* cfb, const double* cmf, const double* eh, double eshift, double dt,
-                   double* w_out, double* eh_out) {
+                   double* w_out, double* eh_out, const double* eshift_dev) {
   if (W <= 0) return AFQ_OK;
   hipLaunchKernelGGL(k_weights, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, w, (const cplx*)oo,
-                     (const cplx*)on, (const cplx*)cfb, (const cplx*)cmf, (const cplx*)eh, eshift, dt,
-                     w_out, (cplx*)eh_out);
+                     (const cplx*)on, (const cplx*)cfb, (const cplx*)cmf, (const cplx*)eh, eshift,
+                     eshift_dev, dt, w_out, (cplx*)eh_out);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }

@@ -26,10 +26,12 @@ __host__ __device__ __forceinline__ u4 philox(u4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-__global__ void k_aux(uint32_t k0, uint32_t k1, uint32_t step, int W, int G, double* __restrict__ xi) {
+__global__ void k_aux(uint32_t k0, uint32_t k1, uint32_t step_arg, const int64_t* __restrict__ step_dev,
+                      int W, int G, double* __restrict__ xi) {
   const int npair = (G + 1) / 2;
   const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (tid >= (long)W * npair) return;
+  const uint32_t step = step_dev ? (uint32_t)*step_dev : step_arg;
   const int w = (int)(tid / npair), k = (int)(tid % npair);
   const u4 o = philox(u4{(uint32_t)k, (uint32_t)w, step, 0u}, k0, k1);
   const uint64_t a = ((uint64_t)o.y << 32) | o.x;
@@ -45,23 +47,40 @@ __global__ void k_aux(uint32_t k0, uint32_t k1, uint32_t step, int W, int G, dou
   if (2 * k + 1 < G) xi[base + 1] = r * sn;
 }
 
+__device__ __host__ __forceinline__ double comb_uniform(uint64_t seed, uint32_t step) {
+  const u4 o = philox(u4{0u, 0u, step, 1u}, (uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)o.y << 32) | o.x;
+  return (double)(a >> 11) * 0x1.0p-53;
+}
+
+// comb offsets of n seeds at the step *step_dev (one thread each)
+__global__ void k_comb_r(int n, const uint64_t* __restrict__ seeds, const int64_t* __restrict__ step_dev,
+                         double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = comb_uniform(seeds[i], (uint32_t)*step_dev);
+}
+
 }  // namespace
 
-int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, double* xi) {
+int aux_fields_launch(hipStream_t s, uint64_t seed, int64_t step, int W, int G, double* xi,
+                      const int64_t* step_dev) {
   const long n = (long)W * ((G + 1) / 2);
   if (n <= 0) return AFQ_OK;
   hipLaunchKernelGGL(k_aux, dim3(cdiv_int(n, 256)), dim3(256), 0, s, (uint32_t)(seed & 0xffffffffu),
-                     (uint32_t)(seed >> 32), (uint32_t)step, W, G, xi);
+                     (uint32_t)(seed >> 32), (uint32_t)step, step_dev, W, G, xi);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
 
-double comb_offset_host(uint64_t seed, int64_t step) {
-  const u4 o = philox(u4{0u, 0u, (uint32_t)step, 1u}, (uint32_t)(seed & 0xffffffffu),
-                      (uint32_t)(seed >> 32));
-  const uint64_t a = ((uint64_t)o.y << 32) | o.x;
-  return (double)(a >> 11) * 0x1.0p-53;
+int comb_offsets_dev_launch(hipStream_t s, int n, const uint64_t* seeds, const int64_t* step_dev,
+                            double* r) {
+  if (n <= 0) return AFQ_OK;
+  hipLaunchKernelGGL(k_comb_r, dim3(cdiv_int(n, 64)), dim3(64), 0, s, n, seeds, step_dev, r);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
 }
+
+double comb_offset_host(uint64_t seed, int64_t step) { return comb_uniform(seed, (uint32_t)step); }
 
 }  // namespace afq
 

@@ -259,7 +259,7 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   if (!xi) {
     double* xg = work_slot(sys, W, XI);
     phase_begin(sys, PH_RNG);
-    rc = aux_fields_launch(s, seed, step, W, G, xg);
+    rc = aux_fields_launch(s, seed, step, W, G, xg, sys->step_dev);
     phase_end(sys, PH_RNG);
     if (rc) return rc;
     xi = xg;
@@ -305,7 +305,7 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   // 8
   phase_begin(sys, PH_WEIGHTS);
   rc = weights_launch(s, W, weights, oo, ovlp_new, cfb, cmf, e_hybrid, eshift, sys->dt, weights,
-                      e_hybrid);
+                      e_hybrid, sys->eshift_dev);
   phase_end(sys, PH_WEIGHTS);
   return rc;
 }
@@ -420,6 +420,23 @@ int afq_system_set_chol_sign(afq_system* s, const double* sign) {
     return afq::fail(AFQ_EUNSUPPORTED,
                      "Cholesky vectors with sign -1 need the FP64 single-determinant path");
   s->chol_sign = neg ? sign : nullptr;
+  return AFQ_OK;
+}
+
+int afq_system_set_force_bias_splits(afq_system* s, int n) {
+  AFQ_REQUIRE(s, "system is NULL");
+  AFQ_REQUIRE(n == 2 || n == 4 || (n == 8 && s->precision != AFQ_BF16X3),
+              "force-bias splits %d (2, 4 or 8; BF16X3: 2 or 4)", n);
+  if (n == s->fb_nsplit) return AFQ_OK;
+  // the FP64 workspace holds one partial force bias per split: re-sized on
+  // the next step
+  if (s->work) {
+    AFQ_CHECK_HIP(hipStreamSynchronize(s->ctx->stream));
+    AFQ_CHECK_HIP(hipFree(s->work));
+    s->work = nullptr;
+    s->cap_walkers = 0;
+  }
+  s->fb_nsplit = n;
   return AFQ_OK;
 }
 
