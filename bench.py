@@ -72,12 +72,13 @@ def bf16x3_taylor_peak(T=6):
 
 
 KERNEL_OF_PHASE = {
-    ("taylor", "fp64"): "k_prop_f64 (expH1 + 6-term Taylor + expH1, FP64 MFMA)",
+    ("taylor", "fp64"): "k_prop<double> (expH1 + 6-term Taylor + expH1, FP64 MFMA)",
+    ("taylor", "fp32"): "k_prop<float> (expH1 + 6-term Taylor + expH1, FP32 MFMA)",
     ("taylor", "bf16x3"): "k_prop_bf3 (expH1 + 6-term Taylor + expH1, BF16x3 MFMA, "
                           "Horner stages 5-6 plain BF16)",
-    ("vhs", "fp64"): "k_vhs_f64 (packed VHS build, FP64 MFMA)",
+    ("vhs", "fp64"): "k_vhs<double> (packed VHS build, FP64 MFMA)",
     ("vhs", "bf16x3"): "k_bf3_gemm<0> (VHS build, BF16x3 MFMA)",
-    ("force_bias", "fp64"): "k_fb_f64 (split-K force bias, FP64 MFMA)",
+    ("force_bias", "fp64"): "k_fb<double> (split-K force bias, FP64 MFMA)",
     ("force_bias", "bf16x3"): "k_bf3_gemm<1> (split-K force bias, BF16x3 MFMA)",
     ("greens", "fp64"): "k_greens_gj (wave Gauss-Jordan) + k_ghalf_inv (FP64 MFMA Ghalf)",
     ("greens", "bf16x3"): "k_greens_gj (wave Gauss-Jordan) + k_ghalf_inv (FP64 MFMA Ghalf)",
@@ -86,10 +87,14 @@ KERNEL_OF_PHASE = {
 
 # rocprofv3 symbol of the dominant kernel at the M=300 workload (MT = 16-row
 # tiles per wave for M=300)
-KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop_f64<3>", ("taylor", "bf16x3"): "k_prop_bf3<3>",
-                 ("vhs", "fp64"): "k_vhs_f64<true>", ("vhs", "bf16x3"): "k_bf3_gemm<0>",
-                 ("force_bias", "fp64"): "k_fb_f64", ("force_bias", "bf16x3"): "k_bf3_gemm<1>",
-                 ("greens", "fp64"): "k_ghalf_inv<10>", ("greens", "bf16x3"): "k_ghalf_inv<10>"}
+KERNEL_SYMBOL = {("taylor", "fp64"): "k_prop<double, 3>", ("taylor", "bf16x3"): "k_prop_bf3<3>",
+                 ("taylor", "fp32"): "k_prop<float, 3>",
+                 ("vhs", "fp64"): "k_vhs<double, true>", ("vhs", "bf16x3"): "k_bf3_gemm<0>",
+                 ("vhs", "fp32"): "k_vhs<float, true>",
+                 ("force_bias", "fp64"): "k_fb<double>", ("force_bias", "bf16x3"): "k_bf3_gemm<1>",
+                 ("force_bias", "fp32"): "k_fb<float>",
+                 ("greens", "fp64"): "k_ghalf_inv<10>", ("greens", "bf16x3"): "k_ghalf_inv<10>",
+                 ("greens", "fp32"): "k_ghalf_inv<10>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phaseless_step_M300_N40_G1800_W1024_hbm_traffic.md")
 
 

@@ -353,13 +353,14 @@ constexpr int PW = 8, PT = 64 * PW, PC = 48;
 constexpr int TAYLOR_BF16_FROM = 5;
 
 // A walker's columns are the concatenation of its two spins (column cg < Na
-// of phia, then Nb of phib), cut into blocks of PC = 48: with N = 40 + 40 the
-// blocks hold 48 and 32 columns, with 16 + 16 one block of 32.  A block's
-// 16-column tiles beyond the last column (x >= nxl) are skipped, a uniform
-// branch per workgroup.
+// of phia, then Nb of phib), cut into nblk blocks of equal width bs <= PC =
+// 48: with N = 40 + 40 two blocks of 40, with 20 + 18 one of 38.  Equal
+// blocks keep a walker's workgroups in step, so the second one finds the
+// walker's VHS rows in L2.  A block's 16-column tiles past its last column
+// (x >= nxl) are skipped, a uniform branch per workgroup.
 template <int MT>
 __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, int Na, int Nb,
-                                                 int nblk, const bf16_t* __restrict__ E,
+                                                 int nblk, int bs, const bf16_t* __restrict__ E,
                                                  const bf16_t* __restrict__ Vb,
                                                  float4* __restrict__ t0s, double* phia,
                                                  double* phib, int nmax) {
@@ -367,15 +368,18 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
   const int KR = Kp + 8;
   const int vb = xcd_remap(blockIdx.x, W * nblk);
   const int w = vb / nblk, cbx = vb % nblk;
-  const int c0 = cbx * PC;
+  const int c0 = cbx * bs;
   const int Nt = Na + Nb;
-  const int nxl = min(3, (Nt - c0 + 15) >> 4);  // live 16-column tiles (uniform)
-  // (spin array, column, row stride) of global column cg; nullptr past the last
-  auto colmap = [&](int cg, int* col, int* ld) -> double* {
-    if (cg < Na) { *col = cg; *ld = Na; return phia; }
-    if (cg < Nt) { *col = cg - Na; *ld = Nb; return phib; }
+  const int ncol = min(bs, Nt - c0);            // columns of this block
+  const int nxl = min(3, (ncol + 15) >> 4);     // live 16-column tiles (uniform)
+  // (spin array, column, row stride) of block column c; nullptr past the block
+  auto colmap = [&](int c, int* col, int* ld) -> double* {
+    const int cg = c0 + c;
     *col = 0; *ld = 1;
-    return nullptr;
+    if (c >= ncol) return nullptr;
+    if (cg < Na) { *col = cg; *ld = Na; return phia; }
+    *col = cg - Na; *ld = Nb;
+    return phib;
   };
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
   const int li = l & 15, lq = l >> 4;
@@ -401,13 +405,13 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
         const int e = e0 + PT * k;
         const int c = e % PC, rp = e / PC;
         int col, ld;
-        const double* base = colmap(min(c0 + c, Nt - 1), &col, &ld);
+        const double* base = colmap(min(c, ncol - 1), &col, &ld);
         const d2v* pw = (const d2v*)base + (long)w * M * ld + col;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = 2 * rp + h;
           const d2v x = pw[(long)min(r, M - 1) * ld];
-          const bool ok = e < np && r < M && c0 + c < Nt;
+          const bool ok = e < np && r < M && c < ncol;
           v[k][h] = ok ? x : d2v{0.0, 0.0};
         }
       }
@@ -484,7 +488,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
         int col, ld;
-        const double* base = colmap(c0 + 16 * x + li, &col, &ld);
+        const double* base = colmap(16 * x + li, &col, &ld);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const bool ok = owns(u) && r0 + r < M && base != nullptr;
@@ -683,7 +687,7 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
       int col, ld;
-      double* base = colmap(c0 + 16 * x + li, &col, &ld);
+      double* base = colmap(16 * x + li, &col, &ld);
       if (!base) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -912,15 +916,17 @@ size_t bf3_prop_scratch_bytes(const Bf3System& b, int W) {
 
 int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, float* t0,
                   double* phia, double* phib, int nmax) {
-  const int nblk = (b.Na + b.Nb + PC - 1) / PC;
+  const int Nt = b.Na + b.Nb;
+  const int nblk = (Nt + PC - 1) / PC;
   if (W <= 0 || nblk == 0) return AFQ_OK;
+  const int bs = (Nt + nblk - 1) / nblk;
   const size_t sm = (size_t)4 * PC * (b.Kp + 8) * sizeof(bf16_t);
   if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: LDS %zu", sm);
   if (b.RT > 4 * PW) return fail(AFQ_EUNSUPPORTED, "bf16x3 propagator: M=%d", b.M);
   if (!t0) return fail(AFQ_EINVAL, "bf16x3 propagator: no scratch");
   const dim3 grid(W * nblk), blk(PT);
 #define AFQ_PROP_BF3(MT)                                                                      \
-  hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, nblk, \
+  hipLaunchKernelGGL((k_prop_bf3<MT>), grid, blk, sm, s, W, b.M, b.RT, b.Kp, b.Na, b.Nb, nblk, bs, \
                      b.E, Vb, (float4*)t0, phia, phib, nmax)
   if (b.RT <= PW) AFQ_PROP_BF3(1);
   else if (b.RT <= 2 * PW) AFQ_PROP_BF3(2);
