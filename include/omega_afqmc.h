@@ -153,8 +153,9 @@ int afq_update_weights_phaseless(afq_ctx* ctx, int nwalkers, const double* weigh
 
 /* afqmc_gto.py:160-181 orthogonalize_walkers(phia, phib) -> (phia, phib, log_detR,
  * sign_detR).  In place on phia/phib; log_detR (W,) real, sign_detR (W,)
- * complex (may be NULL).  Thin QR with positive real diagonal R (modified
- * Gram-Schmidt with one re-orthogonalisation pass). */
+ * complex (may be NULL).  Thin QR with positive real diagonal R (the unique
+ * one, as the reference's sign-fixed Householder QR): CholeskyQR2, with a
+ * shifted CholeskyQR3 when the Gram matrix is ill-conditioned. */
 int afq_orthogonalize_walkers(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
                               double* phia, double* phib, double* log_detR, double* sign_detR);
 
@@ -251,8 +252,10 @@ int afq_local_energy_multidet(afq_ctx* ctx, int nwalkers, int nbasis, int nup, i
  * caller's; occ/ci are copied.  afq_propagate_walkers then runs
  * propagate_walkers_multidet (afqmc_gto.py:728-839: force bias from the
  * multi-det G, overlap_new from the multi-det overlap) and
- * afq_system_local_energy runs local_energy_multidet (FP64).  precision
- * AFQ_BF16X3 applies to the VHS build and the propagator only. */
+ * afq_system_local_energy runs local_energy_multidet.  precision AFQ_FP64
+ * or AFQ_BF16X3; BF16X3 applies to the force bias, the VHS build, the
+ * propagator and each determinant's exchange energy (the determinants,
+ * Green's functions, Coulomb and one-body energy stay FP64). */
 int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int nbasis, int nup, int ndown,
                                int naux, const double* chol, const double* expH1,
                                const double* mf_shift, int ndet, const int32_t* occ_up,

@@ -90,7 +90,7 @@ def make_omegaqmc_trial(coeffs, dets, mo_coeff, n_core=0, index_map=None, meta=N
     """
     if meta and meta.get('indexing') == 'active':
         if index_map is not None:
-            print("  [edga2omegaqmc] indexing=active detected in header, ignoring index_map")
+            print("EDGA header says indexing=active: the given index_map is not used")
         index_map = None
     ndet = len(coeffs)
     na, nb = len(dets[0]['alpha']), len(dets[0]['beta'])
@@ -105,8 +105,9 @@ def make_omegaqmc_trial(coeffs, dets, mo_coeff, n_core=0, index_map=None, meta=N
 
     for k, det in enumerate(dets):
         a, b = det['alpha'], det['beta']
-        assert len(a) == na, f"Det {k}: expected {na} alpha electrons, got {len(a)}"
-        assert len(b) == nb, f"Det {k}: expected {nb} beta electrons, got {len(b)}"
+        if len(a) != na or len(b) != nb:
+            raise ValueError(f"determinant {k} has {len(a)} up / {len(b)} down orbitals, "
+                             f"the first one {na} / {nb}")
         occ_up[k] = sorted(core + mapped(a))
         occ_dn[k] = sorted(core + mapped(b))
     return {'ci_coeffs': np.asarray(coeffs, dtype=np.float64), 'occ_up': occ_up,
