@@ -184,7 +184,7 @@ def run_gpu(sys_np, W, steps, warmup, precision, seed=42, timing=True):
             "comb": comb}
 
 
-def _drivers_for(sys_np, precisions, trial=None):
+def _drivers_for(sys_np, precisions, trial=None, multidet_method=None):
     import torch
     from omegaqmc_amd.driver import AFQMCDriver
     dev = torch.device("cuda", 0)
@@ -192,7 +192,8 @@ def _drivers_for(sys_np, precisions, trial=None):
     integ = {'h1e': t(sys_np.h1e), 'h1e_mod': t(sys_np.h1e_mod), 'chol': t(sys_np.chol),
              'enuc': sys_np.enuc, 'nbasis': sys_np.nbasis, 'nup': sys_np.nup,
              'ndown': sys_np.ndown}
-    return [AFQMCDriver(integ, verbose=False, precision=p, trial=trial) for p in precisions]
+    return [AFQMCDriver(integ, verbose=False, precision=p, trial=trial,
+                        multidet_method=multidet_method) for p in precisions]
 
 
 def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10, trial=None):
@@ -275,9 +276,9 @@ def multidet_system(seed=42):
     return s, multidet_trial(M, Na, Nb, 32)
 
 
-def run_multidet(s, trial, W, steps, warmup, precision, seed=42):
+def run_multidet(s, trial, W, steps, warmup, precision, seed=42, method=None):
     import torch
-    (drv,) = _drivers_for(s, (precision,), trial)
+    (drv,) = _drivers_for(s, (precision,), trial, method)
     st = drv.init_walkers(W)
     eshift = drv.ehf - drv.e_const
     step = 0
@@ -577,14 +578,19 @@ def main():
         md = {"workload": "multi-det trial: M=150 N=20+18 Nchol=800 32 dets W=1024 dt=0.005 "
                           "(size-normalised L_g)"}
         md_sys, md_trial = multidet_system()
-        for prec in ("fp64", "bf16x3"):
-            print(f"[bench] multi-det {prec} step", file=sys.stderr, flush=True)
+        # ratio updates relative to det 0 (the default), and the explicit
+        # per-det inverses beside them
+        for prec, meth in (("fp64", "ratio"), ("bf16x3", "ratio"), ("fp64", "explicit"),
+                           ("bf16x3", "explicit")):
+            key = prec if meth == "ratio" else f"{prec}_explicit"
+            print(f"[bench] multi-det {prec} step ({meth})", file=sys.stderr, flush=True)
             try:
-                r = run_multidet(md_sys, md_trial, W, max(3, args.steps // 2), 1, prec)
+                r = run_multidet(md_sys, md_trial, W, max(3, args.steps // 2), 1, prec,
+                                 method=meth)
             except _lib.AFQError as e:
-                md[f"{prec}_error"] = str(e)[:200]
+                md[f"{key}_error"] = str(e)[:200]
                 continue
-            md[prec] = r
+            md[key] = r
             torch.cuda.empty_cache()
         if "bf16x3" in md and args.acc_steps >= 10:
             print(f"[bench] multi-det bf16x3 vs FP64 energy, same trajectory "

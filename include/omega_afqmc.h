@@ -262,6 +262,18 @@ int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int nbasis, int n
                                const int32_t* occ_dn, const double* ci_coeffs, double dt,
                                double fbbound, int exp_nmax, int det_chunk_size, int precision);
 
+/* How the step of a multi-det system forms the multi-det overlaps and G:
+ * 0 one Gauss-Jordan inverse per (walker, det) (greens_th4.py:75-148);
+ * 1 ratio updates relative to det 0 -- one inverse per walker, each det's
+ * overlap ratio det Q_I by the matrix determinant lemma and its G by a
+ * Woodbury correction (the evaluator of wick.py:239-284, 425-485 used by
+ * propagate_walkers_wick, afqmc_gto.py:842-935).  Method 1 is the default
+ * when every det is within four orbitals of det 0; setting it otherwise
+ * returns AFQ_EUNSUPPORTED.  The block energy always takes every det's
+ * Green's function explicitly.  get returns the method in use. */
+int afq_system_set_multidet_method(afq_system* sys, int method);
+int afq_system_get_multidet_method(afq_system* sys);
+
 /* ---- prepared system and the fused step -------------------------------- */
 
 /* Setup counterpart of build_propagator (afqmc_gto.py:574-627) + the half

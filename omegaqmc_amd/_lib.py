@@ -79,6 +79,8 @@ SIGNATURES = {
     "afq_batch_copy_state": [_P, _I, _P, _P, _P, _P, _P, _P],
     "afq_batch_destroy": [_P],
     "afq_system_set_force_bias_splits": [_P, _I],
+    "afq_system_set_multidet_method": [_P, _I],
+    "afq_system_get_multidet_method": [_P],
     "afq_system_timing_enable": [_P, _I],
     "afq_system_timing_read": [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64), _I, _I],
 }

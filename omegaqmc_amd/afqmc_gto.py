@@ -626,8 +626,10 @@ def multidet_charge_density(trials_up, trials_dn, ci_coeffs):
 class MultiDetSystem(System):
     """An ``afq_system`` with a multi-determinant trial of occupation strings."""
 
+    METHODS = {"explicit": 0, "ratio": 1, "wick": 1}
+
     def __init__(self, propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound=None,
-                 exp_nmax=6, det_chunk_size=5, precision="fp64"):
+                 exp_nmax=6, det_chunk_size=5, precision="fp64", method=None):
         self.chol = _r(chol)
         E = propagator['expH1']
         if isinstance(E, torch.Tensor) and E.is_complex():
@@ -656,18 +658,30 @@ class MultiDetSystem(System):
             ptr(self.ci), self.dt, self.fbbound, self.nmax, int(det_chunk_size),
             _lib.PRECISIONS[precision]), "afq_system_create_multidet")
         self.handle = h
+        if method is not None:
+            # "explicit": one inverse per (walker, det); "ratio" (or the
+            # reference's name "wick"): ratio updates relative to det 0
+            check(_L().afq_system_set_multidet_method(h, self.METHODS[method]),
+                  "afq_system_set_multidet_method")
+
+    @property
+    def method(self):
+        return {0: "explicit", 1: "ratio"}[int(_L().afq_system_get_multidet_method(self.handle))]
 
 
 def propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagator, chol,
                                trials_up, trials_dn, ci_coeffs, eshift, rng_key, fbbound=None,
                                exp_nmax=6, walker_chunk_size=None, chol_chunk_g=None,
-                               det_chunk_size=5, xi=None, precision="fp64"):
-    """afqmc_gto.py:728-839 -> (phia, phib, weights_new, overlap_new, e_hybrid_new, rng_key)."""
+                               det_chunk_size=5, xi=None, precision="fp64", method=None):
+    """afqmc_gto.py:728-839 -> (phia, phib, weights_new, overlap_new, e_hybrid_new, rng_key).
+
+    ``method``: None (ratio updates when every det is within four orbitals of
+    det 0, else explicit), "explicit" or "ratio"."""
     sysobj = _cached_system(
         propagator, (chol, trials_up, trials_dn, ci_coeffs),
-        (fbbound, exp_nmax, det_chunk_size, precision, "md"),
+        (fbbound, exp_nmax, det_chunk_size, precision, "md", method),
         lambda: MultiDetSystem(propagator, chol, trials_up, trials_dn, ci_coeffs, fbbound,
-                               exp_nmax, det_chunk_size, precision))
+                               exp_nmax, det_chunk_size, precision, method))
     pa, pb = _c(phia).clone(), _c(phib).clone()
     w = _r(weights).clone()
     eh = _c(e_hybrid).clone()

@@ -97,6 +97,10 @@ struct afq_ctx {
   double* comb_static = nullptr;
 };
 
+namespace afq {
+struct MdRatio;
+}
+
 struct afq_system {
   afq_ctx* ctx = nullptr;
   int M = 0, Na = 0, Nb = 0, G = 0;
@@ -145,6 +149,12 @@ struct afq_system {
   // the energy shift *eshift_dev instead of the call's arguments
   const int64_t* step_dev = nullptr;
   const double* eshift_dev = nullptr;
+  // excitation tables of the multi-det dets relative to det 0 (owned; NULL
+  // when some det is more than four orbitals away) and the method of the
+  // step's multi-det overlaps and G: 0 one inverse per (walker, det), 1 ratio
+  // updates relative to det 0
+  afq::MdRatio* md_ratio = nullptr;
+  int md_method = 0;
   // split-BF16 planes of chol_packed for the BF16x3 multi-det force bias
   // ([2][G][md_kp], built on first use, owned)
   bf16_t* md_lpk = nullptr;

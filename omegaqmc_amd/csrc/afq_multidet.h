@@ -44,6 +44,20 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
 // Host check of occupation strings (device arrays): entries in [0, M), distinct per det.
 int md_check_occ(hipStream_t s, int nd, int N, int M, const int32_t* occ);
 
+// Ratio updates relative to det 0 (afq_mdratio.hip): excitation tables
+// (ok = false when a det is more than four orbitals from det 0), whether the
+// kernel's LDS fits, and the launch: mode 0 the multi-det overlap, mode 1
+// also B_s (W, M, N_s), so that G_s = B_s phi_s^T.
+struct MdRatio;
+int md_ratio_prepare(afq_system* sys, bool* ok);
+void md_ratio_release(MdRatio* r);
+bool md_ratio_fits(const afq_system* sys);
+int md_ratio_launch(afq_system* sys, int mode, int W, const double* phia, const double* phib,
+                    double* ovlp, double* Ba, double* Bb);
+// G_s = B_s phi_s^T (one batched GEMM per spin), or their sum into Ga (gsum)
+int md_g_from_b(hipStream_t s, int W, int M, int Na, int Nb, const double* phia,
+                const double* phib, bool gsum, const MdBufs& b);
+
 int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, double* weights,
                        double* e_hybrid, double eshift, const double* xi, uint64_t seed,
                        int64_t step, double* ovlp_new);

@@ -495,20 +495,24 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
                          (const cplx*)b.invb, (cplx*)b.Ba, (cplx*)b.Bb);
       AFQ_CHECK_LAUNCH();
     }
-    if (b.Ga) {
-      // G_s[w][p][q] = sum_j B_s[w][p][j] phi_s[w][q][j]; an empty spin gives 0
-      for (int sp = 0; sp < 2; ++sp) {
-        const int N = sp ? Nb : Na;
-        GemmArgs g;
-        g.m = M; g.n = M; g.k = N; g.batch = W;
-        g.A = sp ? b.Bb : b.Ba; g.lda_m = N; g.lda_k = 1; g.sA = (long)M * N; g.a_cplx = true;
-        g.B = sp ? phib : phia; g.ldb_k = 1; g.ldb_n = N; g.sB = (long)M * N; g.b_cplx = true;
-        g.C = (cplx*)((sp && !gsum) ? b.Gb : b.Ga); g.ldc = M; g.sC = (long)M * M;
-        g.beta = (sp && gsum) ? 1.0 : 0.0;
-        int rc = gemm_fp64(s, g);
-        if (rc) return rc;
-      }
-    }
+    if (b.Ga) return md_g_from_b(s, W, M, Na, Nb, phia, phib, gsum, b);
+  }
+  return AFQ_OK;
+}
+
+int md_g_from_b(hipStream_t s, int W, int M, int Na, int Nb, const double* phia,
+                const double* phib, bool gsum, const MdBufs& b) {
+  // G_s[w][p][q] = sum_j B_s[w][p][j] phi_s[w][q][j]; an empty spin gives 0
+  for (int sp = 0; sp < 2; ++sp) {
+    const int N = sp ? Nb : Na;
+    GemmArgs g;
+    g.m = M; g.n = M; g.k = N; g.batch = W;
+    g.A = sp ? b.Bb : b.Ba; g.lda_m = N; g.lda_k = 1; g.sA = (long)M * N; g.a_cplx = true;
+    g.B = sp ? phib : phia; g.ldb_k = 1; g.ldb_n = N; g.sB = (long)M * N; g.b_cplx = true;
+    g.C = (cplx*)((sp && !gsum) ? b.Gb : b.Ga); g.ldc = M; g.sC = (long)M * M;
+    g.beta = (sp && gsum) ? 1.0 : 0.0;
+    int rc = gemm_fp64(s, g);
+    if (rc) return rc;
   }
   return AFQ_OK;
 }
@@ -676,8 +680,13 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   phase_begin(sys, PH_GREENS);
   MdBufs gb = st.b;
   if (sys->chol_packed) gb.Ga = nullptr;  // the packed triangle comes from B_s and phi_s
-  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 MD_INVERSE, true, gb);
+  if (sys->md_method == 1) {
+    rc = md_ratio_launch(sys, 1, W, phia, phib, gb.ovlp, gb.Ba, gb.Bb);
+    if (!rc && gb.Ga) rc = md_g_from_b(s, W, M, Na, Nb, phia, phib, true, gb);
+  } else {
+    rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b,
+                   sys->ci, MD_INVERSE, true, gb);
+  }
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
   // 3: vbias_g = sum_pq L_g,pq (Ga + Gb)_qp: over the packed triangle on the
@@ -792,8 +801,11 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
   phase_begin(sys, PH_OVERLAP);
   MdBufs ob = md_carve(ovb.oda, W, M, Na, Nb, nd, MD_OVERLAP, false);
   ob.ovlp = ovlp_new;
-  rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b, sys->ci,
-                 MD_OVERLAP, false, ob);
+  if (sys->md_method == 1)
+    rc = md_ratio_launch(sys, 0, W, phia, phib, ovlp_new, nullptr, nullptr);
+  else
+    rc = md_greens(s, W, M, Na, Nb, nd, sys->det_chunk, phia, phib, sys->occ_a, sys->occ_b,
+                   sys->ci, MD_OVERLAP, false, ob);
   phase_end(sys, PH_OVERLAP);
   if (rc) return rc;
   // 8
@@ -938,12 +950,29 @@ int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int M, int Na, in
   }
   rc = afq::prepare_fp64(sy);
   if (!rc) rc = afq::system_prepare_lowp(sy);
+  bool ratio = false;
+  if (!rc) rc = afq::md_ratio_prepare(sy, &ratio);
   if (rc) {
     afq_system_destroy(sy);
     return rc;
   }
+  // the step uses ratio updates whenever every det is within four orbitals
+  // of det 0 and the kernel's LDS fits
+  sy->md_method = (ratio && afq::md_ratio_fits(sy)) ? 1 : 0;
   *out = sy;
   return AFQ_OK;
 }
+
+int afq_system_set_multidet_method(afq_system* sys, int method) {
+  AFQ_REQUIRE(sys && sys->ndet > 0, "not a multi-determinant system");
+  AFQ_REQUIRE(method == 0 || method == 1, "method %d (0 explicit, 1 ratio)", method);
+  if (method == 1 && !(sys->md_ratio && afq::md_ratio_fits(sys)))
+    return afq::fail(AFQ_EUNSUPPORTED,
+                     "ratio updates need every det within four orbitals of det 0");
+  sys->md_method = method;
+  return AFQ_OK;
+}
+
+int afq_system_get_multidet_method(afq_system* sys) { return sys ? sys->md_method : -1; }
 
 }  // extern "C"

@@ -482,6 +482,7 @@ int afq_system_destroy(afq_system* s) {
   if (s->chol_packed) (void)hipFree(s->chol_packed);
   if (s->colmap) (void)hipFree(s->colmap);
   if (s->md_lpk) (void)hipFree(s->md_lpk);
+  afq::md_ratio_release(s->md_ratio);
   afq::system_release_lowp(s);
   delete s;
   return AFQ_OK;

@@ -29,7 +29,8 @@ class AFQMCDriver:
     """Set up once (integrals, trial, propagator), run many times."""
 
     def __init__(self, integrals, dt=0.005, e_hf=None, eshift0=None, verbose=True,
-                 precision="fp64", fbbound=None, exp_nmax=6, trial=None, det_chunk_size=5):
+                 precision="fp64", fbbound=None, exp_nmax=6, trial=None, det_chunk_size=5,
+                 multidet_method=None):
         self.dt = float(dt)
         self.verbose = verbose
         self.precision = precision
@@ -78,7 +79,7 @@ class AFQMCDriver:
             self.system = MultiDetSystem(self.propagator, self.chol, self.trials_up,
                                          self.trials_dn, self.ci_coeffs, fbbound=fbbound,
                                          exp_nmax=exp_nmax, det_chunk_size=det_chunk_size,
-                                         precision=precision)
+                                         precision=precision, method=multidet_method)
         else:
             self.system = System(self.propagator, self.chol, self.rchol_a, self.rchol_b,
                                  self.trial_up, self.trial_dn, fbbound=fbbound,
@@ -290,12 +291,15 @@ def get_afqmc_func_from_integrals(h1, chol, ecore, nelec, mo_coeff=None, e_hf=No
                                   det_chunk_size=5, multidet_method='explicit',
                                   precision="fp64"):
     """afqmc_gto.py:1592-1631.  ``trial`` (multi-det) is a dict with 'ci_coeffs',
-    'occ_up', 'occ_dn' (and optionally 'rdm1'); only the explicit multi-det
-    method is built (``multidet_method='explicit'``)."""
-    if trial is not None and multidet_method != 'explicit':
-        raise NotImplementedError("only multidet_method='explicit' is built")
+    'occ_up', 'occ_dn' (and optionally 'rdm1').  ``multidet_method``:
+    'explicit' (one inverse per walker and det) or 'wick' (ratio updates
+    relative to det 0: the step's overlaps and Green's function from one
+    inverse per walker; needs every det within four orbitals of det 0)."""
+    if trial is not None and multidet_method not in ('explicit', 'wick'):
+        raise ValueError(f"multidet_method {multidet_method!r}: 'explicit' or 'wick'")
     if trial is not None and trial.get('mo_coeff') is not None:
         raise NotImplementedError("rotate the integrals into the trial basis (mo_coeff=None)")
     integrals = prepare_afqmc_integrals_from_arrays(h1, chol, ecore, nelec, mo_coeff=mo_coeff)
     return AFQMCDriver(integrals, dt=dt, e_hf=e_hf, eshift0=eshift0, verbose=verbose,
-                       precision=precision, trial=trial, det_chunk_size=det_chunk_size)
+                       precision=precision, trial=trial, det_chunk_size=det_chunk_size,
+                       multidet_method=None if trial is None else multidet_method)

@@ -167,8 +167,9 @@ def m150_md(afq):
     return s, tr, tu, td
 
 
+@pytest.mark.parametrize("method", ["ratio", "explicit"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("bf16x3", 1e-4)])
-def test_multidet_step_m150(afq, m150_md, precision, tol):
+def test_multidet_step_m150(afq, m150_md, precision, tol, method):
     """One multi-determinant step (k_md_gj<20>, k_md_gsym, the force bias over
     the packed triangle, k_prop_f64<2> / k_prop_bf3<2>) at the 32-det
     workload's shape against oracle/multidet.py."""
@@ -186,7 +187,7 @@ def test_multidet_step_m150(afq, m150_md, precision, tol):
     ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -200.0, xi)
     out = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
                                          _t(tr['occ_up']), _t(tr['occ_dn']), _t(ci), -200.0, None,
-                                         xi=_t(xi), precision=precision)
+                                         xi=_t(xi), precision=precision, method=method)
     for k, name in enumerate(("phia", "phib", "weights", "overlap", "e_hybrid")):
         lim = tol if k < 3 else (10 * tol if precision == "fp64" else 100 * tol)
         assert rel_err(_n(out[k]), ref[k]) < lim, (name, rel_err(_n(out[k]), ref[k]))

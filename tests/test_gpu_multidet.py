@@ -104,9 +104,10 @@ def _propagator(afq, s, tu, td, ci, dt=0.01):
                   'dt': dt}
 
 
+@pytest.mark.parametrize("method", ["explicit", "ratio"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("bf16x3", 1e-4)])
-@pytest.mark.parametrize("M,Na,Nb,G,nd,W,seed", CASES[:2])
-def test_propagate_walkers_multidet(afq, precision, tol, M, Na, Nb, G, nd, W, seed):
+@pytest.mark.parametrize("M,Na,Nb,G,nd,W,seed", CASES[:2] + [CASES[3]])
+def test_propagate_walkers_multidet(afq, method, precision, tol, M, Na, Nb, G, nd, W, seed):
     s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
     prop, propd = _propagator(afq, s, tu, td, ci)
     rng = np.random.default_rng(seed + 100)
@@ -116,10 +117,50 @@ def test_propagate_walkers_multidet(afq, precision, tol, M, Na, Nb, G, nd, W, se
     ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -3.0, xi)
     out = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
                                          _t(ou), _t(od), _t(ci), -3.0, None, xi=_t(xi),
-                                         precision=precision)
+                                         precision=precision, method=method)
     # e_hybrid = -(log ratio + cfb + cmf) / dt amplifies the walkers' error by 1/dt
     for k in (0, 1, 2, 3, 4):
         assert rel_err(_n(out[k]), ref[k]) < (tol if k < 4 else 10 * tol), k
+
+
+def test_ratio_method_availability(afq):
+    """Ratio updates need every det within four orbitals of det 0: a det five
+    orbitals away leaves the system on the explicit path, and asking for the
+    ratio path is refused; otherwise the ratio path is the default."""
+    import omegaqmc_amd as A
+    M, Na, Nb, G = 16, 6, 2, 20
+    from omegaqmc_amd.synth import make_system
+    s = make_system(M, Na, Nb, G, seed=3)
+    ci = np.array([1.0, 0.3, 0.2]) + 0j
+    ou = np.array([range(6), [0, 1, 2, 3, 4, 10], [0, 11, 12, 13, 14, 15]], dtype=np.int32)
+    od = np.array([[0, 1], [0, 1], [1, 2]], dtype=np.int32)
+    tu, td = MD.trials_from_occ(M, ou), MD.trials_from_occ(M, od)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    far = A.MultiDetSystem(propd, _t(s.chol), _t(ou), _t(od), _t(ci))
+    assert far.method == "explicit"
+    with pytest.raises(afq.AFQError):
+        A.MultiDetSystem(propd, _t(s.chol), _t(ou), _t(od), _t(ci), method="ratio")
+    near = A.MultiDetSystem(propd, _t(s.chol), _t(ou[:2]), _t(od[:2]), _t(ci[:2]))
+    assert near.method == "ratio"
+
+
+def test_ratio_near_node_walkers(afq):
+    """Walkers far from the trial (noise 0.6 on every orbital coefficient, so
+    det 0's overlap is small and the excited dets' ratios large): the ratio
+    path still follows the oracle to 1e-8 relative."""
+    s, ou, od, ci, tu, td, pa, pb = _case(16, 5, 4, 30, 11, 6, 2, scale=0.6)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    rng = np.random.default_rng(77)
+    W = pa.shape[0]
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    xi = P.aux_fields(8, 1, W, 30)
+    ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -3.0, xi)
+    out = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
+                                         _t(ou), _t(od), _t(ci), -3.0, None, xi=_t(xi),
+                                         method="ratio")
+    for k in range(5):
+        assert rel_err(_n(out[k]), ref[k]) < 1e-8, k
 
 
 def test_multidet_system_energy(afq):
