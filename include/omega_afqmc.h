@@ -332,6 +332,49 @@ int afq_system_timing_enable(afq_system* sys, int on);
 int afq_system_timing_read(afq_system* sys, double* total_ms, int64_t* launches, int nphase,
                            int reset);
 
+/* ---- cavity QED (OmegaQMC/qed_afqmc_gto.py) ----------------------------- */
+/* Walkers carry a photon coordinate q (W,) real.  The QED step is the
+ * electronic step of afq_force_bias / afq_build_vhs / afq_apply_exp_vhs /
+ * afq_apply_one_body with the augmented Cholesky vectors (dipole self-energy
+ * as one more vector, integrals/qed.py:130-190), wrapped by the calls below
+ * (order of propagate_qed_walkers, qed_afqmc_gto.py:335-440). */
+
+/* qed_afqmc_gto.py:262-311 propagate_photon: one half step of the exact
+ * harmonic-oscillator kernel (delta = omega dt / 2), importance-sampled with
+ * the Gaussian trial (s, q0); eta (W,) the N(0,1) draws.  Writes q_out and
+ * the log weight factor log_w (W,). */
+int afq_qed_propagate_photon(afq_ctx* ctx, int nwalkers, const double* q, const double* eta,
+                             double omega, double dt, double s, double q0, double* q_out,
+                             double* log_w);
+
+/* qed_afqmc_gto.py:314-332 _apply_exp_vhs_photon: phi_out[w] =
+ * sum_{n<=nmax} (coef[w] mat)^n phi[w] / n!; coef (W,) and mat (M,M) real,
+ * phi (W,M,N) complex; phi_out may alias phi.  (The reference applies it
+ * with nmax = 4 and coef = -dt/2 sqrt(omega) q.) */
+int afq_apply_exp_scaled(afq_ctx* ctx, int nwalkers, int nbasis, int nocc, const double* coef,
+                         const double* mat, const double* phi, int nmax, double* phi_out);
+
+/* qed_afqmc_gto.py:436-438: weights *= exp(clip(log_w1 + log_w2, -10, 10)). */
+int afq_qed_photon_weights(afq_ctx* ctx, int nwalkers, double* weights, const double* log_w1,
+                           const double* log_w2);
+
+/* qed_afqmc_gto.py:106-153 (the photon share of population_control_comb_qed):
+ * dst[i] = src[idx[i]] for the survivor indices of afq_population_control_comb;
+ * an index outside [0, W) gives NaN. */
+int afq_gather_walker_values(afq_ctx* ctx, int nwalkers, const int32_t* idx, const double* src,
+                             double* dst);
+
+/* qed_afqmc_gto.py:156-201 qed_local_energy: the local_energy of energy.py:94-130
+ * (h1e, augmented rchol, real) plus e_ep = sqrt(omega) q Tr(dip (Ga + Gb))
+ * in e_1b and e_ph = omega/2 (s - s^2 (q - q0)^2 + q^2 - 1);
+ * e_tot = e_1b + e_2b + e_ph.  dip (M,M) real; outputs (W,) complex. */
+int afq_qed_local_energy(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                         const double* h1e, const double* dip, const double* Ga,
+                         const double* Gb, const double* Ghalfa, const double* Ghalfb,
+                         const double* rchol_a, const double* rchol_b, double enuc,
+                         const double* q, double omega, double s, double q0, double* e_tot,
+                         double* e_1b, double* e_2b, double* e_ph);
+
 /* ---- uneven batches of independent systems ----------------------------- */
 /* The driver step of afqmc_gto.py:1383-1451 (QR every stabilize_freq steps
  * before propagating, the fused step, weight clip from step 2 on, comb every

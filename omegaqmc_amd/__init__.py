@@ -1,7 +1,8 @@
 """omegaqmc_amd -- MI355X-native phaseless-AFQMC walker propagator.
 
 Hand-written gfx950 HIP kernels behind a C ABI (include/omega_afqmc.h), with
-a Python mirror of the reference's ``OmegaQMC.afqmc_gto`` operator API.
+a Python mirror of the reference's ``OmegaQMC.afqmc_gto`` operator API (and of
+``OmegaQMC.qed_afqmc_gto`` for cavity QED in :mod:`.qed`).
 """
 
 from .afqmc_gto import (  # noqa: F401
@@ -18,6 +19,8 @@ from .afqmc_gto import (  # noqa: F401
 from ._lib import AFQError  # noqa: F401
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
 from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401
+from .qed import (QEDDriver, get_qed_afqmc_func_from_integrals,  # noqa: F401
+                  prepare_qed_integrals_from_arrays, propagate_qed_walkers, qed_local_energy)
 from .edga import load_edga_result, make_omegaqmc_trial, select_top_determinants  # noqa: F401
 
 __version__ = "0.1.0"

@@ -72,6 +72,12 @@ SIGNATURES = {
                                    _P, _P, _D, _D, _I, _I, _I],
     "afq_propagate_walkers": [_P, _I, _P, _P, _P, _P, _D, _P, _U64, _I64, _P],
     "afq_system_local_energy": [_P, _I, _P, _P, _P, _D, _P, _P, _P],
+    "afq_qed_propagate_photon": [_P, _I, _P, _P, _D, _D, _D, _D, _P, _P],
+    "afq_apply_exp_scaled": [_P, _I, _I, _I, _P, _P, _P, _I, _P],
+    "afq_qed_photon_weights": [_P, _I, _P, _P, _P],
+    "afq_gather_walker_values": [_P, _I, _P, _P, _P],
+    "afq_qed_local_energy": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _D,
+                             _D, _D, _P, _P, _P, _P],
     "afq_batch_create": [_P, ctypes.POINTER(_P), _I, _P, _P, _P, _I, _I, _I, _P],
     "afq_batch_set_eshift": [_P, _P],
     "afq_batch_run": [_P, _I, _I],

@@ -291,12 +291,16 @@ __global__ __launch_bounds__(NTH) void k_fb(int W, int G, FbSplits sp, cplx* __r
 // output may alias the input: a workgroup reads and writes only its own
 // columns, reading all of them before writing any.  With Mf<T>::DFORM the
 // matrix E is expH1 - I and each one-body product is acc = y + E y.
-template <class T, int MT>
+// SCALED: walker w's matrix is scoef[w] * smat (both real, (W,) and (M, M))
+// instead of V[w].
+template <class T, int MT, bool SCALED = false>
 __global__ __launch_bounds__(PTH) void k_prop(int W, int M, int Na, int Nb, int ncb,
                                               const T* __restrict__ E,
                                               const typename Mf<T>::C* __restrict__ V,
                                               const cplx* pa_in, const cplx* pb_in,
-                                              cplx* pa_out, cplx* pb_out, int nmax) {
+                                              cplx* pa_out, cplx* pb_out, int nmax,
+                                              const double* __restrict__ smat = nullptr,
+                                              const double* __restrict__ scoef = nullptr) {
   typedef Mf<T> F;
   typedef typename F::C C;
   typedef typename F::V4 V4;
@@ -309,7 +313,8 @@ __global__ __launch_bounds__(PTH) void k_prop(int W, int M, int Na, int Nb, int 
   const int li = l & 15, lq = l >> 4;
   const int Nt = Na + Nb;
   const long MM = (long)M * M;
-  const C* __restrict__ Vw = V + (long)w * MM;
+  const C* __restrict__ Vw = SCALED ? nullptr : V + (long)w * MM;
+  const double sc = SCALED ? scoef[w] : 0.0;
 
   // element offset of (row, column c of this block) in the spin array, or -1
   auto coloff = [&](int c, long row, bool* spin_b) -> long {
@@ -389,7 +394,10 @@ __global__ __launch_bounds__(PTH) void k_prop(int W, int M, int Na, int Nb, int 
 #pragma unroll
       for (int u = 0; u < MT; ++u) {
         const int row = arow(u);
-        a[u] = (row < M && k < M) ? Vw[(long)row * M + k] : F::mk(0.0, 0.0);
+        if (SCALED)
+          a[u] = (row < M && k < M) ? F::mk(sc * smat[(long)row * M + k], 0.0) : F::mk(0.0, 0.0);
+        else
+          a[u] = (row < M && k < M) ? Vw[(long)row * M + k] : F::mk(0.0, 0.0);
       }
     };
     auto mm = [&](const C* a, int k0) {
@@ -590,6 +598,29 @@ int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* exp
                   const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
                   double* pb_out, int nmax) {
   return prop_launch<double>(s, W, M, Na, Nb, expH1, VHS, pa_in, pb_in, pa_out, pb_out, nmax);
+}
+
+int prop_scaled_f64(hipStream_t s, int W, int M, int N, const double* mat, const double* coef,
+                    const double* phi_in, double* phi_out, int nmax) {
+  if (W <= 0 || N == 0) return AFQ_OK;
+  const int RT = (M + 15) / 16;
+  const int ncb = (N + 15) / 16;
+  const size_t sm = (size_t)RT * 16 * 16 * sizeof(cplx);
+  if (sm > max_lds_bytes())
+    return fail(AFQ_EUNSUPPORTED, "scaled propagator: M=%d needs %zu bytes of LDS", M, sm);
+  const dim3 grid(W * ncb), blk(PTH);
+#define AFQ_LAUNCH_SC(MT)                                                                      \
+  hipLaunchKernelGGL((k_prop<double, MT, true>), grid, blk, sm, s, W, M, N, 0, ncb,              \
+                     (const double*)nullptr, (const cplx*)nullptr, (const cplx*)phi_in,          \
+                     (const cplx*)nullptr, (cplx*)phi_out, (cplx*)nullptr, nmax, mat, coef)
+  if (RT <= PNW) AFQ_LAUNCH_SC(1);
+  else if (RT <= 2 * PNW) AFQ_LAUNCH_SC(2);
+  else if (RT <= 3 * PNW) AFQ_LAUNCH_SC(3);
+  else if (RT <= 4 * PNW) AFQ_LAUNCH_SC(4);
+  else return fail(AFQ_EUNSUPPORTED, "scaled propagator: M=%d > 512", M);
+#undef AFQ_LAUNCH_SC
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
 }
 
 int prop_mfma_f32(hipStream_t s, int W, int M, int Na, int Nb, const float* expH1_minus_I,

@@ -16,6 +16,8 @@ Modules
 ``multidet`` explicit multi-determinant Green's function / overlap / energy.
 ``heg``     electron-gas step and sign-weighted local energy
             (``OmegaQMC/afqmc_pw_heg.py``).
+``qed``     cavity-QED step: photon half steps, dipole Taylor factor, QED
+            local energy and comb (``OmegaQMC/qed_afqmc_gto.py``).
 ``philox``  Philox4x32-10 counter-based generator (Salmon et al., SC'11) and
             the Box-Muller normal stream the GPU generator also implements.
 
