@@ -208,6 +208,30 @@ int afq_accumulate_hybrid(afq_ctx* ctx, int nwalkers, const double* weights,
 int afq_block_stats(afq_ctx* ctx, int nwalkers, const double* weights, const double* e_tot,
                     double* out);
 
+/* ---- per-walker numerical status ---------------------------------------- */
+/* The reference lets NaN/inf propagate through the step (numerical failures
+ * surface as NaN energies) and masks only near-zero multi-det overlaps
+ * (observables/greens.py:264-276, greens_th4.py:17-44).  afq_walker_status
+ * writes one flag word per walker (bits below) from phi_a, phi_b (W,M,N_s)
+ * complex, overlap (W,) complex, weights (W,) real and e_hybrid (W,)
+ * complex; overlap, weights and e_hybrid may be NULL (not checked).  An
+ * overlap with |o| <= zero_overlap_tol counts as zero.  summary (2 device
+ * uint32, may be NULL): [0] = OR of every walker's flags, [1] = number of
+ * walkers with a flag other than AFQ_WALKER_ZERO_WEIGHT (a walker removed
+ * by the phaseless projection is not a numerical failure). */
+enum afq_walker_flag {
+  AFQ_WALKER_NONFINITE_PHI = 1,      /* NaN or inf in phi_a or phi_b */
+  AFQ_WALKER_ZERO_OVERLAP = 2,       /* |overlap| <= zero_overlap_tol */
+  AFQ_WALKER_NONFINITE_OVERLAP = 4,  /* overlap NaN or inf */
+  AFQ_WALKER_BAD_WEIGHT = 8,         /* weight NaN, inf or negative */
+  AFQ_WALKER_NONFINITE_EHYBRID = 16, /* hybrid energy NaN or inf */
+  AFQ_WALKER_ZERO_WEIGHT = 32        /* weight exactly 0 (informational) */
+};
+int afq_walker_status(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                      const double* phia, const double* phib, const double* overlap,
+                      const double* weights, const double* e_hybrid, double zero_overlap_tol,
+                      uint32_t* flags, uint32_t* summary);
+
 /* ---- counter-based auxiliary fields ------------------------------------ */
 
 /* xi[w,g] ~ N(0,1): Philox4x32-10 counter (g/2, w, step, 0), key = seed,
@@ -246,6 +270,35 @@ int afq_local_energy_multidet(afq_ctx* ctx, int nwalkers, int nbasis, int nup, i
                               const double* phia, const double* phib, const int32_t* occ_up,
                               const int32_t* occ_dn, const double* ci_coeffs, double enuc,
                               int det_chunk_size, double* e_tot, double* e_1b, double* e_2b);
+
+/* greens.py:314-369 greens_function_multidet_force_bias: every det's Ghalf
+ * -> Ghalfa_all (ndet,W,Na,M), Ghalfb_all (ndet,W,Nb,M) complex, non-finite
+ * entries set to 0 and no small-overlap mask; overlap (W,) = sum_I
+ * conj(c_I) o_Ia o_Ib over all dets; per-det spin overlaps ovlp_a_all,
+ * ovlp_b_all (ndet,W) complex (may be NULL).  Dets as occupation strings. */
+int afq_greens_function_multidet_force_bias(afq_ctx* ctx, int nwalkers, int nbasis, int nup,
+                                            int ndown, int ndet, const double* phia,
+                                            const double* phib, const int32_t* occ_up,
+                                            const int32_t* occ_dn, const double* ci_coeffs,
+                                            double* Ghalfa_all, double* Ghalfb_all,
+                                            double* overlap, double* ovlp_a_all,
+                                            double* ovlp_b_all);
+
+/* energy.py:318-395 local_energy_multidet_streamed (the memory-lean
+ * estimator of afqmc_gto_estream.py): per det, the one-body trace against
+ * h1e_trials (ndet,M,N_s) complex (h1e @ T_I^*) and the two-body energy of
+ * Ghalf_I with rchols (ndet,G,N_s,M) (dtype AFQ_REAL or AFQ_COMPLEX) in
+ * slabs of e_chunk_g vectors; weighted by conj(c_I) o_Ia o_Ib (per-det
+ * overlaps (ndet,W) complex) over their sum.  No (W,M,M) Green's function is
+ * formed.  Outputs (W,) complex. */
+int afq_local_energy_multidet_streamed(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                       int naux, int ndet, const double* h1e_trials_a,
+                                       const double* h1e_trials_b, const double* Ghalfa_all,
+                                       const double* Ghalfb_all, const double* rchols_a,
+                                       const double* rchols_b, int dtype, const double* ci_coeffs,
+                                       const double* ovlp_a_all, const double* ovlp_b_all,
+                                       double enuc, int e_chunk_g, double* e_tot, double* e_1b,
+                                       double* e_2b);
 
 /* System with a multi-determinant trial (afqmc_gto.py:1163-1225): the
  * propagator (expH1, mf_shift from the |c|^2-weighted charge density) is the

@@ -17,7 +17,11 @@ from .afqmc_gto import (  # noqa: F401
     local_energy_multidet, multidet_charge_density, occupations_from_trials,
     propagate_walkers_multidet)
 from ._lib import AFQError  # noqa: F401
+from .afqmc_gto import (greens_function_multidet_force_bias,  # noqa: F401
+                        half_rotate_cholesky_multidet, local_energy_multidet_streamed,
+                        walker_status, decode_walker_flags)
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
+from .estream import EStreamDriver, get_afqmc_estream_func_from_integrals  # noqa: F401
 from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401
 from .qed import (QEDDriver, get_qed_afqmc_func_from_integrals,  # noqa: F401
                   prepare_qed_integrals_from_arrays, propagate_qed_walkers, qed_local_energy)

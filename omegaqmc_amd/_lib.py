@@ -19,6 +19,10 @@ AFQ_OK = 0
 AFQ_FP64, AFQ_FP32, AFQ_BF16X3 = 0, 1, 2
 AFQ_REAL, AFQ_COMPLEX = 0, 1
 
+# per-walker status bits (enum afq_walker_flag)
+WALKER_FLAGS = {"nonfinite_phi": 1, "zero_overlap": 2, "nonfinite_overlap": 4, "bad_weight": 8,
+                "nonfinite_e_hybrid": 16, "zero_weight": 32}
+
 PRECISIONS = {"fp64": AFQ_FP64, "fp32": AFQ_FP32, "bf16x3": AFQ_BF16X3}
 
 _P = ctypes.c_void_p
@@ -57,6 +61,11 @@ SIGNATURES = {
     "afq_clip_weights": [_P, _I, _P, _D],
     "afq_accumulate_hybrid": [_P, _I, _P, _P, _P],
     "afq_block_stats": [_P, _I, _P, _P, _P],
+    "afq_greens_function_multidet_force_bias": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P,
+                                                _P, _P, _P, _P],
+    "afq_local_energy_multidet_streamed": [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I,
+                                           _P, _P, _P, _D, _I, _P, _P, _P],
+    "afq_walker_status": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P],
     "afq_aux_fields": [_P, _U64, _I64, _I, _I, _P],
     "afq_comb_offset": [_U64, _I64],
     "afq_system_create": [_P, ctypes.POINTER(_P), _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P,

@@ -614,6 +614,63 @@ def local_energy_multidet(h1e, chol, phia, phib, trials_up, trials_dn, ci_coeffs
     return tuple(out)
 
 
+def half_rotate_cholesky_multidet(chol, trials_up, trials_dn, chunk_g=None):
+    """cholesky.py:513-560 -> rchols_a (ndet, G, Na, M), rchols_b (real for
+    real trials; torch on the device, set-up only)."""
+    L = _r(chol)
+    out = []
+    for T in (trials_up, trials_dn):
+        T = T if isinstance(T, torch.Tensor) else torch.from_numpy(np.asarray(T))
+        T = T.to(device=L.device)
+        Tc = T.conj() if T.is_complex() else T.to(F64)
+        if Tc.is_complex():
+            r = torch.einsum('dpi,gpq->dgiq', Tc, L.to(C128))
+        else:
+            r = torch.einsum('dpi,gpq->dgiq', Tc, L)
+        out.append(r.contiguous())
+    return out[0], out[1]
+
+
+def greens_function_multidet_force_bias(phia, phib, trials_up, trials_dn, ci_coeffs):
+    """greens.py:314-369 -> (Ghalfa_all, Ghalfb_all, overlap, ovlp_a_all, ovlp_b_all)."""
+    pa, pb, ci = _c(phia), _c(phib), _c(ci_coeffs)
+    oa, ob = _occ(trials_up), _occ(trials_dn)
+    W, M, Na = pa.shape
+    Nb, nd = pb.shape[2], ci.shape[0]
+    dev = pa.device
+    Gha = torch.empty((nd, W, Na, M), dtype=C128, device=dev)
+    Ghb = torch.empty((nd, W, Nb, M), dtype=C128, device=dev)
+    ov = torch.empty((W,), dtype=C128, device=dev)
+    ova = torch.empty((nd, W), dtype=C128, device=dev)
+    ovb = torch.empty((nd, W), dtype=C128, device=dev)
+    check(_L().afq_greens_function_multidet_force_bias(
+        _ctx(), W, M, Na, Nb, nd, ptr(pa), ptr(pb), ptr(oa), ptr(ob), ptr(ci), ptr(Gha),
+        ptr(Ghb), ptr(ov), ptr(ova), ptr(ovb)), "afq_greens_function_multidet_force_bias")
+    return Gha, Ghb, ov, ova, ovb
+
+
+def local_energy_multidet_streamed(h1e_trials_a, h1e_trials_b, Ghalfa_all, Ghalfb_all, rchols_a,
+                                   rchols_b, ci_coeffs, ovlp_a_all, ovlp_b_all, enuc,
+                                   e_chunk_g=16):
+    """energy.py:318-395 -> (e_tot, e_1b, e_2b), no (W, M, M) Green's function."""
+    hta, htb = _c(h1e_trials_a), _c(h1e_trials_b)
+    Gha, Ghb = _c(Ghalfa_all), _c(Ghalfb_all)
+    ra, dta = _real_or_complex(rchols_a)
+    rb, dtb = _real_or_complex(rchols_b)
+    if dta != dtb:
+        ra, rb, dta = _c(ra), _c(rb), _lib.AFQ_COMPLEX
+    ci, oa, ob = _c(ci_coeffs), _c(ovlp_a_all), _c(ovlp_b_all)
+    nd, W, Na, M = Gha.shape
+    Nb = Ghb.shape[2]
+    G = ra.shape[1]
+    out = [torch.empty((W,), dtype=C128, device=Gha.device) for _ in range(3)]
+    check(_L().afq_local_energy_multidet_streamed(
+        _ctx(), W, M, Na, Nb, G, nd, ptr(hta), ptr(htb), ptr(Gha), ptr(Ghb), ptr(ra), ptr(rb), dta,
+        ptr(ci), ptr(oa), ptr(ob), float(enuc), int(e_chunk_g), ptr(out[0]), ptr(out[1]),
+        ptr(out[2])), "afq_local_energy_multidet_streamed")
+    return tuple(out)
+
+
 def multidet_charge_density(trials_up, trials_dn, ci_coeffs):
     """afqmc_gto.py:1209-1218: sum_d |c_d|^2 (T_d T_d^H)_up + (...)_dn / sum |c|^2."""
     tu, td, ci = _c(trials_up), _c(trials_dn), _c(ci_coeffs)
@@ -690,6 +747,34 @@ def propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagato
     next_key, k = rng_key.split() if isinstance(rng_key, PhiloxKey) else (rng_key, None)
     ov = sysobj.step(pa, pb, w, eh, eshift, xi=xi, key=k)
     return pa, pb, w, ov, eh, next_key
+
+
+# ---------------------------------------------------------------------------
+# Per-walker numerical status (no reference counterpart: the reference lets
+# NaN/inf propagate and masks only multi-det overlaps, greens.py:264-276)
+# ---------------------------------------------------------------------------
+
+def walker_status(phia, phib, overlap=None, weights=None, e_hybrid=None, zero_overlap_tol=1e-300):
+    """Per-walker flag words (int32 tensor (W,), bits of ``_lib.WALKER_FLAGS``)
+    and a device summary int32 (2,): [OR of all flags, number of walkers with
+    a flag other than zero_weight]."""
+    pa, pb = _c(phia), _c(phib)
+    W, M, Na = pa.shape
+    Nb = pb.shape[2]
+    ov = _c(overlap) if overlap is not None else None
+    w = _r(weights) if weights is not None else None
+    eh = _c(e_hybrid) if e_hybrid is not None else None
+    flags = torch.empty((W,), dtype=torch.int32, device=pa.device)
+    summary = torch.empty((2,), dtype=torch.int32, device=pa.device)
+    check(_L().afq_walker_status(_ctx(), W, M, Na, Nb, ptr(pa), ptr(pb), ptr(ov), ptr(w), ptr(eh),
+                                 float(zero_overlap_tol), ptr(flags), ptr(summary)),
+          "walker_status")
+    return flags, summary
+
+
+def decode_walker_flags(word):
+    """Names of the bits set in one flag word."""
+    return [k for k, v in _lib.WALKER_FLAGS.items() if int(word) & v]
 
 
 # ---------------------------------------------------------------------------

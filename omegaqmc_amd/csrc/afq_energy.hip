@@ -105,6 +105,39 @@ __global__ __launch_bounds__(NT) void k_e1b_half(int M, int Na, int Nb,
   if (threadIdx.x == 0) e1b[w] = mkc(sa.x + sb.x + enuc, sa.y + sb.y);
 }
 
+// energy.py:363-393, one det at a time: w_I = conj(c_I) o_Ia o_Ib;
+// acc1 (+)= w_I e1b_I, acc2 (+)= w_I (e_coul_I - e_exch_I)  (I == 0 writes)
+__global__ void k_md_stream_acc(int W, int I, const cplx* __restrict__ ci,
+                                const cplx* __restrict__ oa_all, const cplx* __restrict__ ob_all,
+                                const cplx* __restrict__ e1, const cplx* __restrict__ ec,
+                                const cplx* __restrict__ ex, cplx* __restrict__ acc1,
+                                cplx* __restrict__ acc2) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const cplx wi = cmul(cmul(cconj(ci[I]), oa_all[(long)I * W + w]), ob_all[(long)I * W + w]);
+  const cplx a = cmul(wi, e1[w]), b = cmul(wi, csub(ec[w], ex[w]));
+  acc1[w] = I ? cadd(acc1[w], a) : a;
+  acc2[w] = I ? cadd(acc2[w], b) : b;
+}
+
+// overlap = sum_I w_I; e_1b = acc1 / overlap + enuc; e_2b = acc2 / overlap
+__global__ void k_md_stream_final(int W, int nd, const cplx* __restrict__ ci,
+                                  const cplx* __restrict__ oa_all,
+                                  const cplx* __restrict__ ob_all, double enuc,
+                                  cplx* __restrict__ e_tot, cplx* __restrict__ e1b,
+                                  cplx* __restrict__ e2b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  cplx ov = mkc(0.0, 0.0);
+  for (int I = 0; I < nd; ++I)
+    ov = cadd(ov, cmul(cmul(cconj(ci[I]), oa_all[(long)I * W + w]), ob_all[(long)I * W + w]));
+  const cplx a = cadd(cdiv(e1b[w], ov), mkc(enuc, 0.0));
+  const cplx b = cdiv(e2b[w], ov);
+  e1b[w] = a;
+  e2b[w] = b;
+  e_tot[w] = cadd(a, b);
+}
+
 __global__ void k_combine(int W, const cplx* __restrict__ e1b, const cplx* __restrict__ ec,
                           const cplx* __restrict__ ex, cplx* __restrict__ etot,
                           cplx* __restrict__ e2b) {
@@ -312,6 +345,51 @@ int afq_local_energy_streamed(afq_ctx* ctx, int W, int M, int Na, int Nb, int G,
                      sc.slot[1]);
   if (rc) return rc;
   return afq::combine_launch(s, W, e_1b, sc.slot[0], sc.slot[1], e_tot, e_2b);
+}
+
+int afq_local_energy_multidet_streamed(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, int nd,
+                                       const double* hTa, const double* hTb, const double* Gha,
+                                       const double* Ghb, const double* ra, const double* rb,
+                                       int dtype, const double* ci, const double* oa_all,
+                                       const double* ob_all, double enuc, int e_chunk_g,
+                                       double* e_tot, double* e_1b, double* e_2b) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0 && nd > 0 && e_chunk_g >= 1,
+              "bad shape");
+  AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
+  AFQ_REQUIRE(e_tot && e_1b && e_2b && ci && oa_all && ob_all, "NULL argument");
+  AFQ_REQUIRE((Na == 0 || (hTa && Gha && (ra || G == 0))) && (Nb == 0 || (hTb && Ghb && (rb || G == 0))),
+              "NULL per-det array");
+  if (W == 0) return AFQ_OK;
+  hipStream_t s = ctx->stream;
+  afq::E2bScratch sc;
+  int rc = afq::e2b_scratch(ctx, W, Na, Nb, G, e_chunk_g, &sc);
+  if (rc) return rc;
+  const int rw = dtype == AFQ_COMPLEX ? 2 : 1;
+  for (int I = 0; I < nd; ++I) {
+    const double* ga = Gha ? Gha + 2L * I * W * Na * M : nullptr;
+    const double* gb = Ghb ? Ghb + 2L * I * W * Nb * M : nullptr;
+    // per-det one-body trace (enuc added once, at the end)
+    if ((rc = afq::e1b_half_launch(s, W, M, Na, Nb, hTa ? hTa + 2L * I * M * Na : nullptr,
+                                   hTb ? hTb + 2L * I * M * Nb : nullptr, ga, gb, 0.0,
+                                   sc.slot[2])))
+      return rc;
+    if ((rc = afq::two_body(s, sc, W, M, Na, Nb, G, ga, gb,
+                            ra ? ra + (long)rw * I * G * Na * M : nullptr,
+                            rb ? rb + (long)rw * I * G * Nb * M : nullptr, dtype == AFQ_COMPLEX,
+                            sc.slot[0], sc.slot[1])))
+      return rc;
+    hipLaunchKernelGGL(afq::k_md_stream_acc, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, I,
+                       (const cplx*)ci, (const cplx*)oa_all, (const cplx*)ob_all,
+                       (const cplx*)sc.slot[2], (const cplx*)sc.slot[0], (const cplx*)sc.slot[1],
+                       (cplx*)e_1b, (cplx*)e_2b);
+    AFQ_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(afq::k_md_stream_final, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W, nd,
+                     (const cplx*)ci, (const cplx*)oa_all, (const cplx*)ob_all, enuc,
+                     (cplx*)e_tot, (cplx*)e_1b, (cplx*)e_2b);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
 }
 
 }  // extern "C"

@@ -9,7 +9,9 @@ namespace afq {
 //   MD_INVERSE  every S_I^-1 and the scattered sum B_s = sum_I z_I T_I^* S_I^-1
 //               / overlap, from which G_s = B_s phi_s^T (one GEMM per spin);
 //   MD_GHALF    every Ghalf_I (the local energy needs them) and G.
-enum MdKind { MD_OVERLAP = 0, MD_INVERSE = 1, MD_GHALF = 2 };
+// MD_GHALF_RAW: every det's Ghalf and overlaps as MD_GHALF, without zeroing
+// the Ghalf of masked dets (greens.py:314-369 keeps them)
+enum MdKind { MD_OVERLAP = 0, MD_INVERSE = 1, MD_GHALF = 2, MD_GHALF_RAW = 3 };
 
 // Scratch of one multi-determinant evaluation for W walkers (bytes), and the
 // pieces carved out of it.

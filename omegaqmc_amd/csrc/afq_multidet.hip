@@ -366,7 +366,7 @@ size_t a256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 size_t md_bytes(int W, int M, int Na, int Nb, int nd, int kind, bool gfull) {
   size_t o = 0;
-  if (kind == MD_GHALF) {
+  if (kind == MD_GHALF || kind == MD_GHALF_RAW) {
     o += a256(16UL * nd * W * Na * M);
     o += a256(16UL * nd * W * Nb * M);
   }
@@ -391,7 +391,7 @@ MdBufs md_carve(void* base, int W, int M, int Na, int Nb, int nd, int kind, bool
     p += a256(n);
     return r;
   };
-  if (kind == MD_GHALF) {
+  if (kind == MD_GHALF || kind == MD_GHALF_RAW) {
     b.gha = (double*)take(16UL * nd * W * Na * M);
     b.ghb = (double*)take(16UL * nd * W * Nb * M);
   }
@@ -439,7 +439,7 @@ int md_greens(hipStream_t s, int W, int M, int Na, int Nb, int nd, int chunk, co
   const int Nmax = std::max(Na, Nb);
   if (Nmax > GJ_NMAX) return fail(AFQ_EUNSUPPORTED, "multi-det: nocc %d > %d", Nmax, GJ_NMAX);
   if (gsum && kind == MD_GHALF) return fail(AFQ_EINVAL, "multi-det: summed G needs MD_INVERSE");
-  const int mode = kind == MD_GHALF ? 1 : kind == MD_INVERSE ? 2 : 0;
+  const int mode = (kind == MD_GHALF || kind == MD_GHALF_RAW) ? 1 : kind == MD_INVERSE ? 2 : 0;
   // register row length: N rounded up to a multiple of 4 (every padding
   // column is a full update per elimination step)
   const int NB = Nmax <= 8 ? 8 : (Nmax + 3) / 4 * 4;
@@ -835,6 +835,38 @@ int system_energy_multidet(afq_system* sys, int W, const double* phia, const dou
                    sys->occ_b, enuc, b, scr + nb, e_tot, e_1b, e_2b, lp);
 }
 
+namespace {
+
+// greens.py:353-358: non-finite Ghalf entries (the singular S of a det the
+// walker is orthogonal to) become 0
+__global__ void k_md_sanitize(long n, cplx* __restrict__ g) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const cplx x = g[e];
+    if (!(isfinite(x.x) && isfinite(x.y))) g[e] = mkc(0.0, 0.0);
+  }
+}
+
+// greens.py:360-364: overlap = sum_I conj(c_I) o_Ia o_Ib over all dets in
+// order (no mask); per-det overlaps out in the (ndet, W) layout
+__global__ void k_md_fb_overlaps(int W, int nd, const cplx* __restrict__ ci,
+                                 const cplx* __restrict__ oda, const cplx* __restrict__ odb,
+                                 cplx* __restrict__ ovlp, cplx* __restrict__ oa_all,
+                                 cplx* __restrict__ ob_all) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  cplx acc = mkc(0.0, 0.0);
+  for (int I = 0; I < nd; ++I) {
+    const cplx a = oda[(long)w * nd + I], b = odb[(long)w * nd + I];
+    acc = cadd(acc, cmul(cmul(cconj(ci[I]), a), b));
+    if (oa_all) oa_all[(long)I * W + w] = a;
+    if (ob_all) ob_all[(long)I * W + w] = b;
+  }
+  ovlp[w] = acc;
+}
+
+}  // namespace
+
 }  // namespace afq
 
 // ---------------------------------------------------------------------------
@@ -882,6 +914,44 @@ int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int W, int M, int Na
   b.ovlp = overlap;
   return afq::md_greens(s, W, M, Na, Nb, nd, det_chunk_size, phia, phib, occ_up, occ_dn, ci,
                         afq::MD_OVERLAP, false, b);
+}
+
+int afq_greens_function_multidet_force_bias(afq_ctx* ctx, int W, int M, int Na, int Nb, int nd,
+                                            const double* phia, const double* phib,
+                                            const int32_t* occ_up, const int32_t* occ_dn,
+                                            const double* ci, double* Ghalfa_all,
+                                            double* Ghalfb_all, double* overlap,
+                                            double* ovlp_a_all, double* ovlp_b_all) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
+  AFQ_REQUIRE(phia && phib && ci && overlap, "NULL argument");
+  AFQ_REQUIRE((Ghalfa_all || Na == 0) && (Ghalfb_all || Nb == 0), "NULL Ghalf output");
+  hipStream_t s = ctx->stream;
+  int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
+  if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
+  if (rc || W == 0) return rc;
+  // the Ghalf planes are the caller's outputs; the rest comes from scratch
+  const size_t n = afq::md_bytes(W, M, Na, Nb, nd, afq::MD_OVERLAP, false);
+  void* scr = afq::scratch(ctx, n);
+  if (!scr) return AFQ_ENOMEM;
+  afq::MdBufs b = afq::md_carve(scr, W, M, Na, Nb, nd, afq::MD_OVERLAP, false);
+  b.gha = Ghalfa_all;
+  b.ghb = Ghalfb_all;
+  if ((rc = afq::md_greens(s, W, M, Na, Nb, nd, 1, phia, phib, occ_up, occ_dn, ci,
+                           afq::MD_GHALF_RAW, false, b)))
+    return rc;
+  for (int sp = 0; sp < 2; ++sp) {
+    const long ne = (long)nd * W * (sp ? Nb : Na) * M;
+    if (ne == 0) continue;
+    hipLaunchKernelGGL(afq::k_md_sanitize, dim3((unsigned)std::min<long>((ne + 255) / 256, 8192)),
+                       dim3(256), 0, s, ne, (cplx*)(sp ? Ghalfb_all : Ghalfa_all));
+    AFQ_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(afq::k_md_fb_overlaps, dim3(cdiv_int(W, 128)), dim3(128), 0, s, W, nd,
+                     (const cplx*)ci, (const cplx*)b.oda, (const cplx*)b.odb, (cplx*)overlap,
+                     (cplx*)ovlp_a_all, (cplx*)ovlp_b_all);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
 }
 
 int afq_local_energy_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, int nd,

@@ -21,8 +21,9 @@ from . import _lib
 from ._lib import check, ptr
 from .afqmc_gto import (C128, F64, MultiDetSystem, PhiloxKey, System, WEIGHT_CLIP_FRACTION, _r,
                         build_propagator, determinant_energy_from_integrals, do_binning_analysis,
+                        greens_function_multidet_overlap_only, greens_function_overlap,
                         half_rotate_cholesky, multidet_charge_density,
-                        prepare_afqmc_integrals_from_arrays)
+                        prepare_afqmc_integrals_from_arrays, walker_status)
 
 
 class AFQMCDriver:
@@ -122,8 +123,7 @@ class AFQMCDriver:
             check(L.afq_orthogonalize_walkers(ctx, W, M, Na, Nb, ptr(st['phia']), ptr(st['phib']),
                                               ptr(st['log_detR']), None), "orthogonalize")
         key = PhiloxKey(seed, step_count)
-        st['overlap'] = self.system.step(st['phia'], st['phib'], st['weights'], st['e_hybrid'],
-                                         eshift, key=key)
+        st['overlap'] = self._propagate(st, key, eshift)
         if step_count > 1:
             check(L.afq_clip_weights(ctx, W, ptr(st['weights']), WEIGHT_CLIP_FRACTION), "clip")
         if step_count % pop_control_freq == 0:
@@ -148,6 +148,11 @@ class AFQMCDriver:
         check(L.afq_accumulate_hybrid(ctx, W, ptr(st['weights']), ptr(st['e_hybrid']),
                                       ptr(st['acc'])), "accumulate")
 
+    def _propagate(self, st, key, eshift):
+        """The fused step of every walker in place; returns overlap_new."""
+        return self.system.step(st['phia'], st['phib'], st['weights'], st['e_hybrid'], eshift,
+                                key=key)
+
     def block_energy(self, st):
         """Mixed-estimator block energy (afqmc_gto.py:1466-1496); device values."""
         e_tot, _, _ = self.system.local_energy(st['phia'], st['phib'], self.h1e, self.enuc)
@@ -157,9 +162,23 @@ class AFQMCDriver:
                                 ptr(st['stats'])), "block_stats")
         return e_tot
 
+    def walker_status(self, st, zero_overlap_tol=1e-300):
+        """Per-walker flag words and summary (device) for the current
+        walkers, with their overlaps recomputed (the state's overlap is the
+        last step's, before any comb gather)."""
+        if self.multidet:
+            ov = greens_function_multidet_overlap_only(st['phia'], st['phib'], self.trials_up,
+                                                       self.trials_dn, self.ci_coeffs)
+        else:
+            ov = greens_function_overlap(st['phia'], st['phib'], self.system.trial_up,
+                                         self.system.trial_dn)
+        return walker_status(st['phia'], st['phib'], ov, st['weights'], st['e_hybrid'],
+                             zero_overlap_tol)
+
     def __call__(self, rng_key=None, num_walkers=100, num_blocks=100, num_steps_per_block=25,
                  stabilize_freq=5, pop_control_freq=5, num_blocks_equil=10, fname_log=None,
-                 walker_chunk_size=None, eshift0=None, local_energy_cap=False):
+                 walker_chunk_size=None, eshift0=None, local_energy_cap=False,
+                 check_status=False):
         seed = rng_key.seed if isinstance(rng_key, PhiloxKey) else (42 if rng_key is None
                                                                    else int(rng_key))
         fout = sys.stdout if not fname_log else open(fname_log, 'w', 1)
@@ -170,6 +189,7 @@ class AFQMCDriver:
         ebound = float(np.sqrt(2.0 / self.dt))
         step_count = 0
         energy_blocks = []
+        status_blocks = []
         n_capped = 0
         t_prev = time.perf_counter()
         for iblock in range(num_blocks_equil + num_blocks):
@@ -191,6 +211,13 @@ class AFQMCDriver:
                 stats = st['stats'].cpu().numpy()
                 e_block = float(stats[0])
             energy_blocks.append(e_block)
+            if check_status:
+                _, summ = self.walker_status(st)
+                summ = summ.cpu().numpy()
+                status_blocks.append((int(summ[0]) & 0xffffffff, int(summ[1])))
+                if self.verbose and summ[1] > 0:
+                    print(f"  block {iblock}: {int(summ[1])} walkers flagged "
+                          f"(flags {int(summ[0]):#x})", file=fout)
             acc = st['acc'].cpu().numpy()
             if acc[0] > 1e-10:
                 eshift = float(acc[1] / acc[0])
@@ -208,6 +235,9 @@ class AFQMCDriver:
         out = {'energy_blocks': energy_blocks, 'energy_mean': float(mean),
                'energy_err': float(err), 'energy_std': float(std), 'kappa': float(kappa),
                'ehf': self.ehf, 'state': st}
+        if check_status:
+            # per block: (OR of the walker flag words, walkers flagged)
+            out['status_blocks'] = status_blocks
         if local_energy_cap:
             out['local_energy_cap_bound'] = ebound
             out['n_local_energy_capped'] = n_capped

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
-Follows OmegaQMC/observables/greens_th4.py, observables/energy_th4.py and
+Follows OmegaQMC/observables/greens_th4.py, observables/energy_th4.py, the
+streamed pair of observables/greens.py:314-369 and energy.py:318-395, and
 afqmc_gto.py:728-839 (propagate_walkers_multidet); the trial set-up follows
 afqmc_gto.py:1163-1225.  As in the single-determinant oracle, the auxiliary
 fields ``xi`` are an argument instead of a jax.random draw.
@@ -18,6 +19,9 @@ Pins (tests/test_oracle_multidet.py):
                       exact ground state (all dets, FCI coefficients) it is
                       E_0 for every walker (zero variance).
   propagate_walkers_multidet  one determinant reproduces the single-det step.
+  local_energy_multidet_streamed (with greens_function_multidet_force_bias)
+                      equals local_energy_multidet (full G, chunked dets)
+                      for walkers whose dets are all above the mask.
 """
 
 from __future__ import annotations
@@ -166,6 +170,47 @@ def local_energy_multidet(h1e, chol, phia, phib, trials_up, trials_dn, ci_coeffs
     Gb = Gb / ovlp[:, None, None]
     e_1b = O.local_energy_1body(h1e, Ga, Gb, enuc)
     e_2b = e2b / ovlp
+    return e_1b + e_2b, e_1b, e_2b
+
+
+def half_rotate_cholesky_multidet(chol, trials_up, trials_dn):
+    """cholesky.py:470-510 -> rchols_a (ndet, G, Na, M), rchols_b."""
+    ra = np.einsum('dpi,gpq->dgiq', trials_up.conj(), chol)
+    rb = np.einsum('dpi,gpq->dgiq', trials_dn.conj(), chol)
+    return ra, rb
+
+
+def greens_function_multidet_force_bias(phia, phib, trials_up, trials_dn, ci_coeffs):
+    """greens.py:314-369 -> (Ghalfa_all, Ghalfb_all, overlap, ovlp_a_all,
+    ovlp_b_all): per-det Ghalf with non-finite entries set to 0, no
+    small-overlap mask, one sum over all dets."""
+    ga, oa = zip(*[_gf_spin_single_det(phia, t) for t in trials_up])
+    gb, ob = zip(*[_gf_spin_single_det(phib, t) for t in trials_dn])
+    Gha, Ghb = np.stack(ga), np.stack(gb)
+    Gha = np.where(np.isfinite(Gha), Gha, 0.0)
+    Ghb = np.where(np.isfinite(Ghb), Ghb, 0.0)
+    oa, ob = np.stack(oa), np.stack(ob)
+    ovlp = np.sum(ci_coeffs.conj()[:, None] * oa * ob, axis=0)
+    return Gha, Ghb, ovlp, oa, ob
+
+
+def local_energy_multidet_streamed(h1e_trials_a, h1e_trials_b, Ghalfa_all, Ghalfb_all, rchols_a,
+                                   rchols_b, ci_coeffs, ovlp_a_all, ovlp_b_all, enuc,
+                                   e_chunk_g=16):
+    """energy.py:318-395 -> (e_tot, e_1b, e_2b): per-det one-body trace
+    against the precontracted h1e_trials (ndet, M, N), per-det streamed
+    two-body, both weighted by c_I^* o_Ia o_Ib over the summed overlap."""
+    w = ci_coeffs.conj()[:, None] * ovlp_a_all * ovlp_b_all
+    ovlp = np.sum(w, axis=0)
+    e1 = (np.einsum('dpi,dwip->dw', h1e_trials_a, Ghalfa_all)
+          + np.einsum('dpi,dwip->dw', h1e_trials_b, Ghalfb_all))
+    e_1b = np.sum(w * e1, axis=0) / ovlp + enuc
+    e2 = []
+    for I in range(rchols_a.shape[0]):
+        ec, ex = O.local_energy_2body_streamed(Ghalfa_all[I], Ghalfb_all[I], rchols_a[I],
+                                               rchols_b[I], e_chunk_g)
+        e2.append(ec - ex)
+    e_2b = np.sum(w * np.stack(e2), axis=0) / ovlp
     return e_1b + e_2b, e_1b, e_2b
 
 

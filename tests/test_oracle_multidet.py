@@ -147,3 +147,40 @@ def test_orthogonal_dets_are_masked():
     e, _, _ = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc)
     ehf = O.determinant_energy_from_integrals(s.h1e, s.chol, s.enuc, NA, NB)
     np.testing.assert_allclose(e.real, ehf, rtol=1e-12)
+
+
+@pytest.mark.parametrize("seed,e_chunk_g", [(0, 3), (1, 16)])
+def test_streamed_energy_equals_brute_force(seed, e_chunk_g):
+    """energy.py:318-395 with greens.py:314-369: the per-det one-body trace
+    against h1e @ T_I^*, the per-det slabbed two-body, weighted by
+    c_I^* o_Ia o_Ib, equals <Psi_T|H|phi>/<Psi_T|phi> from the brute-force
+    Hamiltonian.  A walker exactly orthogonal to one det (singular S:
+    non-finite Ghalf, zeroed, and a zero weight) drops that det's
+    <D_I|H|phi>, as the masked full-G estimator does: there the two agree."""
+    s = _sys(seed)
+    rng = np.random.default_rng(20 + seed)
+    occ_up, occ_dn = _dets(6, rng)
+    ci = rng.normal(size=6) + 0.3j * rng.normal(size=6)
+    tu, td = MD.trials_from_occ(M, occ_up), MD.trials_from_occ(M, occ_dn)
+    pa, pb = _walkers(4, rng)
+    # walker 3: alpha orbitals without any weight on det 1's first orbital row
+    pa[3, occ_up[1][0], :] = 0.0
+    pa[3, occ_up[1][1], :] = 0.0
+    Gha, Ghb, ov, oa, ob = MD.greens_function_multidet_force_bias(pa, pb, tu, td, ci)
+    assert oa[1, 3] == 0.0 and np.all(np.isfinite(Gha))
+    ra, rb = MD.half_rotate_cholesky_multidet(s.chol, tu, td)
+    hta = np.einsum('pq,dqi->dpi', s.h1e, tu.conj())
+    htb = np.einsum('pq,dqi->dpi', s.h1e, td.conj())
+    e, e1, e2 = MD.local_energy_multidet_streamed(hta, htb, Gha, Ghb, ra, rb, ci, oa, ob, s.enuc,
+                                                  e_chunk_g)
+    H = hamiltonian(s.h1e_mod, s.chol, s.enuc, M, NA, NB)
+    t = _trial_vector(occ_up, occ_dn, ci)
+    for w in range(4):
+        v = np.kron(det_vector(pa[w]), det_vector(pb[w]))
+        assert abs(ov[w] - np.vdot(t, v)) < 1e-12 * abs(np.vdot(t, v))
+        if w < 3:
+            ref = np.vdot(t, H @ v) / np.vdot(t, v)
+            assert abs(e[w] - ref) < 1e-9 * abs(ref), w
+    em, e1m, e2m = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc)
+    np.testing.assert_allclose(e, em, rtol=1e-12)
+    np.testing.assert_allclose(e1, e1m, rtol=1e-12)
