@@ -29,6 +29,11 @@ int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, co
                        int nmax, double* phi_out, double* tmp0, double* tmp1);
 int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const double* phi,
                   double* out);
+// expH1 . exp(VHS) . expH1 by batched GEMMs (any M; scratch: a_s, b_s of
+// W M N_s complex, t0, t1 of W M max(N_a, N_b) complex)
+int prop_generic_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
+                     const double* VHS, double* pa, double* pb, double* a1, double* a2,
+                     double* b1, double* b2, double* t0, double* t1, int nmax);
 // eshift_dev (optional): device energy shift used instead of eshift
 int weights_launch(hipStream_t s, int W, const double* w, const double* oo, const double* on,
                    const double* cfb, const double* cmf, const double* eh, double eshift, double dt,

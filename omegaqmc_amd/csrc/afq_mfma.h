@@ -31,6 +31,8 @@ int prop_mfma_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* exp
                   const double* VHS, const double* pa_in, const double* pb_in, double* pa_out,
                   double* pb_out, int nmax);
 size_t prop_f64_smem(int M);
+// the fused FP64 propagator handles M (its column block fits LDS and M <= 512)
+bool prop_f64_fused_fits(int M);
 // phi_out[w] = sum_{n<=nmax} (coef[w] mat)^n phi_in[w] / n! (FP64; coef (W,) and
 // mat (M, M) real, phi (W, M, N) complex); phi_out may alias phi_in.
 int prop_scaled_f64(hipStream_t s, int W, int M, int N, const double* mat, const double* coef,

@@ -565,6 +565,7 @@ int fb_mfma_f32(hipStream_t s, int W, int G, const FbSplits& sp, double* part) {
 }
 
 size_t prop_f64_smem(int M) { return (size_t)((M + 15) / 16) * 16 * 16 * sizeof(cplx); }
+bool prop_f64_fused_fits(int M) { return M <= 512 && prop_f64_smem(M) <= max_lds_bytes(); }
 
 template <class T>
 static int prop_launch(hipStream_t s, int W, int M, int Na, int Nb, const T* E, const void* VHS,

@@ -793,7 +793,23 @@ int propagate_multidet(afq_system* sys, int W, double* phia, double* phib, doubl
     phase_end(sys, PH_VHS);
     if (rc) return rc;
     phase_begin(sys, PH_TAYLOR);
-    rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, st.vhs, phia, phib, phia, phib, sys->nmax);
+    if (prop_f64_fused_fits(M)) {
+      rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, st.vhs, phia, phib, phia, phib, sys->nmax);
+    } else {
+      // batched-GEMM propagator; its scratch lives in the (W, M, M) G^T slot,
+      // free once the force bias is formed (2 (Na + Nb + max N) <= M here)
+      const long Nm = std::max(Na, Nb);
+      if (2L * (Na + Nb + Nm) > M) return fail(AFQ_EUNSUPPORTED, "multi-det propagator scratch");
+      double* g = st.Gt;
+      double* a1 = g;
+      double* a2 = a1 + 2L * W * M * Na;
+      double* b1 = a2 + 2L * W * M * Na;
+      double* b2 = b1 + 2L * W * M * Nb;
+      double* t0 = b2 + 2L * W * M * Nb;
+      double* t1 = t0 + 2L * W * M * Nm;
+      rc = prop_generic_f64(s, W, M, Na, Nb, sys->expH1, st.vhs, phia, phib, a1, a2, b1, b2, t0,
+                            t1, sys->nmax);
+    }
     phase_end(sys, PH_TAYLOR);
     if (rc) return rc;
   }

@@ -197,6 +197,26 @@ int apply_exp_vhs_fp64(hipStream_t s, int W, int M, int N, const double* VHS, co
   return AFQ_OK;
 }
 
+// expH1 . exp(VHS) . expH1 on both spins through the batched GEMMs, for
+// orbital counts whose column block does not fit the fused kernel's LDS.
+// phi_s is overwritten; a_s, b_s (W M N_s complex) and t0, t1 (W M max N_s
+// complex) are scratch.
+int prop_generic_f64(hipStream_t s, int W, int M, int Na, int Nb, const double* expH1,
+                     const double* VHS, double* pa, double* pb, double* a1, double* a2,
+                     double* b1, double* b2, double* t0, double* t1, int nmax) {
+  for (int sp = 0; sp < 2; ++sp) {
+    const int N = sp ? Nb : Na;
+    double* phi = sp ? pb : pa;
+    double* x = sp ? b1 : a1;
+    double* y = sp ? b2 : a2;
+    int rc = one_body_fp64(s, W, M, N, expH1, phi, x);
+    if (!rc) rc = apply_exp_vhs_fp64(s, W, M, N, VHS, x, nmax, y, t0, t1);
+    if (!rc) rc = one_body_fp64(s, W, M, N, expH1, y, phi);
+    if (rc) return rc;
+  }
+  return AFQ_OK;
+}
+
 int one_body_fp64(hipStream_t s, int W, int M, int N, const double* expH1, const double* phi,
                   double* out) {
   if (N == 0 || W == 0) return AFQ_OK;
@@ -234,7 +254,7 @@ int afq_apply_exp_vhs(afq_ctx* ctx, int W, int M, int N, const double* VHS, cons
                       int nmax, double* phi_out) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && N >= 0 && nmax >= 0, "bad shape");
-  if (M <= 512 && afq::prop_f64_smem(M) <= afq::max_lds_bytes())
+  if (afq::prop_f64_fused_fits(M))
     return afq::prop_mfma_f64(ctx->stream, W, M, N, 0, nullptr, VHS, phi, nullptr, phi_out,
                               nullptr, nmax);
   size_t one = align256(sizeof(double) * 2 * (size_t)W * M * N);

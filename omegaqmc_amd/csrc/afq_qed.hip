@@ -71,6 +71,15 @@ __global__ void k_qed_energy(int W, const double* __restrict__ q, double omega, 
   e_tot[i] = cadd(cadd(e1, e_2b[i]), mkc(ph, 0.0));
 }
 
+// V[w] = coef[w] * mat as complex (the batched-GEMM form of the scaled
+// exponential, for M beyond the fused propagator)
+__global__ void k_scaled_matrix(long n, long MM, const double* __restrict__ coef,
+                                const double* __restrict__ mat, cplx* __restrict__ V) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x)
+    V[e] = mkc(coef[e / MM] * mat[e % MM], 0.0);
+}
+
 }  // namespace
 
 }  // namespace afq
@@ -95,7 +104,27 @@ int afq_apply_exp_scaled(afq_ctx* ctx, int W, int M, int N, const double* coef, 
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && N >= 0 && nmax >= 0, "bad shape");
   AFQ_REQUIRE(W == 0 || N == 0 || (coef && mat && phi && phi_out), "NULL array");
-  return afq::prop_scaled_f64(ctx->stream, W, M, N, mat, coef, phi, phi_out, nmax);
+  if (W == 0 || N == 0) return AFQ_OK;
+  if (afq::prop_f64_fused_fits(M))
+    return afq::prop_scaled_f64(ctx->stream, W, M, N, mat, coef, phi, phi_out, nmax);
+  // batched GEMMs on the explicit (W, M, M) matrices; an aliased phi is
+  // copied first (the Taylor sum accumulates into phi_out while reading phi)
+  const size_t nv = align256(16UL * W * M * M), nt = align256(16UL * W * M * N);
+  char* scr = (char*)afq::scratch(ctx, nv + 3 * nt);
+  if (!scr) return AFQ_ENOMEM;
+  const long n = (long)W * M * M;
+  hipLaunchKernelGGL(afq::k_scaled_matrix, dim3((unsigned)std::min<long>((n + 255) / 256, 16384)),
+                     dim3(256), 0, ctx->stream, n, (long)M * M, coef, mat, (cplx*)scr);
+  AFQ_CHECK_LAUNCH();
+  const double* src = phi;
+  if (phi_out == phi) {
+    double* c = (double*)(scr + nv);
+    int rc = afq::copy_async(ctx->stream, phi, c, 2L * W * M * N);
+    if (rc) return rc;
+    src = c;
+  }
+  return afq::apply_exp_vhs_fp64(ctx->stream, W, M, N, (const double*)scr, src, nmax, phi_out,
+                                 (double*)(scr + nv + nt), (double*)(scr + nv + 2 * nt));
 }
 
 int afq_qed_photon_weights(afq_ctx* ctx, int W, double* weights, const double* log_w1,

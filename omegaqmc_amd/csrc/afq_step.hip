@@ -270,7 +270,6 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
                      gha, ghb, oo, nullptr, work_slot(sys, W, INV));
   phase_end(sys, PH_GREENS);
   if (rc) return rc;
-  (void)a1; (void)a2; (void)b1; (void)b2; (void)t0; (void)t1;
   // 3 (the first one-body half step (2) is fused into the propagator below:
   //    it does not feed the force bias, which uses the pre-step Ghalf)
   phase_begin(sys, PH_FORCEBIAS);
@@ -293,7 +292,11 @@ int propagate_fp64(afq_system* sys, int W, double* phia, double* phib, double* w
   if (rc) return rc;
   // 2, 5, 6: expH1 . exp(VHS) . expH1, in place
   phase_begin(sys, PH_TAYLOR);
-  rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, vhs, phia, phib, phia, phib, sys->nmax);
+  if (prop_f64_fused_fits(M))
+    rc = prop_mfma_f64(s, W, M, Na, Nb, sys->expH1, vhs, phia, phib, phia, phib, sys->nmax);
+  else
+    rc = prop_generic_f64(s, W, M, Na, Nb, sys->expH1, vhs, phia, phib, a1, a2, b1, b2, t0, t1,
+                          sys->nmax);
   phase_end(sys, PH_TAYLOR);
   if (rc) return rc;
   // 7
