@@ -219,10 +219,10 @@ def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10, trial
         "block_end_capped_err_mHa": {str(k): mha(c) for k, _, c, _, _ in ends},
         "trace_err_mHa": {str(k): mha(d) for k, d, _, _, _ in r['trace']},
         "fp64_walkers_beyond_cap": {str(k): n for k, _, _, n, _ in r['trace']},
-        # the blocks in which the FP64 population has no walker further than
-        # sqrt(2/dt) from the median energy scale (see DESIGN: the mixed
-        # estimator's sensitivity to any perturbation grows by orders of
-        # magnitude once hundreds of walkers sit in its tails)
+        # the first two blocks; later the FP64 local energies spread over
+        # many times sqrt(2/dt) (fp64_walkers_beyond_cap, and the statistical
+        # error block_end_fp64_stat_err_mHa), and the mixed estimator then
+        # magnifies any perturbation of the walkers, rounding included
         "max_err_first_50_steps_mHa": mha(max((d for k, d, *_ in r['trace'] if k <= 50),
                                               default=0.0)),
         "max_block_err_mHa": mha(r['max_block_raw']),
@@ -449,7 +449,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(walker sharding over GPUs is designed, not built)"}))
         sys.exit(2)
 
     import torch
