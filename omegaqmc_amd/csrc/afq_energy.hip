@@ -6,9 +6,11 @@
 //   T_s[g] = Lhalf_s[g] Ghalf_s^T  (N x N)
 //   E_exch = 1/2 sum_s sum_g sum_ij T_ij T_ji  (= Tr(T T), not Tr(T T^T))
 //
-// FP64 exchange: g is processed in slabs; each slab's T block for all
-// walkers is one GEMM (Lhalf slab (gs N x M) times Ghalf^T (M x W N)) into
-// scratch, contracted per walker by k_exch_contract in a fixed order.
+// FP64 exchange: with real Lhalf and N <= 40 the fused MFMA kernel of
+// afq_exch.hip (T in registers and LDS only); otherwise g is processed in
+// slabs, each slab's T block for all walkers one GEMM (Lhalf slab (gs N x M)
+// times Ghalf^T (M x W N)) into scratch, contracted per walker by
+// k_exch_contract in a fixed order.
 #include "afq_gemm.h"
 #include "afq_kernels.h"
 
@@ -160,6 +162,9 @@ int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf
     if (!acc) AFQ_CHECK_HIP(hipMemsetAsync(e_exch, 0, sizeof(cplx) * W, s));
     return AFQ_OK;
   }
+  if (!rchol_cplx && exchange_f64_fused_fits(N))
+    return exchange_f64_fused(s, W, M, N, G, Ghalf, rchol, partial,
+                              16UL * (size_t)gchunk * N * W * N, e_exch, accumulate, sign);
   for (int g0 = 0; g0 < G; g0 += gchunk) {
     int gs = std::min(gchunk, G - g0);
     GemmArgs ga;

@@ -66,6 +66,14 @@ int block_stats_launch(hipStream_t s, int W, const double* weights, const double
 int accumulate_launch(hipStream_t s, int W, const double* weights, const double* eh, double* acc);
 
 // energy (afq_energy.hip)
+// fused FP64 MFMA exchange (real Lhalf, N <= EXCH_F64_NMAX): T stays in
+// registers and LDS; partial: per (row tile, walker) sums, partial_bytes of
+// them (the row tiles run in as many launches as they need)
+constexpr int EXCH_F64_NMAX = 40;
+bool exchange_f64_fused_fits(int N);
+int exchange_f64_fused(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
+                       const double* rchol, double* partial, size_t partial_bytes,
+                       double* e_exch, bool accumulate, const double* sign);
 int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf,
                   const double* rchol, bool rchol_cplx, int gchunk, double* partial,
                   double* e_exch, bool accumulate, const double* sign = nullptr);
