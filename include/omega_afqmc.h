@@ -358,6 +358,15 @@ int afq_system_set_chol_sign(afq_system* sys, const double* chol_sign);
  * autotuner.  nsplit in {2, 4, 8} (BF16X3: {2, 4}); default 4. */
 int afq_system_set_force_bias_splits(afq_system* sys, int nsplit);
 
+/* Widest column block of the BF16X3 propagator (no counterpart in the
+ * reference): a walker's Na + Nb columns are cut into ceil((Na + Nb) / cols)
+ * equal workgroup blocks; wider blocks reuse the walker's VHS over more
+ * columns, narrower ones give more workgroups to small systems.  cols in
+ * {16, 32, 48}, default 48; the autotuner's second knob.  Other precisions
+ * accept and ignore it (their propagators use 16-column blocks).  Set it
+ * before a batch captures the system's steps (the workspace is re-laid). */
+int afq_system_set_propagator_block(afq_system* sys, int cols);
+
 /* afqmc_gto.py:630-725 propagate_walkers(...) for one step, in place on
  * phia/phib/weights/e_hybrid; overlap_new (W,) complex is written.  The
  * reference ignores its `overlap` argument, so there is none here.  xi (W,G)

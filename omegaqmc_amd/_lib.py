@@ -65,6 +65,7 @@ SIGNATURES = {
                                                 _P, _P, _P, _P],
     "afq_local_energy_multidet_streamed": [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I,
                                            _P, _P, _P, _D, _I, _P, _P, _P],
+    "afq_system_set_propagator_block": [_P, _I],
     "afq_walker_status": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P],
     "afq_aux_fields": [_P, _U64, _I64, _I, _I, _P],
     "afq_comb_offset": [_U64, _I64],

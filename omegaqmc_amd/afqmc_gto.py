@@ -465,6 +465,12 @@ class System:
                                          ptr(ov)), "afq_propagate_walkers")
         return ov
 
+    def set_propagator_block(self, cols):
+        """Widest column block of the BF16x3 propagator (16, 32 or 48; other
+        precisions ignore it): a tuning knob, the results do not change."""
+        check(_L().afq_system_set_propagator_block(self.handle, int(cols)),
+              "afq_system_set_propagator_block")
+
     def local_energy(self, phia, phib, h1e, enuc):
         _ctx()
         W = phia.shape[0]

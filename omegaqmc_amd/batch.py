@@ -11,7 +11,9 @@ driver cycle of the whole batch is a single hipGraph launch
 (``afq_batch_run`` mode 2).
 
 :func:`autotune` picks, per (M, N, Nchol, precision) bucket, the K-split
-count of the force-bias GEMM by timing the candidates on the bucket's first
+count of the force-bias GEMM and then the BF16x3 propagator's column block
+(wide blocks reuse a walker's VHS over more columns, narrow ones give small
+systems more workgroups) by timing the candidates on the bucket's first
 system, then the number of streams for the whole batch, assigning systems to
 streams longest-first by their measured step time.  Choices are cached in a
 JSON file keyed by bucket and device name, so a second run reuses them.
@@ -33,6 +35,8 @@ from ._lib import check, ptr
 
 MODES = {"sequential": 0, "streams": 1, "graph": 2}
 FB_SPLITS = {"fp64": (2, 4, 8), "fp32": (2, 4, 8), "bf16x3": (2, 4)}
+# widest propagator column block (afq_system_set_propagator_block; BF16x3 only)
+PROP_BLOCKS = {"fp64": (48,), "fp32": (48,), "bf16x3": (48, 32, 16)}
 STREAM_CHOICES = (1, 2, 4)
 DEFAULT_CACHE = pathlib.Path(os.environ.get(
     "OMEGAQMC_AMD_TUNE_CACHE", pathlib.Path.home() / ".cache" / "omegaqmc_amd" / "tune.json"))
@@ -140,9 +144,10 @@ def _time_steps(systems, walkers, nsteps, mode, nstreams=1, stream_of=None, seed
 
 
 def autotune(systems, walkers, precisions, cache=DEFAULT_CACHE, nsteps=10, verbose=False):
-    """Choose the force-bias K splits per bucket and the batch's stream count
-    and stream assignment.  Returns {'fb_splits': {bucket: n}, 'nstreams': k,
-    'stream_of': [...], 'step_ms': [...]} and applies the splits to the
+    """Choose the force-bias K splits and the propagator column block per
+    bucket and the batch's stream count and stream assignment.  Returns
+    {'fb_splits': {bucket: n}, 'prop_block': {bucket: cols}, 'nstreams': k,
+    'stream_of': [...], 'step_ms': [...]} and applies the choices to the
     systems."""
     cache = pathlib.Path(cache)
     dev = _device_name()
@@ -154,24 +159,39 @@ def autotune(systems, walkers, precisions, cache=DEFAULT_CACHE, nsteps=10, verbo
         buckets.setdefault(bucket_key(s, p), []).append(i)
     choice = {}
     for key, idx in buckets.items():
-        if key in entries:
-            choice[key] = int(entries[key]["fb_splits"])
+        if key in entries and "prop_block" in entries[key]:
+            choice[key] = (int(entries[key]["fb_splits"]), int(entries[key]["prop_block"]))
             continue
         i0 = idx[0]
-        s0, w0 = systems[i0], walkers[i0]
+        s0, w0, p0 = systems[i0], walkers[i0], precisions[i0]
+        # the two knobs in turn: force-bias K splits, then the propagator's
+        # column block with the chosen splits
         best, best_t = None, None
-        for n in FB_SPLITS[precisions[i0]]:
+        for n in FB_SPLITS[p0]:
             check(L.afq_system_set_force_bias_splits(s0.handle, n), "set splits")
+            check(L.afq_system_set_propagator_block(s0.handle, PROP_BLOCKS[p0][0]), "set block")
             t = _time_steps([s0], [w0], nsteps, "graph")
             if verbose:
                 print(f"[autotune] {key} W={w0} fb_splits={n}: {t * 1e3:.3f} ms/step")
             if best_t is None or t < best_t:
-                best, best_t = n, t
+                best, best_t = (n, PROP_BLOCKS[p0][0]), t
+        check(L.afq_system_set_force_bias_splits(s0.handle, best[0]), "set splits")
+        for c in PROP_BLOCKS[p0][1:]:
+            check(L.afq_system_set_propagator_block(s0.handle, c), "set block")
+            t = _time_steps([s0], [w0], nsteps, "graph")
+            if verbose:
+                print(f"[autotune] {key} W={w0} prop_block={c}: {t * 1e3:.3f} ms/step")
+            if t < best_t:
+                best, best_t = (best[0], c), t
         choice[key] = best
-        entries[key] = {"fb_splits": best, "step_ms": round(best_t * 1e3, 4), "walkers": w0}
+        entries[key] = {"fb_splits": best[0], "prop_block": best[1],
+                        "step_ms": round(best_t * 1e3, 4), "walkers": w0}
     for key, idx in buckets.items():
         for i in idx:
-            check(L.afq_system_set_force_bias_splits(systems[i].handle, choice[key]), "set splits")
+            check(L.afq_system_set_force_bias_splits(systems[i].handle, choice[key][0]),
+                  "set splits")
+            check(L.afq_system_set_propagator_block(systems[i].handle, choice[key][1]),
+                  "set block")
     # per-system step time with the chosen tiles, for the stream assignment
     step_ms = [_time_steps([s], [w], nsteps, "graph") * 1e3 for s, w in zip(systems, walkers)]
     best_ns, best_t, best_assign = None, None, None
@@ -194,5 +214,7 @@ def autotune(systems, walkers, precisions, cache=DEFAULT_CACHE, nsteps=10, verbo
             json.dump(db, f, indent=1, sort_keys=True)
     except OSError:
         pass
-    return {"fb_splits": choice, "nstreams": best_ns, "stream_of": best_assign,
+    return {"fb_splits": {k: v[0] for k, v in choice.items()},
+            "prop_block": {k: v[1] for k, v in choice.items()}, "nstreams": best_ns,
+            "stream_of": best_assign,
             "step_ms": step_ms, "batch_ms_per_step": best_t * 1e3}

@@ -23,6 +23,9 @@ struct Bf3System {
   bf16_t* Lhb;
   // expH1 as [plane][16*RT][Kp]
   bf16_t* E;
+  // widest column block of the propagator (16, 32 or 48): a walker's
+  // concatenated spin columns go to ceil(Na + Nb / pcols) workgroups
+  int pcols = 48;
 };
 
 int bf3_prepare(afq_system* s, Bf3System* b);

@@ -910,14 +910,15 @@ int bf3_packed_force_bias(hipStream_t s, int W, int G, int Kp, const bf16_t* gpl
 }
 
 size_t bf3_prop_scratch_bytes(const Bf3System& b, int W) {
-  const long ncb = (b.Na + b.Nb + PC - 1) / PC;
+  const long ncb = (b.Na + b.Nb + b.pcols - 1) / b.pcols;
   return (size_t)W * ncb * b.RT * 3 * 64 * 2 * sizeof(float4);
 }
 
 int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, float* t0,
                   double* phia, double* phib, int nmax) {
   const int Nt = b.Na + b.Nb;
-  const int nblk = (Nt + PC - 1) / PC;
+  if (b.pcols < 16 || b.pcols > PC) return fail(AFQ_EINVAL, "bf16x3 propagator: block %d", b.pcols);
+  const int nblk = (Nt + b.pcols - 1) / b.pcols;
   if (W <= 0 || nblk == 0) return AFQ_OK;
   const int bs = (Nt + nblk - 1) / nblk;
   const size_t sm = (size_t)4 * PC * (b.Kp + 8) * sizeof(bf16_t);

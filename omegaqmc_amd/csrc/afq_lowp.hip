@@ -104,6 +104,19 @@ int system_prepare_lowp(afq_system* sys) {
   return bf3_prepare(sys, b);
 }
 
+// the workspace is re-laid on the next step (its propagator scratch depends
+// on the column blocking)
+int lowp_invalidate(afq_system* sys) {
+  LowpWork* L = (LowpWork*)sys->rchol_lp;
+  if (L && L->buf) {
+    AFQ_CHECK_HIP(hipStreamSynchronize(sys->ctx->stream));
+    AFQ_CHECK_HIP(hipFree(L->buf));
+    L->buf = nullptr;
+    L->cap = 0;
+  }
+  return AFQ_OK;
+}
+
 void system_release_lowp(afq_system* sys) {
   if (sys->precision == AFQ_FP32) {
     f32_release(sys);
@@ -224,3 +237,15 @@ int system_energy_lowp(afq_system* sys, int W, const double* phia, const double*
 }
 
 }  // namespace afq
+
+extern "C" int afq_system_set_propagator_block(afq_system* sys, int cols) {
+  AFQ_REQUIRE(sys, "system is NULL");
+  AFQ_REQUIRE(cols == 16 || cols == 32 || cols == 48, "propagator block %d (16, 32 or 48)", cols);
+  if (sys->precision != AFQ_BF16X3 || !sys->chol_lp) return AFQ_OK;  // fixed 16-column blocks
+  afq::Bf3System* b = (afq::Bf3System*)sys->chol_lp;
+  if (b->pcols == cols) return AFQ_OK;
+  int rc = afq::lowp_invalidate(sys);
+  if (rc) return rc;
+  b->pcols = cols;
+  return AFQ_OK;
+}

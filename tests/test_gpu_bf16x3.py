@@ -151,3 +151,33 @@ def test_lockstep_energy_full_length(afq, workload):
     assert len(r['block_ends']) == 4
     assert all(d < 0.1e-3 for k, d, *_ in r['trace'] if k <= 50), r['trace']
     assert all(d < 0.1 * sg for k, d, _, _, sg in r['block_ends'] if k > 50), r['block_ends']
+
+
+@pytest.mark.parametrize("M,Na,Nb,G,W", [(40, 5, 4, 120, 9), (64, 20, 19, 200, 5)])
+def test_propagator_block_knob_same_result(afq, M, Na, Nb, G, W):
+    """The propagator's column blocking (16, 32 or 48 columns per workgroup)
+    changes which workgroup multiplies a column, not the products: one step
+    gives bit-identical walkers and overlaps for every block width, and stays
+    within the BF16x3 bound of the oracle."""
+    s, d64, dbf = _drivers(M, Na, Nb, G)
+    od = O.Driver(s.h1e, s.h1e_mod, s.chol, s.enuc, Na, Nb)
+    rng = np.random.default_rng(8)
+    pa = od.trial_up + 0.1 * (rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na)))
+    pb = od.trial_dn + 0.1 * (rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb)))
+    xi = P.aux_fields(6, 1, W, G)
+    eshift = od.ehf - od.e_const
+    ref = O.propagate_walkers(pa, pb, np.ones(W), None, np.zeros(W, complex), od.propagator,
+                              od.chol, od.rchol_a, od.rchol_b, od.trial_up, od.trial_dn, eshift,
+                              xi)
+    outs = []
+    for cols in (48, 32, 16, 48):
+        dbf.system.set_propagator_block(cols)
+        ta, tb, tw, te = _t(pa), _t(pb), _t(np.ones(W)), _t(np.zeros(W, complex))
+        ov = dbf.system.step(ta, tb, tw, te, eshift, xi=_t(xi))
+        outs.append((_n(ta), _n(tb), _n(ov), _n(tw)))
+        assert rel_err(outs[-1][0], ref[0]) < 1e-4 and rel_err(outs[-1][1], ref[1]) < 1e-4
+    for o in outs[1:]:
+        for x, y in zip(o, outs[0]):
+            assert np.array_equal(x, y)
+    with pytest.raises(afq.AFQError):
+        dbf.system.set_propagator_block(24)
