@@ -253,6 +253,13 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
     const bf16_t* st = lds + buf * GSTAGE + frag;
+    // the next K step into the other buffer (last read before the previous
+    // barrier), all of it at the start so that every chunk has the whole
+    // step's products to land in
+    if (kt + 1 < nk) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) fetch1(buf ^ 1, S.kbeg + (kt + 1) * GK, i);
+    }
     uint4 ah[4], al[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -265,8 +272,6 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
       const int c = 128 * wc + 16 * w;
       const uint4 bh = *(const uint4*)(st + (2 * GB + c) * GK);
       const uint4 bl = *(const uint4*)(st + (3 * GB + c) * GK);
-      // the other buffer was last read before the previous barrier
-      if (kt + 1 < nk) fetch1(buf ^ 1, S.kbeg + (kt + 1) * GK, w);
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
     }
