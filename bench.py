@@ -390,6 +390,44 @@ def run_uneven_batch(steps=50, nsys=64, precision="bf16x3", seed=7, tune_cache=N
     return out
 
 
+def run_config_m11(W=256, nsteps=400, seed=42):
+    """The LiH-like workload (M=11, N=2+2, Nchol=50, literal generator) as a
+    whole driver run -- 400 steps in blocks of 25 with their block energies,
+    QR every 5, comb every 10 -- eager against graph=True (each block's steps
+    as hipGraph replays).  Wall clock of the second of two identical runs."""
+    import torch
+    from omegaqmc_amd import PhiloxKey
+    from omegaqmc_amd.driver import AFQMCDriver
+    from omegaqmc_amd.synth import make_system
+    s = make_system(11, 2, 2, 50, seed=42)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()  # noqa: E731
+    integ = {'h1e': t(s.h1e), 'h1e_mod': t(s.h1e_mod), 'chol': t(s.chol), 'enuc': s.enuc,
+             'nbasis': 11, 'nup': 2, 'ndown': 2}
+    d = AFQMCDriver(integ, verbose=False)
+    nb = nsteps // 25
+    args = dict(rng_key=PhiloxKey(seed), num_walkers=W, num_blocks=nb - 4, num_steps_per_block=25,
+                stabilize_freq=5, pop_control_freq=10, num_blocks_equil=4)
+    out = {"workload": f"M=11 N=2+2 Nchol=50 W={W}, {nsteps} steps (QR every 5, comb every 10) "
+                       "as a driver run incl. the 16 block energies, FP64"}
+    res = {}
+    for mode in ("eager", "graph"):
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = d(graph=(mode == "graph"), **args)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        res[mode] = r
+        out[mode] = {"walker_steps_per_s": round(W * nsteps / wall, 1), "wall_s": round(wall, 4)}
+    out["graph_speedup"] = round(out["graph"]["walker_steps_per_s"]
+                                 / out["eager"]["walker_steps_per_s"], 3)
+    out["blocks_bit_identical"] = bool(np.array_equal(res["graph"]["energy_blocks"],
+                                                      res["eager"]["energy_blocks"]))
+    out["energy_mean"] = round(res["graph"]["energy_mean"], 8)
+    out["energy_err"] = round(res["graph"]["energy_err"], 8)
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling HBM byte counters."""
     import torch
@@ -429,8 +467,8 @@ def cpu_baseline(sys_np, sample_walkers=8, nsteps=1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--walkers", type=int, default=1024)
     ap.add_argument("--precision", default="auto", help="fp64 | fp32 | bf16x3 | auto")
     ap.add_argument("--no-cpu", action="store_true")
@@ -442,6 +480,8 @@ def main():
                     help="skip the 32-determinant M=150 workload")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-system uneven batch")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the M=11 driver run (eager against hipGraph)")
     ap.add_argument("--no-config2", action="store_true",
                     help="skip the M=120 FP64 / FP32 / BF16x3 workload")
     ap.add_argument("--acc-steps", type=int, default=100,
@@ -542,6 +582,10 @@ def main():
     if not args.no_batch:
         print("[bench] uneven batch of 64 systems", file=sys.stderr, flush=True)
         extra["uneven_batch"] = run_uneven_batch(steps=50)
+
+    if not args.no_config1:
+        print("[bench] M=11 driver run, eager and graph", file=sys.stderr, flush=True)
+        extra["config_m11"] = run_config_m11()
 
     if not args.no_config2:
         # the FP64 vs FP32-MFMA workload (the BF16x3 step beside them)

@@ -475,6 +475,10 @@ int afq_batch_state(afq_batch* b, int i, double** phia, double** phib, double** 
  * the context's stream. */
 int afq_batch_copy_state(afq_batch* b, int i, double* phia, double* phib, double* weights,
                          double* e_hybrid, double* overlap, double* acc);
+/* Zero every system's hybrid-energy accumulators (acc[2]), on the context's
+ * stream: the per-block reset of acc_weight / acc_ehybrid
+ * (afqmc_gto.py:1384-1385) between afq_batch_run calls. */
+int afq_batch_zero_accumulators(afq_batch* b);
 int afq_batch_destroy(afq_batch* b);
 
 #ifdef __cplusplus

@@ -93,6 +93,7 @@ SIGNATURES = {
     "afq_batch_run": [_P, _I, _I],
     "afq_batch_state": [_P, _I, _P, _P, _P, _P, _P, _P, _P],
     "afq_batch_copy_state": [_P, _I, _P, _P, _P, _P, _P, _P],
+    "afq_batch_zero_accumulators": [_P],
     "afq_batch_destroy": [_P],
     "afq_system_set_force_bias_splits": [_P, _I],
     "afq_system_set_multidet_method": [_P, _I],

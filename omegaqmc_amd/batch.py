@@ -77,6 +77,22 @@ class Batch:
         _lib.context()
         check(_lib.load().afq_batch_run(self.handle, int(nsteps), MODES[mode]), "afq_batch_run")
 
+    def zero_accumulators(self):
+        """Reset every system's hybrid-energy accumulators (the per-block
+        reset, afqmc_gto.py:1384-1385)."""
+        _lib.context()
+        check(_lib.load().afq_batch_zero_accumulators(self.handle), "afq_batch_zero_accumulators")
+
+    def copy_state_into(self, i, st):
+        """Copy system i's walkers, weights, e_hybrid, overlaps and
+        accumulators into the matching tensors of the driver state dict
+        ``st`` (as made by ``AFQMCDriver.init_walkers``)."""
+        _lib.context()
+        check(_lib.load().afq_batch_copy_state(self.handle, int(i), ptr(st['phia']),
+                                               ptr(st['phib']), ptr(st['weights']),
+                                               ptr(st['e_hybrid']), ptr(st['overlap']),
+                                               ptr(st['acc'])), "afq_batch_copy_state")
+
     def state(self, i):
         """Copies of system i's walkers, weights, e_hybrid, last overlaps and
         accumulators (torch tensors on the device)."""

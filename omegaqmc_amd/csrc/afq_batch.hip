@@ -50,6 +50,10 @@ __global__ void k_init_scalars(int W, double* __restrict__ w, cplx* __restrict__
   if (i < 2) acc[i] = 0.0;
 }
 
+__global__ void k_zero_acc(double* __restrict__ acc) {
+  if (threadIdx.x < 2) acc[threadIdx.x] = 0.0;
+}
+
 int lcm_int(int a, int b) { return a / std::gcd(a, b) * b; }
 
 }  // namespace
@@ -433,6 +437,14 @@ int afq_batch_copy_state(afq_batch* b, int i, double* phia, double* phib, double
   for (auto& c : cp)
     if (c.dst && c.bytes)
       AFQ_CHECK_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToDevice, st));
+  return AFQ_OK;
+}
+
+int afq_batch_zero_accumulators(afq_batch* b) {
+  AFQ_REQUIRE(b, "NULL argument");
+  for (const afq::BatchSys& x : b->s)
+    hipLaunchKernelGGL(afq::k_zero_acc, dim3(1), dim3(64), 0, b->ctx->stream, x.acc);
+  AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
 

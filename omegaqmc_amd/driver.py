@@ -178,9 +178,20 @@ class AFQMCDriver:
     def __call__(self, rng_key=None, num_walkers=100, num_blocks=100, num_steps_per_block=25,
                  stabilize_freq=5, pop_control_freq=5, num_blocks_equil=10, fname_log=None,
                  walker_chunk_size=None, eshift0=None, local_energy_cap=False,
-                 check_status=False):
+                 check_status=False, graph=False):
         seed = rng_key.seed if isinstance(rng_key, PhiloxKey) else (42 if rng_key is None
                                                                    else int(rng_key))
+        batch = None
+        if graph:
+            # the block's steps as hipGraph replays of a one-system afq_batch:
+            # the same kernels, fields, comb offsets and energy shift as the
+            # eager steps, so the blocks are bit-identical to graph=False
+            if self.multidet or type(self)._propagate is not AFQMCDriver._propagate:
+                raise ValueError("graph=True drives the single-determinant step only")
+            from .batch import Batch
+            batch = Batch([self.system], [num_walkers], seeds=[seed],
+                          stabilize_freq=stabilize_freq, pop_control_freq=pop_control_freq,
+                          nstreams=1)
         fout = sys.stdout if not fname_log else open(fname_log, 'w', 1)
         st = self.init_walkers(num_walkers)
         if eshift0 is None:
@@ -193,10 +204,17 @@ class AFQMCDriver:
         n_capped = 0
         t_prev = time.perf_counter()
         for iblock in range(num_blocks_equil + num_blocks):
-            st['acc'].zero_()
-            for _ in range(num_steps_per_block):
-                step_count += 1
-                self.step(st, step_count, seed, eshift, stabilize_freq, pop_control_freq)
+            if batch is None:
+                st['acc'].zero_()
+                for _ in range(num_steps_per_block):
+                    step_count += 1
+                    self.step(st, step_count, seed, eshift, stabilize_freq, pop_control_freq)
+            else:
+                batch.zero_accumulators()
+                batch.set_eshift([eshift])
+                batch.run(num_steps_per_block, "graph")
+                step_count += num_steps_per_block
+                batch.copy_state_into(0, st)
             e_tot = self.block_energy(st)
             if local_energy_cap:
                 e_ref = ((float(eshift0) if eshift0 is not None else self.ehf) if iblock == 0

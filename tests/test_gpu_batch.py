@@ -128,3 +128,38 @@ def test_batch_member_follows_oracle(afq):
     got = b.state(0)
     for k, ref in (('phia', pa), ('phib', pb), ('weights', w), ('e_hybrid', eh)):
         assert rel_err(_n(got[k]), ref) < 1e-10, k
+
+
+def test_driver_graph_mode_bit_identical(afq):
+    """The LiH-like workload (M=11, N=2+2, Nchol=50, 256 walkers, 400 steps,
+    QR every 5, comb every 10) through the driver with graph=True -- each
+    block's steps as hipGraph replays of a one-system batch -- gives the
+    eager driver's block energies and final walkers bit for bit."""
+    import torch
+    from omegaqmc_amd.driver import AFQMCDriver
+    from omegaqmc_amd.synth import make_system
+    s = make_system(11, 2, 2, 50, seed=42)
+    integ = {'h1e': _t(s.h1e), 'h1e_mod': _t(s.h1e_mod), 'chol': _t(s.chol), 'enuc': s.enuc,
+             'nbasis': 11, 'nup': 2, 'ndown': 2}
+    args = dict(rng_key=afq.PhiloxKey(42), num_walkers=256, num_blocks=12,
+                num_steps_per_block=25, stabilize_freq=5, pop_control_freq=10,
+                num_blocks_equil=4)
+    eager = AFQMCDriver(integ, verbose=False)(**args)
+    graph = AFQMCDriver(integ, verbose=False)(graph=True, **args)
+    torch.cuda.synchronize()
+    assert np.array_equal(graph['energy_blocks'], eager['energy_blocks'])
+    for k in ('phia', 'phib', 'weights', 'e_hybrid'):
+        assert torch.equal(graph['state'][k], eager['state'][k]), k
+
+
+def test_driver_graph_mode_refuses_multidet(afq):
+    from omegaqmc_amd.driver import AFQMCDriver
+    from omegaqmc_amd.synth import make_system
+    s = make_system(12, 3, 2, 30, seed=4)
+    integ = {'h1e': _t(s.h1e), 'h1e_mod': _t(s.h1e_mod), 'chol': _t(s.chol), 'enuc': s.enuc,
+             'nbasis': 12, 'nup': 3, 'ndown': 2}
+    trial = {'occ_up': np.array([[0, 1, 2], [0, 1, 3]]), 'occ_dn': np.array([[0, 1], [0, 2]]),
+             'ci_coeffs': np.array([0.9, 0.3], dtype=complex)}
+    d = AFQMCDriver(integ, verbose=False, trial=trial, multidet_method='explicit')
+    with pytest.raises(ValueError):
+        d(num_walkers=8, num_blocks=1, num_steps_per_block=2, num_blocks_equil=0, graph=True)
