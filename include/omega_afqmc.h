@@ -334,7 +334,8 @@ int afq_system_get_multidet_method(afq_system* sys);
  * arrays chol (G,M,M) real, rchol_a (G,Na,M) / rchol_b (G,Nb,M) real,
  * expH1 (M,M) real, mf_shift (G,) complex, trial_up (M,Na) / trial_dn
  * (M,Nb) complex, and builds the reduced-precision copies the chosen
- * precision needs.  The caller keeps those arrays alive and unchanged. */
+ * precision needs.  The caller keeps those arrays alive and unchanged.
+ * AFQ_BF16X3 needs every L_g exactly symmetric (AFQ_EUNSUPPORTED otherwise). */
 int afq_system_create(afq_ctx* ctx, afq_system** out, int nbasis, int nup, int ndown, int naux,
                       const double* chol, const double* rchol_a, const double* rchol_b,
                       const double* expH1, const double* mf_shift,
@@ -349,8 +350,10 @@ int afq_system_destroy(afq_system* sys);
  * local energy weights Coulomb and exchange by the sign
  * (afqmc_pw_heg.py:414-457).  chol_sign: device (G,) of +1 / -1, kept alive
  * and unchanged by the caller; NULL restores all +1.  Negative entries need
- * an FP64 single-determinant system (AFQ_EUNSUPPORTED otherwise: the split
- * BF16 VHS assumes symmetric L_g). */
+ * an FP64 or FP32 single-determinant system (AFQ_EUNSUPPORTED otherwise: the
+ * split-BF16 VHS assumes symmetric L_g, the electron gas's sign -1 vectors are
+ * antisymmetric); the block energy of a signed FP32 system is the FP64
+ * estimator's. */
 int afq_system_set_chol_sign(afq_system* sys, const double* chol_sign);
 
 /* K-splits of the force-bias contraction (no counterpart in the reference,

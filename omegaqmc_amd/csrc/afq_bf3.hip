@@ -849,8 +849,9 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
       S.bbytes = (unsigned)bb;
     }
   }
-  *nsplit_out = n;
-  if (W <= 0 || n == 0) return AFQ_OK;
+  // no vectors: no partials (the caller zero-fills the force bias)
+  *nsplit_out = b.G > 0 ? n : 0;
+  if (W <= 0 || n == 0 || b.G <= 0) return AFQ_OK;
   if (gemm_lds() > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 GEMM: LDS %zu", gemm_lds());
   a.nsplit = n;
   a.rows = 2 * W;

@@ -55,6 +55,7 @@ int f32_prepare(afq_system* s) {
   hipStream_t st = s->ctx->stream;
   const int M = s->M, G = s->G;
   int rc = AFQ_OK;
+  f->P = M * M;  // no vectors: the VHS GEMM (K = 0) writes the full zero matrix
   if (G > 0) {
     if (s->chol_packed) {
       f->P = s->packed_P;

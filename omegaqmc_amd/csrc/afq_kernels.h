@@ -5,6 +5,7 @@
 namespace afq {
 
 int copy_async(hipStream_t s, const double* src, double* dst, long n);
+int zero_async(hipStream_t s, void* p, size_t bytes);  // bytes a multiple of 4
 
 // greens (afq_greens.hip)
 int greens_launch(hipStream_t s, int mode, int W, int M, int Na, int Nb, const double* phia,

@@ -92,13 +92,14 @@ int lowp_step_buffers(afq_system* sys, int W, bf16_t** X, bf16_t** vhs, float** 
 
 int system_prepare_lowp(afq_system* sys) {
   if (sys->precision == AFQ_FP64) return AFQ_OK;
-  if (sys->precision == AFQ_FP32) {
-    if (sys->chol_sign)
-      return fail(AFQ_EUNSUPPORTED, "signed Cholesky vectors need the FP64 path");
-    return f32_prepare(sys);
-  }
+  if (sys->precision == AFQ_FP32) return f32_prepare(sys);
   if (sys->precision != AFQ_BF16X3)
     return fail(AFQ_EUNSUPPORTED, "precision %d is not built", sys->precision);
+  // the split-BF16 VHS is built on the upper-triangle blocks and mirrored:
+  // every L_g must be exactly symmetric (prepare_fp64 packs it only then)
+  if (sys->G > 0 && !sys->chol_packed)
+    return fail(AFQ_EUNSUPPORTED,
+                "bf16x3 needs exactly symmetric Cholesky vectors (FP64 and FP32 take any)");
   Bf3System* b = new Bf3System();
   sys->chol_lp = b;
   return bf3_prepare(sys, b);
@@ -176,12 +177,12 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
   if (rc) return rc;
   if (nsplit > FB_SPLITS_MAX) return fail(AFQ_EUNSUPPORTED, "force-bias splits %d", nsplit);
   if (nsplit == 0) {
-    AFQ_CHECK_HIP(hipMemsetAsync(part, 0, sizeof(float2) * (size_t)W * G, s));
+    if ((rc = zero_async(s, part, sizeof(float2) * (size_t)W * G))) return rc;
     nsplit = 1;
   }
   phase_begin(sys, PH_SHIFT);
   rc = shift_launch(s, W, G, nullptr, sys->mf_shift, xi, sys->dt, sys->fbbound, nullptr, nullptr,
-                    cfb, cmf, nsplit, part, X, b.Gp);
+                    cfb, cmf, nsplit, part, X, b.Gp, sys->chol_sign);
   phase_end(sys, PH_SHIFT);
   if (rc) return rc;
   // 4
@@ -213,7 +214,7 @@ int propagate_lowp(afq_system* sys, int W, double* phia, double* phib, double* w
 int system_energy_lowp(afq_system* sys, int W, const double* phia, const double* phib,
                        const double* h1e, double enuc, double* e_tot, double* e_1b, double* e_2b) {
   const Bf3System& b = *(const Bf3System*)sys->chol_lp;
-  if (!bf3_exchange_supported(b))
+  if (!bf3_exchange_supported(b) || sys->chol_sign)
     return system_energy_fp64(sys, W, phia, phib, h1e, enuc, e_tot, e_1b, e_2b);
   hipStream_t s = sys->ctx->stream;
   LowpWork* L = nullptr;

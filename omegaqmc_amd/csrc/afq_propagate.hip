@@ -112,12 +112,28 @@ __global__ void k_copy(long n, const double* __restrict__ a, double* __restrict_
     b[i] = a[i];
 }
 
+__global__ void k_zero(long n, unsigned* __restrict__ p) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
 }  // namespace
 
 int copy_async(hipStream_t s, const double* src, double* dst, long n) {
   if (n <= 0 || src == dst) return AFQ_OK;
   int blocks = (int)std::min<long>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, n, src, dst);
+  AFQ_CHECK_LAUNCH();
+  return AFQ_OK;
+}
+
+// zero fill by a kernel rather than hipMemsetAsync: these run inside steps
+// that a caller may capture into a hipGraph
+int zero_async(hipStream_t s, void* p, size_t bytes) {
+  const long n = (long)(bytes / 4);
+  if (n <= 0) return AFQ_OK;
+  int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, n, (unsigned*)p);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
@@ -142,9 +158,7 @@ int force_bias_gemm_fp64(hipStream_t s, int W, int M, int Na, int Nb, int G, con
     if (rc) return rc;
     first = false;
   }
-  if (first) {  // no electrons: vbias = 0
-    AFQ_CHECK_HIP(hipMemsetAsync(vbiasT, 0, sizeof(cplx) * (size_t)W * G, s));
-  }
+  if (first) return zero_async(s, vbiasT, sizeof(cplx) * (size_t)W * G);  // no electrons
   return AFQ_OK;
 }
 
@@ -284,7 +298,7 @@ int afq_force_bias(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const doub
   if (!vb && W * (size_t)G) return AFQ_ENOMEM;
   int rc;
   if (sp.n == 0) {
-    AFQ_CHECK_HIP(hipMemsetAsync(vb, 0, sizeof(double) * 2 * (size_t)W * G, ctx->stream));
+    if ((rc = afq::zero_async(ctx->stream, vb, sizeof(double) * 2 * (size_t)W * G))) return rc;
   } else if ((rc = afq::fb_mfma_f64(ctx->stream, W, G, sp, vb))) {
     return rc;
   }

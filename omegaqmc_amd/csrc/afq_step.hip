@@ -419,9 +419,13 @@ int afq_system_set_chol_sign(afq_system* s, const double* sign) {
     AFQ_REQUIRE(v == 1.0 || v == -1.0, "chol_sign entries must be +1 or -1 (got %g)", v);
     neg = neg || v < 0.0;
   }
-  if (neg && (s->precision != AFQ_FP64 || s->ndet > 0))
+  // FP64 and FP32 single-determinant steps take signed vectors (the shift
+  // kernel folds the sign into the VHS coefficients; the energy is the FP64
+  // estimator with the signs); a BF16x3 system would need its vectors
+  // symmetric, which the electron gas's antisymmetric (sign -1) ones are not
+  if (neg && (s->precision == AFQ_BF16X3 || s->ndet > 0))
     return afq::fail(AFQ_EUNSUPPORTED,
-                     "Cholesky vectors with sign -1 need the FP64 single-determinant path");
+                     "Cholesky vectors with sign -1 need the FP64 or FP32 single-determinant path");
   s->chol_sign = neg ? sign : nullptr;
   return AFQ_OK;
 }

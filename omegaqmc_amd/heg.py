@@ -15,8 +15,9 @@ grid differences, (q, -q) pairs in that order) follow the reference so that
 a given system has the same basis, trial and vector order.
 
 The split-BF16 step needs symmetric L_g (its VHS is built on one triangle),
-so systems with antisymmetric vectors run in FP64; ``precision="bf16x3"``
-raises for them.
+so systems with antisymmetric vectors run in FP64 or FP32-MFMA (its VHS
+GEMM takes full M x M vectors; the block energy is the FP64 estimator);
+``precision="bf16x3"`` raises for them.
 """
 
 from __future__ import annotations

@@ -30,10 +30,11 @@ def _n(t):
     return t.detach().cpu().numpy()
 
 
-def _driver(rs, n_elec, n_pw, polarization='unpolarized', coulomb=True, dt=0.01):
+def _driver(rs, n_elec, n_pw, polarization='unpolarized', coulomb=True, dt=0.01,
+            precision="fp64"):
     from omegaqmc_amd import heg as H
     s = H.build_3deg_system(rs, n_elec, n_pw, polarization=polarization)
-    return s, H.HEGDriver(s, dt=dt, include_coulomb=coulomb, verbose=False)
+    return s, H.HEGDriver(s, dt=dt, include_coulomb=coulomb, verbose=False, precision=precision)
 
 
 def _oracle_inputs(d):
@@ -110,12 +111,14 @@ def test_local_energy_matches_oracle(afq):
     np.testing.assert_allclose(_n(e_tr).real, d.e_trial, rtol=1e-12)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("n_elec,pol", [(14, 'unpolarized'), (7, 'polarized')])
-def test_free_electron_energy_is_kinetic(afq, n_elec, pol):
+def test_free_electron_energy_is_kinetic(afq, n_elec, pol, precision):
     """Without the Coulomb term every walker stays a scaled Fermi sea: the
     AFQMC energy is the sum of the occupied kinetic energies (reference
-    tests/test_afqmc_3deg.py:296-373, here exact)."""
-    s, d = _driver(2.0, n_elec, 19, polarization=pol, coulomb=False)
+    tests/test_afqmc_3deg.py:296-373, here exact; no Cholesky vectors, so
+    the reduced precisions take the same path with zero fields)."""
+    s, d = _driver(2.0, n_elec, 19, polarization=pol, coulomb=False, precision=precision)
     eps = np.sort(np.diag(_n(d.h1e)))
     exact = (2.0 if pol == 'unpolarized' else 1.0) * np.sum(eps[:s['nup']])
     out = d(num_walkers=16, num_blocks=3, num_steps_per_block=5, num_blocks_equil=1)
@@ -154,3 +157,48 @@ def test_twist_averaged_run(afq):
         np.testing.assert_allclose(out['e_trial_twists'][i], e_hf, rtol=1e-12)
     assert np.all(np.isfinite(out['e_twists']))
     np.testing.assert_allclose(out['energy_mean'], np.mean(out['e_twists']), rtol=1e-14)
+
+
+def test_fp32_step_and_energy_with_signed_vectors(afq):
+    """FP32-MFMA takes the signed (and antisymmetric) vectors: one step
+    against the oracle at the FP32 step bound, and the block energy of the
+    stepped walkers equal to the FP64 system's (the same FP64 estimator)."""
+    s, d = _driver(2.0, 14, 19, precision="fp32")
+    _, d64 = _driver(2.0, 14, 19)
+    prop, L, ra, rb, tu, td, sign, phase = _oracle_inputs(d)
+    assert np.any(sign < 0)
+    W, M, Na, Nb, G = 11, s['N_pw'], s['nup'], s['ndown'], L.shape[0]
+    rng = np.random.default_rng(5)
+    pa = tu + 0.15 * (rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na)))
+    pb = td + 0.15 * (rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb)))
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    xi = rng.normal(size=(W, G))
+    ref = OH.propagate_3deg_walkers(pa, pb, w, None, eh, prop, L, ra, rb, tu.astype(complex),
+                                    td.astype(complex), -0.4, xi, True, ndown=Nb,
+                                    chol_phase=phase)
+    ga, gb, gw, ge = _t(pa), _t(pb), _t(w), _t(eh)
+    ov = d.system.step(ga, gb, gw, ge, -0.4, xi=_t(xi))
+    for x, y in ((ga, ref[0]), (gb, ref[1]), (ov, ref[3])):
+        assert rel_err(_n(x), y) < 1e-5
+    e32 = d.system.local_energy(ga, gb, d.h1e, d.enuc)
+    e64 = d64.system.local_energy(ga, gb, d64.h1e, d64.enuc)
+    for x, y in zip(e32, e64):
+        assert np.array_equal(_n(x), _n(y))
+
+
+def test_bf16x3_refuses_nonsymmetric_vectors(afq):
+    """The split-BF16 VHS mirrors the upper triangle, so a BF16x3 system
+    with any L_g not exactly symmetric is refused at creation."""
+    import torch
+    from omegaqmc_amd.afqmc_gto import System
+    from omegaqmc_amd.synth import make_system
+    sy = make_system(16, 3, 3, 20, seed=1)
+    chol = sy.chol.copy()
+    chol[3, 2, 5] += 1e-3
+    tu, td = torch.eye(16, 3, dtype=torch.complex128), torch.eye(16, 3, dtype=torch.complex128)
+    prop = {'expH1': _t(np.eye(16)), 'mf_shift': _t(np.zeros(20, complex)), 'dt': 0.01}
+    ra = _t(np.ascontiguousarray(chol[:, :3, :]))
+    with pytest.raises(afq.AFQError, match="symmetric"):
+        System(prop, _t(chol), ra, ra, tu.cuda(), td.cuda(), precision="bf16x3")
+    System(prop, _t(chol), ra, ra, tu.cuda(), td.cuda(), precision="fp32")
