@@ -896,7 +896,8 @@ int afq_greens_function_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int
                                  double* Ga, double* Gb, double* overlap) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
-  AFQ_REQUIRE(phia && phib && ci && Ga && Gb && overlap, "NULL argument");
+  AFQ_REQUIRE((phia || Na == 0) && (phib || Nb == 0) && ci && Ga && Gb && overlap,
+              "NULL argument");
   hipStream_t s = ctx->stream;
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
@@ -919,7 +920,7 @@ int afq_greens_function_multidet_overlap_only(afq_ctx* ctx, int W, int M, int Na
                                               double* overlap) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
-  AFQ_REQUIRE(phia && phib && ci && overlap, "NULL argument");
+  AFQ_REQUIRE((phia || Na == 0) && (phib || Nb == 0) && ci && overlap, "NULL argument");
   hipStream_t s = ctx->stream;
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
@@ -940,7 +941,7 @@ int afq_greens_function_multidet_force_bias(afq_ctx* ctx, int W, int M, int Na, 
                                             double* ovlp_a_all, double* ovlp_b_all) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0, "bad shape");
-  AFQ_REQUIRE(phia && phib && ci && overlap, "NULL argument");
+  AFQ_REQUIRE((phia || Na == 0) && (phib || Nb == 0) && ci && overlap, "NULL argument");
   AFQ_REQUIRE((Ghalfa_all || Na == 0) && (Ghalfb_all || Nb == 0), "NULL Ghalf output");
   hipStream_t s = ctx->stream;
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
@@ -978,7 +979,9 @@ int afq_local_energy_multidet(afq_ctx* ctx, int W, int M, int Na, int Nb, int G,
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && Na <= M && Nb <= M && nd > 0 && G >= 0,
               "bad shape");
-  AFQ_REQUIRE(h1e && chol && phia && phib && ci && e_tot && e_1b, "NULL argument");
+  AFQ_REQUIRE(h1e && (chol || G == 0) && (phia || Na == 0) && (phib || Nb == 0) && ci && e_tot &&
+                  e_1b,
+              "NULL argument");
   hipStream_t s = ctx->stream;
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);
@@ -1007,7 +1010,7 @@ int afq_system_create_multidet(afq_ctx* ctx, afq_system** out, int M, int Na, in
   AFQ_REQUIRE(dt > 0 && nmax >= 0 && det_chunk_size > 0, "bad dt / nmax / det_chunk_size");
   AFQ_REQUIRE(precision == AFQ_FP64 || precision == AFQ_BF16X3,
               "multi-det: precision %d is not built (use FP64 or BF16X3)", precision);
-  AFQ_REQUIRE(chol && expH1 && mf && ci, "NULL system array");
+  AFQ_REQUIRE((chol || G == 0) && expH1 && (mf || G == 0) && ci, "NULL system array");
   hipStream_t s = ctx->stream;
   int rc = afq::md_check_occ(s, nd, Na, M, occ_up);
   if (!rc) rc = afq::md_check_occ(s, nd, Nb, M, occ_dn);

@@ -308,3 +308,55 @@ def test_multidet_driver_trajectory_matches_oracle(afq):
     assert rel_err(_n(e), re) < 1e-9
     eb = float(st['stats'][0].item())
     np.testing.assert_allclose(eb, np.sum(w * re.real) / np.sum(w), rtol=1e-9)
+
+
+@pytest.mark.parametrize("method", ["explicit", "ratio"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("bf16x3", 1e-4)])
+def test_polarized_multidet_step_and_energy(afq, method, precision, tol):
+    """A fully polarized multi-det trial (no beta electrons): the step and
+    the local energy against the oracle."""
+    M, Na, Nb, G, nd, W, seed = 14, 4, 0, 24, 5, 6, 12
+    s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
+    assert od.shape == (nd, 0) and pb.shape == (W, M, 0)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(0.5, 1.5, W)
+    eh = rng.normal(size=W) + 0.1j * rng.normal(size=W)
+    xi = P.aux_fields(3, 1, W, G)
+    ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -3.0, xi)
+    out = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
+                                         _t(ou), _t(od), _t(ci), -3.0, None, xi=_t(xi),
+                                         precision=precision, method=method)
+    for k in (0, 2, 3, 4):
+        assert rel_err(_n(out[k]), ref[k]) < (tol if k < 4 else 10 * tol), k
+    et, e1, e2 = afq.local_energy_multidet(_t(s.h1e), _t(s.chol), _t(pa), _t(pb), _t(ou), _t(od),
+                                           _t(ci), s.enuc, 4)
+    rt, r1, r2 = MD.local_energy_multidet(s.h1e, s.chol, pa, pb, tu, td, ci, s.enuc, 4)
+    assert rel_err(_n(et), rt) < 1e-10 and rel_err(_n(e2), r2) < 1e-10
+
+
+def test_single_det_multidet_equals_single_det_step(afq):
+    """One determinant with coefficient 1: the multi-det step is the
+    single-determinant step (oracle and GPU, FP64)."""
+    M, Na, Nb, G, W = 16, 4, 3, 30, 7
+    from omegaqmc_amd.synth import make_system
+    s = make_system(M, Na, Nb, G, seed=21)
+    ou = np.arange(Na, dtype=np.int32)[None]
+    od = np.arange(Nb, dtype=np.int32)[None]
+    ci = np.array([1.0 + 0j])
+    tu, td = MD.trials_from_occ(M, ou), MD.trials_from_occ(M, od)
+    rng = np.random.default_rng(4)
+    pa = walkers_near_trial(np.eye(M, Na).astype(complex), W, rng, 0.2)
+    pb = walkers_near_trial(np.eye(M, Nb).astype(complex), W, rng, 0.2)
+    w = np.ones(W)
+    eh = np.zeros(W, complex)
+    xi = P.aux_fields(8, 1, W, G)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    md = afq.propagate_walkers_multidet(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
+                                        _t(ou), _t(od), _t(ci), -1.0, None, xi=_t(xi),
+                                        method="explicit")
+    ra, rb = O.half_rotate_cholesky(s.chol, tu[0], td[0])
+    ref = O.propagate_walkers(pa, pb, w, None, eh, prop, s.chol, ra.real, rb.real, tu[0], td[0],
+                              -1.0, xi)
+    for k in range(5):
+        assert rel_err(_n(md[k]), ref[k]) < 1e-10, k
