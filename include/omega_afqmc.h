@@ -195,6 +195,18 @@ int afq_population_control_apply(afq_ctx* ctx, int nwalkers, int nbasis, int nup
                                  const double* e_hybrid, double* weights_new, double* phia_new,
                                  double* phib_new, double* e_hybrid_new);
 
+/* afqmc_gto.py:241-284 (population_control_pair_branch), in place: walkers
+ * ordered by weight (ties in index order, a stable argsort), the k-th
+ * lightest paired with the k-th heaviest; when the heavy weight exceeds
+ * 4 <w> and the light one is below 0.1 <w> (<w> = sum w / W, summed left to
+ * right) the light walker becomes a copy of the heavy one and both weights
+ * become their mean.  src (W,) int32, optional: the walker each walker was
+ * copied from (itself when unchanged).  W <= AFQ_PAIR_BRANCH_MAX_WALKERS (one
+ * workgroup sorts the weights in LDS). */
+#define AFQ_PAIR_BRANCH_MAX_WALKERS 8192
+int afq_population_control_pair_branch(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                                       double* weights, double* phia, double* phib, int32_t* src);
+
 /* afqmc_gto.py:1428-1432 weight clip w <- clip(w, 0, frac * sum|w|) in place. */
 int afq_clip_weights(afq_ctx* ctx, int nwalkers, double* weights, double frac);
 

@@ -19,7 +19,9 @@ from .afqmc_gto import (  # noqa: F401
 from ._lib import AFQError  # noqa: F401
 from .afqmc_gto import (greens_function_multidet_force_bias,  # noqa: F401
                         half_rotate_cholesky_multidet, local_energy_multidet_streamed,
-                        walker_status, decode_walker_flags)
+                        walker_status, decode_walker_flags, Walkers,
+                        population_control_pair_branch, WickTrial, build_wick_trial,
+                        propagate_walkers_wick)
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
 from .estream import EStreamDriver, get_afqmc_estream_func_from_integrals  # noqa: F401
 from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401

@@ -58,6 +58,7 @@ SIGNATURES = {
     "afq_population_control_comb_noalloc": [_P, _I, _I, _I, _I, _P, _P, _P, _D, _P, _P, _P, _P,
                                             _P, _P],
     "afq_population_control_apply": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+    "afq_population_control_pair_branch": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
     "afq_clip_weights": [_P, _I, _P, _D],
     "afq_accumulate_hybrid": [_P, _I, _P, _P, _P],
     "afq_block_stats": [_P, _I, _P, _P, _P],

@@ -278,6 +278,26 @@ def apply_one_body(expH1, phi):
     return out
 
 
+class Walkers:
+    """afqmc_gto.py:76-123: nwalkers copies of the trial (complex128 device
+    tensors), weights 1, overlaps 1, hybrid energies 0, log_detR and
+    log_shift 0."""
+
+    def __init__(self, trial_up, trial_dn, nwalkers, nbasis, nup, ndown):
+        tu, td = _c(trial_up), _c(trial_dn)
+        if tu.shape != (nbasis, nup) or td.shape != (nbasis, ndown):
+            raise ValueError("trial shapes do not match (nbasis, nup) / (nbasis, ndown)")
+        self.nwalkers, self.nbasis, self.nup, self.ndown = int(nwalkers), nbasis, nup, ndown
+        dev = tu.device
+        self.phia = tu.unsqueeze(0).repeat(self.nwalkers, 1, 1).contiguous()
+        self.phib = td.unsqueeze(0).repeat(self.nwalkers, 1, 1).contiguous()
+        self.weights = torch.ones(self.nwalkers, dtype=F64, device=dev)
+        self.overlap = torch.ones(self.nwalkers, dtype=C128, device=dev)
+        self.e_hybrid = torch.zeros(self.nwalkers, dtype=C128, device=dev)
+        self.log_detR = torch.zeros(self.nwalkers, dtype=F64, device=dev)
+        self.log_shift = torch.zeros(self.nwalkers, dtype=F64, device=dev)
+
+
 def orthogonalize_walkers(phia, phib):
     """afqmc_gto.py:160-181 -> (phia, phib, log_detR, sign_detR) (new arrays)."""
     pa, pb = _c(phia).clone(), _c(phib).clone()
@@ -310,6 +330,22 @@ def population_control_comb(weights, phia, phib, rng_key, e_hybrid=None, return_
           "population_control_comb")
     out = (wn, pan, pbn) if eh is None else (wn, pan, pbn, ehn)
     return (out + (idx,)) if return_indices else out
+
+
+def population_control_pair_branch(weights, phia, phib, rng_key=None, return_sources=False):
+    """afqmc_gto.py:241-284 -> (weights_new, phia_new, phib_new[, src]).
+
+    The lightest and heaviest walkers are paired; a heavy walker above 4<w>
+    paired with a light one below 0.1<w> replaces it, both at the mean
+    weight.  ``rng_key`` is unused, as in the reference.  Ties in weight are
+    ordered by walker index (a stable argsort)."""
+    w, pa, pb = _r(weights).clone(), _c(phia).clone(), _c(phib).clone()
+    W, M, Na = pa.shape
+    Nb = pb.shape[2]
+    src = torch.empty((W,), dtype=torch.int32, device=pa.device)
+    check(_L().afq_population_control_pair_branch(_ctx(), W, M, Na, Nb, ptr(w), ptr(pa), ptr(pb),
+                                                  ptr(src)), "population_control_pair_branch")
+    return (w, pa, pb, src) if return_sources else (w, pa, pb)
 
 
 # ---------------------------------------------------------------------------
@@ -753,6 +789,42 @@ def propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagato
     next_key, k = rng_key.split() if isinstance(rng_key, PhiloxKey) else (rng_key, None)
     ov = sysobj.step(pa, pb, w, eh, eshift, xi=xi, key=k)
     return pa, pb, w, ov, eh, next_key
+
+
+class WickTrial:
+    """The multi-det trial of ``build_wick_trial`` (wick.py:190-226), kept as
+    occupation strings and CI coefficients: the ratio-update step forms what
+    the reference's WickTrial precomputes (det 0's inverse, each det's
+    excitation, the adjugate weights) on the device at every step."""
+
+    def __init__(self, occ_up, occ_dn, ci_coeffs):
+        self.occ_up = occ_up
+        self.occ_dn = occ_dn
+        self.ci_coeffs = ci_coeffs
+        self.ndet = len(ci_coeffs)
+
+
+def build_wick_trial(occ_up, occ_dn, ci_coeffs, chol=None, h1e=None, chunk_g=None, orbs_up=None,
+                     orbs_dn=None):
+    """wick.py:190-226 for occupation-string trials in the walker basis (det 0
+    the reference determinant).  Other orbital sets (UHF-based trials) are
+    not built."""
+    if orbs_up is not None or orbs_dn is not None:
+        raise NotImplementedError("Wick trials over other orbital sets are not built")
+    return WickTrial(occ_up, occ_dn, ci_coeffs)
+
+
+def propagate_walkers_wick(phia, phib, weights, overlap, e_hybrid, propagator, chol, wick_trial,
+                           eshift, rng_key, fbbound=None, exp_nmax=6, walker_chunk_size=None,
+                           chol_chunk_g=None, det_chunk_size=5, xi=None, precision="fp64"):
+    """afqmc_gto.py:842-935 -> (phia, phib, weights_new, overlap_new, e_hybrid_new, rng_key):
+    the multi-det step on ratio updates relative to det 0 (one inverse per
+    walker and spin)."""
+    t = wick_trial
+    return propagate_walkers_multidet(phia, phib, weights, overlap, e_hybrid, propagator, chol,
+                                      t.occ_up, t.occ_dn, t.ci_coeffs, eshift, rng_key, fbbound,
+                                      exp_nmax, walker_chunk_size, chol_chunk_g, det_chunk_size,
+                                      xi=xi, precision=precision, method="ratio")
 
 
 # ---------------------------------------------------------------------------

@@ -5,6 +5,7 @@
 namespace afq {
 
 int copy_async(hipStream_t s, const double* src, double* dst, long n);
+size_t pair_branch_lds(int W);
 int zero_async(hipStream_t s, void* p, size_t bytes);  // bytes a multiple of 4
 
 // greens (afq_greens.hip)

@@ -342,6 +342,37 @@ def population_control_comb(weights, phia, phib, r, e_hybrid=None):
     return wn, phia[idx], phib[idx], eh, idx
 
 
+def population_control_pair_branch(weights, phia, phib):
+    """afqmc_gto.py:241-284 -> (weights_new, phia_new, phib_new, src).
+
+    The reference orders the walkers with np.argsort's default (not stable)
+    sort; here ties are ordered by index (kind='stable'), which agrees with it
+    whenever the weights are distinct.  The total weight is summed left to
+    right (the reference's jnp.sum may round the last bit differently, which
+    matters only for a weight exactly at 0.1 <w> or 4 <w>).
+    """
+    w = np.array(weights, dtype=np.float64)
+    pa, pb = np.array(phia), np.array(phib)
+    n = w.shape[0]
+    total = 0.0
+    for x in w:
+        total += x
+    avg = total / n
+    order = np.argsort(w, kind='stable')
+    lo, hi = 0.1 * avg, 4.0 * avg
+    src = np.arange(n, dtype=np.int32)
+    for k in range(n // 2):
+        il, ih = order[k], order[n - 1 - k]
+        if w[ih] > hi and w[il] < lo:
+            m = 0.5 * (w[il] + w[ih])
+            w[il] = m
+            w[ih] = m
+            pa[il] = pa[ih]
+            pb[il] = pb[ih]
+            src[il] = ih
+    return w, pa, pb, src
+
+
 # ---------------------------------------------------------------------------
 # Statistics (utils.py:86-149)
 # ---------------------------------------------------------------------------

@@ -295,3 +295,37 @@ def test_propagate_one_step(afq, M, Na, Nb, G, W):
     for x, y in zip(out[:5], ref):
         assert rel_err(_n(x), y) < TOL
 
+
+
+@pytest.mark.parametrize("W", [7, 1000, 8191, 8192])
+def test_pair_branch_matches_oracle(afq, W):
+    """afqmc_gto.py:241-284 on the GPU: weights, walkers and sources equal
+    the oracle's bit for bit (heavy-tailed weights with many exact zeros, so
+    ties are ordered by index)."""
+    rng = np.random.default_rng(W)
+    w = rng.lognormal(0.0, 1.5, size=W)
+    w[rng.uniform(size=W) < 0.25] = 0.0
+    M, Na, Nb = 5, 2, 1
+    pa = rng.normal(size=(W, M, Na)) + 1j * rng.normal(size=(W, M, Na))
+    pb = rng.normal(size=(W, M, Nb)) + 1j * rng.normal(size=(W, M, Nb))
+    rw, rpa, rpb, rsrc = O.population_control_pair_branch(w, pa, pb)
+    assert np.any(rsrc != np.arange(W)) or W == 7
+    gw, gpa, gpb, gsrc = afq.population_control_pair_branch(_t(w), _t(pa), _t(pb),
+                                                            return_sources=True)
+    assert np.array_equal(_n(gsrc), rsrc)
+    assert np.array_equal(_n(gw), rw)
+    assert np.array_equal(_n(gpa), rpa) and np.array_equal(_n(gpb), rpb)
+
+
+def test_pair_branch_limits(afq):
+    import torch
+    W = 8193
+    z = torch.zeros((W, 2, 1), dtype=torch.complex128, device="cuda")
+    with pytest.raises(afq.AFQError):
+        afq.population_control_pair_branch(torch.ones(W, dtype=torch.float64, device="cuda"), z, z)
+    # the reference's container: trial copies, unit weights and overlaps
+    tu = torch.eye(6, 2, dtype=torch.float64)
+    wk = afq.Walkers(tu, tu[:, :1], 4, 6, 2, 1)
+    assert wk.phia.shape == (4, 6, 2) and wk.phib.shape == (4, 6, 1)
+    assert torch.all(wk.weights == 1) and torch.all(wk.overlap == 1)
+    assert torch.all(wk.e_hybrid == 0) and torch.all(wk.log_detR == 0)

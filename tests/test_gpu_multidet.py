@@ -360,3 +360,24 @@ def test_single_det_multidet_equals_single_det_step(afq):
                               -1.0, xi)
     for k in range(5):
         assert rel_err(_n(md[k]), ref[k]) < 1e-10, k
+
+
+def test_propagate_walkers_wick_is_ratio_step(afq):
+    """afqmc_gto.py:842-935 mirror: a Wick trial of occupation strings runs
+    the ratio-update step, against the oracle at 1e-10."""
+    M, Na, Nb, G, nd, W, seed = 16, 5, 4, 30, 11, 6, 2
+    s, ou, od, ci, tu, td, pa, pb = _case(M, Na, Nb, G, nd, W, seed)
+    prop, propd = _propagator(afq, s, tu, td, ci)
+    rng = np.random.default_rng(seed + 100)
+    w = rng.uniform(0.5, 1.5, W)
+    eh = rng.normal(size=W) + 0.1j * rng.normal(size=W)
+    xi = P.aux_fields(7, 1, W, G)
+    ref = MD.propagate_walkers_multidet(pa, pb, w, None, eh, prop, s.chol, tu, td, ci, -3.0, xi)
+    trial = afq.build_wick_trial(_t(ou), _t(od), _t(ci), _t(s.chol), _t(s.h1e))
+    out = afq.propagate_walkers_wick(_t(pa), _t(pb), _t(w), None, _t(eh), propd, _t(s.chol),
+                                     trial, -3.0, None, xi=_t(xi))
+    for k in (0, 1, 2, 3):
+        assert rel_err(_n(out[k]), ref[k]) < 1e-10, k
+    assert rel_err(_n(out[4]), ref[4]) < 1e-9
+    with pytest.raises(NotImplementedError):
+        afq.build_wick_trial(_t(ou), _t(od), _t(ci), orbs_up=_t(np.eye(M)))

@@ -285,6 +285,46 @@ def test_comb_properties_and_brute_force(W):
     assert np.sum(wn) == W and np.array_equal(idx2, idx) and np.array_equal(eh.real, idx)
 
 
+def test_pair_branch_known_answer():
+    """Worked by hand: <w> = 14.01/6, bounds 0.2335 and 9.34; the sorted
+    order is 1 (0.01), 0, 3, 4, 5 (1 each, index order), 2 (10): only the
+    pair (1, 2) qualifies, both at 5.005, walker 1 a copy of walker 2."""
+    w = np.array([1.0, 0.01, 10.0, 1.0, 1.0, 1.0])
+    pa = np.arange(6.0)[:, None, None] + 0j
+    pb = -np.arange(6.0)[:, None, None] + 0j
+    wn, pan, pbn, src = O.population_control_pair_branch(w, pa, pb)
+    np.testing.assert_array_equal(wn, [1.0, 5.005, 5.005, 1.0, 1.0, 1.0])
+    np.testing.assert_array_equal(src, [0, 2, 2, 3, 4, 5])
+    assert pan[1, 0, 0] == 2.0 and pbn[1, 0, 0] == -2.0 and pan[2, 0, 0] == 2.0
+    # nothing beyond the bounds: unchanged
+    wn, _, _, src = O.population_control_pair_branch(np.ones(5), pa[:5], pb[:5])
+    assert np.array_equal(wn, np.ones(5)) and np.array_equal(src, np.arange(5))
+
+
+@pytest.mark.parametrize("W", [9, 200])
+def test_pair_branch_invariants(W):
+    """Total weight kept (each clone pair keeps its sum), every copy made from
+    a walker above 4<w> onto one below 0.1<w>, sources never overwritten, and
+    the k-th lightest paired with the k-th heaviest (ties by index)."""
+    rng = np.random.default_rng(W)
+    w = rng.lognormal(0.0, 1.5, size=W)
+    w[rng.uniform(size=W) < 0.3] = 0.0
+    pa = rng.normal(size=(W, 3, 2)) + 1j * rng.normal(size=(W, 3, 2))
+    wn, pan, _, src = O.population_control_pair_branch(w, pa, pa)
+    avg = np.sum(w) / W
+    np.testing.assert_allclose(np.sum(wn), np.sum(w), rtol=1e-14)
+    moved = np.nonzero(src != np.arange(W))[0]
+    assert len(moved) > 0
+    order = sorted(range(W), key=lambda i: (w[i], i))
+    partner = {order[k]: order[W - 1 - k] for k in range(W // 2)}
+    for i in moved:
+        j = src[i]
+        assert partner[i] == j and w[j] > 4 * avg and w[i] < 0.1 * avg
+        assert src[j] == j and np.array_equal(pan[i], pa[j]) and wn[i] == wn[j]
+    for i in set(range(W)) - set(moved) - set(src[moved]):
+        assert wn[i] == w[i] and np.array_equal(pan[i], pa[i])
+
+
 # --------------------------------------------------------------------------- statistics
 
 def test_binning_closed_forms():
