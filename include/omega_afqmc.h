@@ -101,6 +101,23 @@ int afq_local_energy(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
                      const double* rchol_a, const double* rchol_b, int dtype, double enuc,
                      double* e_tot, double* e_1b, double* e_2b);
 
+/* afqmc_pw_heg.py:414-457 local_energy_2body_pw: the two-body energy with the
+ * ERI sign of each vector, (pq|rs) = sum_g sign_g L_g,pq L_g,rs: Coulomb
+ * 1/2 sum_g sign_g X_g^2 and exchange 1/2 sum_g sign_g sum_ij T_ij T_ji.
+ * chol_sign: device (G,) of +1 / -1. */
+int afq_local_energy_2body_pw(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown,
+                              int naux, const double* Ghalfa, const double* Ghalfb,
+                              const double* rchol_a, const double* rchol_b, int rchol_dtype,
+                              const double* chol_sign, double* e_coul, double* e_exch);
+
+/* afqmc_pw_heg.py:460-471 local_energy_pw: afq_local_energy with the signed
+ * two-body part above. */
+int afq_local_energy_pw(afq_ctx* ctx, int nwalkers, int nbasis, int nup, int ndown, int naux,
+                        const double* h1e, const double* Ga, const double* Gb,
+                        const double* Ghalfa, const double* Ghalfb, const double* rchol_a,
+                        const double* rchol_b, int dtype, double enuc, const double* chol_sign,
+                        double* e_tot, double* e_1b, double* e_2b);
+
 /* energy.py:218-252 local_energy_streamed(h1e_trial_a (M,Na), h1e_trial_b (M,Nb),
  * Ghalfa, Ghalfb, rchol_a, rchol_b, enuc, e_chunk_g).  h1e_trial_* complex.
  * e_chunk_g only fixes the g-slab order of the exchange sum (any value >= 1). */

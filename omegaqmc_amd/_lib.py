@@ -46,6 +46,9 @@ SIGNATURES = {
     "afq_greens_function": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "afq_local_energy_2body": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P],
     "afq_local_energy": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P],
+    "afq_local_energy_2body_pw": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P],
+    "afq_local_energy_pw": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P,
+                            _P],
     "afq_local_energy_streamed": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I,
                                   _P, _P, _P],
     "afq_build_vhs": [_P, _I, _I, _I, _P, _P, _D, _P],

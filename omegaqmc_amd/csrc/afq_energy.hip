@@ -237,17 +237,19 @@ static int e2b_scratch(afq_ctx* ctx, int W, int Na, int Nb, int G, int gchunk_re
   return AFQ_OK;
 }
 
+// sign (optional, (G,)): ERI signs of the vectors, Coulomb and exchange
+// weighted by them (afqmc_pw_heg.py:414-457)
 static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, int Nb, int G,
                     const double* Gha, const double* Ghb, const double* ra, const double* rb,
-                    bool rc_cplx, double* e_coul, double* e_exch) {
+                    bool rc_cplx, double* e_coul, double* e_exch, const double* sign = nullptr) {
   int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, sc.X);
   if (rc) return rc;
   hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)sc.X, (cplx*)e_coul,
-                     nullptr);
+                     sign);
   AFQ_CHECK_LAUNCH();
-  rc = exchange_fp64(s, W, M, Na, G, Gha, ra, rc_cplx, sc.gchunk, sc.part, e_exch, false);
+  rc = exchange_fp64(s, W, M, Na, G, Gha, ra, rc_cplx, sc.gchunk, sc.part, e_exch, false, sign);
   if (rc) return rc;
-  return exchange_fp64(s, W, M, Nb, G, Ghb, rb, rc_cplx, sc.gchunk, sc.part, e_exch, true);
+  return exchange_fp64(s, W, M, Nb, G, Ghb, rb, rc_cplx, sc.gchunk, sc.part, e_exch, true, sign);
 }
 
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
@@ -304,10 +306,50 @@ int afq_local_energy_2body(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, co
                        e_coul, e_exch);
 }
 
+int afq_local_energy_2body_pw(afq_ctx* ctx, int W, int M, int Na, int Nb, int G,
+                              const double* Gha, const double* Ghb, const double* ra,
+                              const double* rb, int dtype, const double* chol_sign,
+                              double* e_coul, double* e_exch) {
+  AFQ_REQUIRE(ctx, "ctx is NULL");
+  AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
+  AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
+  AFQ_REQUIRE(chol_sign || G == 0, "chol_sign is NULL");
+  if (W == 0) return AFQ_OK;
+  afq::E2bScratch sc;
+  int rc = afq::e2b_scratch(ctx, W, Na, Nb, G, 0, &sc);
+  if (rc) return rc;
+  return afq::two_body(ctx->stream, sc, W, M, Na, Nb, G, Gha, Ghb, ra, rb, dtype == AFQ_COMPLEX,
+                       e_coul, e_exch, chol_sign);
+}
+
+static int local_energy_impl(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* h1e,
+                             const double* Ga, const double* Gb, const double* Gha,
+                             const double* Ghb, const double* ra, const double* rb, int dtype,
+                             double enuc, const double* sign, double* e_tot, double* e_1b,
+                             double* e_2b);
+
+int afq_local_energy_pw(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* h1e,
+                        const double* Ga, const double* Gb, const double* Gha, const double* Ghb,
+                        const double* ra, const double* rb, int dtype, double enuc,
+                        const double* chol_sign, double* e_tot, double* e_1b, double* e_2b) {
+  AFQ_REQUIRE(chol_sign || G == 0, "chol_sign is NULL");
+  return local_energy_impl(ctx, W, M, Na, Nb, G, h1e, Ga, Gb, Gha, Ghb, ra, rb, dtype, enuc,
+                           chol_sign, e_tot, e_1b, e_2b);
+}
+
 int afq_local_energy(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* h1e,
                      const double* Ga, const double* Gb, const double* Gha, const double* Ghb,
                      const double* ra, const double* rb, int dtype, double enuc, double* e_tot,
                      double* e_1b, double* e_2b) {
+  return local_energy_impl(ctx, W, M, Na, Nb, G, h1e, Ga, Gb, Gha, Ghb, ra, rb, dtype, enuc,
+                           nullptr, e_tot, e_1b, e_2b);
+}
+
+static int local_energy_impl(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const double* h1e,
+                             const double* Ga, const double* Gb, const double* Gha,
+                             const double* Ghb, const double* ra, const double* rb, int dtype,
+                             double enuc, const double* sign, double* e_tot, double* e_1b,
+                             double* e_2b) {
   AFQ_REQUIRE(ctx, "ctx is NULL");
   AFQ_REQUIRE(W >= 0 && M > 0 && Na >= 0 && Nb >= 0 && G >= 0, "bad shape");
   AFQ_REQUIRE(dtype == AFQ_REAL || dtype == AFQ_COMPLEX, "bad dtype %d", dtype);
@@ -325,7 +367,7 @@ int afq_local_energy(afq_ctx* ctx, int W, int M, int Na, int Nb, int G, const do
   int rc = afq::e2b_scratch(ctx, W, Na, Nb, G, 0, &sc);
   if (rc) return rc;
   rc = afq::two_body(s, sc, W, M, Na, Nb, G, Gha, Ghb, ra, rb, dtype == AFQ_COMPLEX, sc.slot[0],
-                     sc.slot[1]);
+                     sc.slot[1], sign);
   if (rc) return rc;
   return afq::combine_launch(s, W, e_1b, sc.slot[0], sc.slot[1], e_tot, e_2b);
 }

@@ -28,7 +28,9 @@ import sys
 import numpy as np
 import torch
 
-from .afqmc_gto import C128, F64, System, half_rotate_cholesky
+from . import _lib
+from ._lib import check, ptr
+from .afqmc_gto import C128, F64, System, _c, _r, _real_or_complex, half_rotate_cholesky
 from .driver import AFQMCDriver
 
 
@@ -228,6 +230,45 @@ def hf_energy_3deg(h1e, chol, chol_sign, trial_up, trial_dn, enuc):
             T = np.einsum('gij,jk->gik', chol, G)
             e -= 0.5 * float(np.einsum('g,gij,gji->', chol_sign, T, T))
     return e
+
+
+def local_energy_2body_pw(Ghalfa, Ghalfb, rchol_a, rchol_b, chol_sign):
+    """afqmc_pw_heg.py:414-457 -> (e_coul, e_exch) with the ERI sign of each
+    vector (afq_local_energy_2body_pw)."""
+    Gha, Ghb = _c(Ghalfa), _c(Ghalfb)
+    ra, dta = _real_or_complex(rchol_a)
+    rb, dtb = _real_or_complex(rchol_b)
+    if dta != dtb:
+        ra, rb, dta = _c(ra), _c(rb), _lib.AFQ_COMPLEX
+    sg = _r(chol_sign)
+    W, Na, M = Gha.shape
+    Nb, G = Ghb.shape[1], ra.shape[0]
+    ec = torch.empty((W,), dtype=C128, device=Gha.device)
+    ex = torch.empty((W,), dtype=C128, device=Gha.device)
+    check(_lib.load().afq_local_energy_2body_pw(_lib.context().handle, W, M, Na, Nb, G, ptr(Gha),
+                                                ptr(Ghb), ptr(ra), ptr(rb), dta, ptr(sg), ptr(ec),
+                                                ptr(ex)), "local_energy_2body_pw")
+    return ec, ex
+
+
+def local_energy_pw(h1e, chol, Ga, Gb, Ghalfa, Ghalfb, rchol_a, rchol_b, enuc, chol_sign):
+    """afqmc_pw_heg.py:460-471 -> (e_tot, e_1b, e_2b) (``chol`` is not read,
+    as in the reference)."""
+    Ga, Gb, Gha, Ghb = _c(Ga), _c(Gb), _c(Ghalfa), _c(Ghalfb)
+    h, dth = _real_or_complex(h1e)
+    ra, dta = _real_or_complex(rchol_a)
+    rb, dtb = _real_or_complex(rchol_b)
+    if len({dth, dta, dtb}) > 1:
+        h, ra, rb, dth = _c(h), _c(ra), _c(rb), _lib.AFQ_COMPLEX
+    sg = _r(chol_sign)
+    W, Na, M = Gha.shape
+    Nb, G = Ghb.shape[1], ra.shape[0]
+    out = [torch.empty((W,), dtype=C128, device=Gha.device) for _ in range(3)]
+    check(_lib.load().afq_local_energy_pw(_lib.context().handle, W, M, Na, Nb, G, ptr(h), ptr(Ga),
+                                          ptr(Gb), ptr(Gha), ptr(Ghb), ptr(ra), ptr(rb), dth,
+                                          float(enuc), ptr(sg), ptr(out[0]), ptr(out[1]),
+                                          ptr(out[2])), "local_energy_pw")
+    return tuple(out)
 
 
 # ---------------------------------------------------------------------------

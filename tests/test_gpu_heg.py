@@ -202,3 +202,27 @@ def test_bf16x3_refuses_nonsymmetric_vectors(afq):
     with pytest.raises(afq.AFQError, match="symmetric"):
         System(prop, _t(chol), ra, ra, tu.cuda(), td.cuda(), precision="bf16x3")
     System(prop, _t(chol), ra, ra, tu.cuda(), td.cuda(), precision="fp32")
+
+
+def test_local_energy_pw_functions_match_oracle(afq):
+    """afqmc_pw_heg.py:414-471 mirrors (signed Coulomb and exchange) against
+    oracle.heg on the same Green's functions."""
+    from omegaqmc_amd import heg as H
+    s, d = _driver(2.0, 14, 19)
+    prop, L, ra, rb, tu, td, sign, phase = _oracle_inputs(d)
+    W, M = 6, s['N_pw']
+    rng = np.random.default_rng(31)
+    pa = tu + 0.2 * (rng.normal(size=(W, M, 7)) + 1j * rng.normal(size=(W, M, 7)))
+    pb = td + 0.2 * (rng.normal(size=(W, M, 7)) + 1j * rng.normal(size=(W, M, 7)))
+    Ga, Gb, Gha, Ghb, _ = A.greens_function(pa, pb, tu.astype(complex), td.astype(complex))
+    ref = OH.local_energy_pw(_n(d.h1e), Ga, Gb, Gha, Ghb, ra, rb, d.enuc, sign)
+    got = H.local_energy_pw(d.h1e, d.chol, _t(Ga), _t(Gb), _t(Gha), _t(Ghb), _t(ra), _t(rb),
+                            d.enuc, d.chol_sign)
+    for x, y in zip(got, ref):
+        assert rel_err(_n(x), y) < 1e-11
+    ec, ex = H.local_energy_2body_pw(_t(Gha), _t(Ghb), _t(ra), _t(rb), d.chol_sign)
+    np.testing.assert_allclose(_n(ec - ex), ref[2], rtol=1e-11, atol=1e-12)
+    # all signs +1 is the plain two-body energy
+    ec1, ex1 = H.local_energy_2body_pw(_t(Gha), _t(Ghb), _t(ra), _t(rb), _t(np.ones(len(sign))))
+    ec0, ex0 = afq.local_energy_2body(_t(Gha), _t(Ghb), _t(ra), _t(rb))
+    assert np.array_equal(_n(ec1), _n(ec0)) and np.array_equal(_n(ex1), _n(ex0))
