@@ -25,7 +25,7 @@ from .afqmc_gto import (greens_function_multidet_force_bias,  # noqa: F401
 from .driver import AFQMCDriver, get_afqmc_func_from_integrals  # noqa: F401
 from .estream import EStreamDriver, get_afqmc_estream_func_from_integrals  # noqa: F401
 from .heg import HEGDriver, build_3deg_system, get_afqmc_3deg_func  # noqa: F401
-from .qed import (QEDDriver, get_qed_afqmc_func_from_integrals,  # noqa: F401
+from .qed import (QEDDriver, QEDWalkers, get_qed_afqmc_func_from_integrals,  # noqa: F401
                   prepare_qed_integrals_from_arrays, propagate_qed_walkers, qed_local_energy)
 from .edga import load_edga_result, make_omegaqmc_trial, select_top_determinants  # noqa: F401
 

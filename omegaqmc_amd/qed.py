@@ -41,7 +41,7 @@ from .afqmc_gto import (C128, F64, FBBOUND_DEFAULT, WEIGHT_CLIP_FRACTION, Philox
                         force_bias, greens_function, greens_function_force_bias,
                         greens_function_overlap, half_rotate_cholesky, _build_vhs,
                         _apply_exp_vhs, _update_weights_phaseless, apply_one_body,
-                        population_control_comb)
+                        population_control_comb, Walkers)
 
 PHOTON_TAYLOR_ORDER = 4  # qed_afqmc_gto.py:387 and :424
 
@@ -49,6 +49,15 @@ PHOTON_TAYLOR_ORDER = 4  # qed_afqmc_gto.py:387 and :424
 # ---------------------------------------------------------------------------
 # Integrals and propagator (integrals/qed.py, qed_afqmc_gto.py:204-259)
 # ---------------------------------------------------------------------------
+
+class QEDWalkers(Walkers):
+    """qed_afqmc_gto.py:80-102: the walker container plus the photon
+    displacement coordinate q of each walker (all q0)."""
+
+    def __init__(self, trial_up, trial_dn, nwalkers, nbasis, nup, ndown, q0=0.0):
+        super().__init__(trial_up, trial_dn, nwalkers, nbasis, nup, ndown)
+        self.q = torch.full((self.nwalkers,), float(q0), dtype=F64, device=self.weights.device)
+
 
 def prepare_qed_integrals_from_arrays(h1e, chol_mo, dip_mo, enuc, nelec, quad_mo=None,
                                       omega=None):

@@ -189,3 +189,12 @@ def test_energy_at_trial_closed_form(qed):
     np.testing.assert_allclose(_n(e_ph).real, drv.omega / 2.0 * (drv.s - 1.0 + drv.q0 ** 2),
                                rtol=1e-13)
     assert abs(drv.q0 + D / np.sqrt(drv.omega)) < 1e-12 * max(1.0, abs(drv.q0))
+
+
+def test_qed_walkers_container(qed):
+    """qed_afqmc_gto.py:80-102: trial copies with unit weights and q = q0."""
+    import torch
+    tu = torch.eye(5, 2, dtype=torch.float64)
+    wk = qed.QEDWalkers(tu, tu, 3, 5, 2, 2, q0=-0.25)
+    assert wk.phia.shape == (3, 5, 2) and wk.phia.is_cuda
+    assert torch.all(wk.q == -0.25) and torch.all(wk.weights == 1)
