@@ -69,10 +69,12 @@ __device__ __forceinline__ void split4(f4v v, uint2& hi, uint2& lo) {
   lo.y = pk_bf16(r2, r3);
 }
 
-// 16-byte chunk position of K chunk c in row li of a 16-row block: the XOR
-// keeps MFMA operand reads (16 rows, one chunk) free of bank conflicts with
-// either 32 or 64 LDS banks
-__device__ __forceinline__ int xswz(int li, int c) { return c ^ (((li >> 1) ^ (li >> 3)) & 3); }
+// 16-byte chunk position of K chunk c in row li of a 16-row block (rows of 64
+// bytes).  Measured on gfx950 with a ds_read_b128 probe: reading one chunk of
+// all 16 rows takes 4 LDS cycles per wave with this XOR, against 8 with no
+// swizzle or with c ^ ((li >> 1) ^ (li >> 3)) & 3, although every one of
+// these maps the 16 lanes of a quarter-wave to distinct 16-byte bank groups
+__device__ __forceinline__ int xswz(int li, int c) { return c ^ ((li >> 1) & 3); }
 
 __device__ __forceinline__ uint4 neg8(uint4 a) {
   return make_uint4(a.x ^ 0x80008000u, a.y ^ 0x80008000u, a.z ^ 0x80008000u, a.w ^ 0x80008000u);
@@ -957,21 +959,30 @@ int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, fl
 // ----------------------------------------------------------------------------
 namespace {
 
-constexpr int XNW = 4, XNT = 64 * XNW, XWT = 80, XRT = 2 * XWT, XCT = 2 * XWT, XGR = 8;
+// Workgroup: XWR x XWC waves of XWT x XWT, i.e. XRT rows (g, i) by XCT columns
+// (walker, re/im, j).
+constexpr int XWT = 80, XWR = 4, XWC = 2, XNW = XWR * XWC, XNT = 64 * XNW;
+constexpr int XWPC = 1;  // workgroups per CU (131 KB of LDS each)
+constexpr int XRT = XWR * XWT, XCT = XWC * XWT, XGR = 8;
 constexpr int XMAXN = XWT / 2;
-constexpr int XSLOT = (2 * XRT) * 4 / XNT;  // 16-byte staging slots per thread and operand
-// One wave's epilogue block: the T planes of one g block, T_c[wi][i][j] at
-// ((2 wi + c) N + i) XP + j (c = 0 real, 1 imaginary; nw 2N <= XWT rows of XP
-// floats), then a dummy area of 256 floats for the lanes of columns outside
-// the tile.  XP = 4 mod 8 keeps both the column writes of 16 lanes and the
-// 16-byte row reads of 8 or 16 lanes on distinct banks (32 or 64 banks).
+// 16-row blocks of one staging buffer: A hi, A lo (XRT rows each), then B hi,
+// B lo (XCT rows each); wave wv fills blocks wv + XNW i
+constexpr int XABLK = 2 * XRT / 16, XNBLK = XABLK + 2 * XCT / 16;
+constexpr int XSLOT = (XNBLK + XNW - 1) / XNW;
+// One wave's epilogue block: one plane c (real or imaginary) of the T blocks
+// of one g block, T_c[wi][i][j] at (wi N + i) XP + j (nw N <= XWT / 2 rows of
+// XP floats), then a dummy area of 256 floats for the lanes whose values
+// belong to the other plane or to columns outside the tile.  The epilogue
+// blocks follow staging buffer 0, which receives the next tile's first K step
+// while the epilogue runs.
 constexpr int XP = 44;
-constexpr int XDUM = XWT * XP;  // floats
+constexpr int XDUM = (XWT / 2) * XP;  // floats
 constexpr int XEPI = XDUM + 256;
-static_assert(XP >= XMAXN && XP % 8 == 4 && XDUM % 4 == 0 && XEPI % 4 == 0, "epilogue layout");
-// one staging buffer: planes (A hi, A lo, B hi, B lo) of XRT rows x GK, rows
-// of 64 bytes in blocks of 16 rows written by one LDS-DMA instruction each
-constexpr int XSTAGE = 4 * XRT * GK;  // bf16
+static_assert(XP >= XMAXN && XDUM % 4 == 0 && XEPI % 4 == 0, "epilogue layout");
+// one staging buffer: planes (A hi, A lo) of XRT rows and (B hi, B lo) of XCT
+// rows, each row GK bf16 (64 bytes), in blocks of 16 rows written by one
+// LDS-DMA instruction each
+constexpr int XSTAGE = XNBLK * 16 * GK;  // bf16
 
 
 
@@ -993,20 +1004,22 @@ __device__ __forceinline__ void xwave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroup tile: 160 rows (g, i) x 160 columns (walker, re/im, j), 4 waves of
-// 80 x 80 in a 2 x 2 grid, K step 32; two workgroups per CU.  The grid is
-// persistent: XCD x owns a contiguous range of logical tiles and its
-// workgroups take them in turn, so at any time an XCD works on neighbouring
-// tiles (inside a group of XGR row tiles the row tile runs fastest, so the L2
-// holds both operands).  Operand chunks go from global memory straight into
-// two LDS stages by buffer-load DMA (no staging registers): the next K step's
-// chunk is in flight while this one computes, one barrier per step.  The
-// epilogue reuses both stages.
+// Workgroup tile: 320 rows (g, i) x 160 columns (walker, re/im, j), 8 waves of
+// 80 x 80 in a 4 x 2 grid, K step 32; one workgroup per CU.  The operand
+// bytes moved into LDS per output bound this kernel (half the DMA volume runs
+// it a quarter faster), so the tile is as large as the LDS holds: two stages
+// of 60 KB.  The grid is persistent: XCD x owns a contiguous range of logical
+// tiles and its workgroups take them in turn, so at any time an XCD works on
+// neighbouring tiles (inside a group of XGR row tiles the row tile runs
+// fastest, so the L2 holds both operands).  Operand chunks go from global
+// memory straight into two LDS stages by buffer-load DMA (no staging
+// registers): the next K step's chunks are issued at the start of this one,
+// one barrier per step.  The epilogue reuses both stages.
 template <bool VEC>
-__global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
+__global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
   extern __shared__ __align__(16) bf16_t lds[];
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
+  const int wr = wv / XWC, wc = wv % XWC;
   const int li = l & 15, lq = l >> 4;
   const int N = a.N, NR = a.ng * N, NC = a.nw * 2 * N;
   const int nx = gridDim.x >> 3, jx = blockIdx.x >> 3;
@@ -1031,66 +1044,71 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
 
   const i32x4 rsa = buf_rsrc(a.a, 4u * a.aplane), rsb = buf_rsrc(a.b, 4u * a.bplane);
 
-  // DMA slots: wave wv's instruction i fills 16-row block wv + 4 i of the A
-  // planes (rows 0..2*XRT-1 = (plane, row)) and of the B planes; lane l
+  // DMA slots: wave wv's instruction i fills 16-row block b = wv + XNW i of
+  // the stage (blocks < XABLK: A rows (plane, row), then B rows); lane l
   // fetches row l / 4 of the block, K chunk xswz(l / 4, l % 4).  An empty
   // slot (row outside the tile's g blocks or walkers) reads offset 0: it only
   // feeds outputs the epilogue ignores.
+  const int wvu = __builtin_amdgcn_readfirstlane(wv);
   const int lrow = l >> 2, lkc = xswz(lrow, l & 3);
-  int oa[XSLOT], ob[XSLOT];
+  int off[XSLOT];
   auto slots = [&](int tr, int tc) {
     int tt = t;  // opaque per tile: keeps the index arithmetic out of registers between tiles
     asm volatile("" : "+v"(tt));
-    const int ww = tt >> 6, lr = (tt & 63) >> 2;
+    const int lr = (tt & 63) >> 2;
 #pragma unroll
     for (int i = 0; i < XSLOT; ++i) {
-      const int rp = 16 * (ww + 4 * i) + lr, p = rp >= XRT, r = rp - p * XRT;
-      const int wrr = r >= XWT, rr = r - wrr * XWT;  // row-wave half, row in it
-      {
+      const int b = wvu + XNW * i;  // uniform
+      off[i] = 0;
+      if (b < XABLK) {
+        const int p = b >= XRT / 16, r = 16 * (b - p * (XRT / 16)) + lr;
+        const int wrr = r / XWT, rr = r - wrr * XWT;  // row wave, row in it
         const int gq = (rr * a.mN) >> 16, ii = rr - gq * N;
-        const int g = (tr * 2 + wrr) * a.ng + gq;
+        const int g = (tr * XWR + wrr) * a.ng + gq;
         const bool ok = rr < NR && g < a.G;
-        oa[i] = ok ? 2 * (p * a.aplane + g * a.ka + ii * a.ldq + 8 * lkc) : 0;
-      }
-      {
+        off[i] = ok ? 2 * (p * a.aplane + g * a.ka + ii * a.ldq + 8 * lkc) : 0;
+      } else if (b < XNBLK) {
+        const int bb = b - XABLK, p = bb >= XCT / 16, r = 16 * (bb - p * (XCT / 16)) + lr;
+        const int wcc = r / XWT, rr = r - wcc * XWT;  // column wave, column in it
         const int wq = (rr * a.m2N) >> 16, rem = rr - wq * 2 * N;
         const int ri = rem >= N, j = rem - ri * N;
-        const int w = (tc * 2 + wrr) * a.nw + wq;
+        const int w = (tc * XWC + wcc) * a.nw + wq;
         const bool ok = rr < NC && w < a.W;
-        ob[i] = ok ? 2 * (p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * lkc) : 0;
+        off[i] = ok ? 2 * (p * a.bplane + (2 * w + ri) * a.kb + j * a.ldq + 8 * lkc) : 0;
       }
     }
   };
-  // chunk at K offset k0 into stage st (rows are ldq long, a multiple of GK,
+  // chunks at K offset k0 into stage st (rows are ldq long, a multiple of GK,
   // zero past M: chunks never run into the next row)
   auto fetch_slot = [&](int st, int k0, int i) {
     const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
-    const int blk = wv + 4 * i;
-    dma16(rsa, base + blk * 1024, oa[i], 2 * k0);
-    dma16(rsb, base + (2 * XRT / 16 + blk) * 1024, ob[i], 2 * k0);
+    const int b = wvu + XNW * i;  // 16-row block, uniform
+    if (b < XABLK) dma16(rsa, base + b * 1024, off[i], 2 * k0);
+    else if (b < XNBLK) dma16(rsb, base + b * 1024, off[i], 2 * k0);
   };
   auto fetch = [&](int st, int k0) {
-    const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
 #pragma unroll
-    for (int i = 0; i < XSLOT; ++i) {
-      const int blk = wv + 4 * i;  // 16-row block, uniform
-      dma16(rsa, base + blk * 1024, oa[i], 2 * k0);
-      dma16(rsb, base + (2 * XRT / 16 + blk) * 1024, ob[i], 2 * k0);
-    }
+    for (int i = 0; i < XSLOT; ++i) fetch_slot(st, k0, i);
   };
   auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
   const int nk = a.ldq / GK;
 
-  float* Tw = (float*)lds + wv * XEPI;
-  double* wsum = (double*)((float*)lds + XNW * XEPI);  // [XNW][nw][2]
+  float* Tw = (float*)(lds + XSTAGE) + wv * XEPI;
+  double* wsum = (double*)((float*)(lds + XSTAGE) + XNW * XEPI);  // [XNW][nw][2]
   // lane offset of (row li, chunk lq) inside a 16-row block, in bf16
   const int frag = li * GK + 8 * xswz(li, lq);
 
-  for (; v < v1; v = next_tile(v + nx)) {
+  {
     int tr, tc;
     tile_of(v, tr, tc);
     slots(tr, tc);
     fetch(0, 0);
+  }
+  while (v < v1) {
+    int tr, tc;
+    tile_of(v, tr, tc);
+    // stage 0 holds this tile's first K step; the barrier also ends the
+    // previous tile's epilogue, whose area the K loop's DMA overwrites
     dma_wait();
     __syncthreads();
 
@@ -1112,29 +1130,45 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       for (int w = 0; w < 5; ++w) {
         const int c = XWT * wc + 16 * w;
         const uint4 bh = *(const uint4*)(st + (2 * XRT + c) * GK);
-        const uint4 bl = *(const uint4*)(st + (3 * XRT + c) * GK);
-        if (kt + 1 < nk) fetch_slot((kt + 1) & 1, (kt + 1) * GK, w);
+        const uint4 bl = *(const uint4*)(st + (2 * XRT + XCT + c) * GK);
 #pragma unroll
-        for (int u = 0; u < 5; ++u) acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
+        for (int u = 0; u < 5; ++u) {
+          acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
+          // the next K step's DMA slots, one after each of the first product
+          // groups: issued all at once they back up the vector memory queue
+          // and stall the wave before its products
+          const int q = 5 * w + u;  // static
+          if (q < XSLOT && kt + 1 < nk) fetch_slot((kt + 1) & 1, (kt + 1) * GK, q);
+        }
       }
       dma_wait();
       __syncthreads();
     }
 
+    // the next tile's first K step goes to stage 0 while the epilogue runs
+    const int vn = next_tile(v + nx);
+    if (vn < v1) {
+      int trn, tcn;
+      tile_of(vn, trn, tcn);
+      slots(trn, tcn);
+      fetch(0, 0);
+    }
+
     // Epilogue: E(g, w) = sum_ij T_ij T_ji for the g blocks and walkers of
-    // the wave tile.  Pass gi covers g block gi: the lanes holding its rows
-    // write them into the wave's planes T_c[wi][i][j] (a lane holds rows
-    // 4 lq .. 4 lq + 3 of one column), and after a wave barrier each reads
-    // the transposed partners T_c[wi][j][i .. i + 3] of its rows, one
-    // 16-byte load per plane (one load per element when N % 4 != 0, where a
-    // lane's rows can span two g blocks).  With T = Tr + i Ti,
+    // the wave tile.  With T = Tr + i Ti,
     //   Re E = sum Tr_ij Tr_ji - sum Ti_ij Ti_ji,
-    //   Im E = sum Tr_ij Ti_ji + sum Ti_ij Tr_ji,
-    // so every lane reads the partner in its own plane (A) and in the other
-    // one (B).  Products and a lane's partial sum over
-    // one g block (at most 16 terms) are FP32, the sums over g blocks, lanes
-    // and tiles FP64, all in a fixed order.
-    int wb[5], rb[5], ro[5], wiv[5];
+    //   Im E = sum Tr_ij Ti_ji + sum Ti_ij Tr_ji.
+    // Pass (gi, c) covers plane c of g block gi: the lanes holding values of
+    // that plane write them into the wave's block T_c[wi][i][j] (a lane holds
+    // rows 4 lq .. 4 lq + 3 of one column), and after a wave barrier every
+    // lane reads the transposed partners T_c[wi][j][i .. i + 3] of its rows,
+    // one 16-byte load (one load per element when N % 4 != 0, where a lane's
+    // rows can span two g blocks).  A lane of plane c adds the products to
+    // its own-plane sum A, a lane of the other plane to its cross sum B.
+    // Products and a lane's partial sums over one g block (at most 16 terms
+    // per plane) are FP32, the sums over g blocks, lanes and tiles FP64, all
+    // in a fixed order.
+    int wb[5], rb[5], wiv[5];
     bool cvv[5], cimv[5];
     {
       int lj = li;  // opaque per tile, as in slots()
@@ -1150,92 +1184,127 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
         cvv[w] = cv;
         cimv[w] = cim;
         wiv[w] = wi;
-        wb[w] = cv ? (2 * wi + (cim ? 1 : 0)) * N * XP + j : XDUM + l;  // + i XP
-        rb[w] = cv ? ((2 * wi + (cim ? 1 : 0)) * N + j) * XP : XDUM + 4 * l;  // + i
-        ro[w] = cv ? ((2 * wi + (cim ? 0 : 1)) * N + j) * XP : XDUM + 4 * l;
+        wb[w] = wi * N * XP + j;    // + i XP: this lane's element (i, j)
+        rb[w] = (wi * N + j) * XP;  // + i: its partner (j, i)
       }
     }
     double cA[5], cB[5];
 #pragma unroll
     for (int w = 0; w < 5; ++w) cA[w] = cB[w] = 0.0;
-    const int gw0 = (tr * 2 + wr) * a.ng;
-    for (int gi = 0; gi < a.ng; ++gi) {
-      if (gw0 + gi >= a.G) break;  // wave-uniform
-      const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
-#pragma unroll
-      for (int u = 0; u < 5; ++u) {
-        if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
-        if (VEC) {
-          const int r0 = 16 * u + 4 * lq;
-          if (r0 >= lo && r0 < hi) {
-            const int i0 = r0 - lo;
-#pragma unroll
-            for (int w = 0; w < 5; ++w) {
-              float* d = Tw + wb[w] + (cvv[w] ? i0 * XP : 0);
-#pragma unroll
-              for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
-            }
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = 16 * u + 4 * lq + r;
-            const bool own = row >= lo && row < hi;
-#pragma unroll
-            for (int w = 0; w < 5; ++w)
-              Tw[own && cvv[w] ? wb[w] + (row - lo) * XP : XDUM + l] = acc[u][w][r];
-          }
-        }
-      }
-      xwave_sync();
-      float A[5], B[5];
-#pragma unroll
-      for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
-#pragma unroll
-      for (int u = 0; u < 5; ++u) {
-        if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
-        if (VEC) {
-          const int r0 = 16 * u + 4 * lq;
-          if (r0 >= lo && r0 < hi) {
-            const int i0 = r0 - lo;
-#pragma unroll
-            for (int w = 0; w < 5; ++w) {
-              const int di = cvv[w] ? i0 : 0;
-              const float4 ps = *(const float4*)(Tw + rb[w] + di);
-              const float4 pi = *(const float4*)(Tw + ro[w] + di);
-              const f4v x = acc[u][w];
-              A[w] = fmaf(x[0], ps.x, A[w]);
-              A[w] = fmaf(x[1], ps.y, A[w]);
-              A[w] = fmaf(x[2], ps.z, A[w]);
-              A[w] = fmaf(x[3], ps.w, A[w]);
-              B[w] = fmaf(x[0], pi.x, B[w]);
-              B[w] = fmaf(x[1], pi.y, B[w]);
-              B[w] = fmaf(x[2], pi.z, B[w]);
-              B[w] = fmaf(x[3], pi.w, B[w]);
-            }
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = 16 * u + 4 * lq + r;
-            const bool own = row >= lo && row < hi;
-#pragma unroll
-            for (int w = 0; w < 5; ++w) {
-              const bool ok = own && cvv[w];
-              const int di = ok ? row - lo : 0;
-              const float x = ok ? acc[u][w][r] : 0.f;
-              A[w] = fmaf(x, Tw[rb[w] + di], A[w]);
-              B[w] = fmaf(x, Tw[ro[w] + di], B[w]);
-            }
-          }
-        }
-      }
+    const int gw0 = (tr * XWR + wr) * a.ng;
+    if (VEC) {
+      // a lane's rows 4 lq + 16 u + (0..3) are rows i0 .. i0 + 3 of g block gi
+      // (i0 = 16 u + 4 lq - gi N, whole when N % 4 == 0); its element (i, j)
+      // sits at wpos + (16 u - gi N) XP, the partners (j, i0 .. i0 + 3) at
+      // rpos + 16 u - gi N
+      int wpos[5], rpos[5];
 #pragma unroll
       for (int w = 0; w < 5; ++w) {
-        cA[w] += (double)A[w];
-        cB[w] += (double)B[w];
+        wpos[w] = wb[w] + 4 * lq * XP;
+        rpos[w] = cvv[w] ? rb[w] + 4 * lq : 0;
       }
-      xwave_sync();
+      for (int gi = 0; gi < a.ng; ++gi) {
+        if (gw0 + gi >= a.G) break;  // wave-uniform
+        const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
+        float S[2][5];  // S[c][w]: the pass-c sums (plane c partners)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int w = 0; w < 5; ++w) S[c][w] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+          for (int u = 0; u < 5; ++u) {
+            if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+            const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
+            if (r0 >= lo && r0 < hi) {
+#pragma unroll
+              for (int w = 0; w < 5; ++w) {
+                if (cvv[w] && (int)cimv[w] == c) {  // only the lanes of plane c write
+                  float* d = Tw + wpos[w] + du * XP;
+#pragma unroll
+                  for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
+                }
+              }
+            }
+          }
+          xwave_sync();
+#pragma unroll
+          for (int u = 0; u < 5; ++u) {
+            if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+            const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
+            if (r0 >= lo && r0 < hi) {
+              float4 ps[5];  // all five partner loads in flight together
+#pragma unroll
+              for (int w = 0; w < 5; ++w) ps[w] = *(const float4*)(Tw + rpos[w] + du);
+#pragma unroll
+              for (int w = 0; w < 5; ++w) {
+                const f4v x = acc[u][w];
+                S[c][w] = fmaf(x[0], ps[w].x, S[c][w]);
+                S[c][w] = fmaf(x[1], ps[w].y, S[c][w]);
+                S[c][w] = fmaf(x[2], ps[w].z, S[c][w]);
+                S[c][w] = fmaf(x[3], ps[w].w, S[c][w]);
+              }
+            }
+          }
+          xwave_sync();
+        }
+        // own-plane partners feed A, the other plane's feed B
+#pragma unroll
+        for (int w = 0; w < 5; ++w) {
+          cA[w] += (double)(cimv[w] ? S[1][w] : S[0][w]);
+          cB[w] += (double)(cimv[w] ? S[0][w] : S[1][w]);
+        }
+      }
+    } else {
+      for (int gi = 0; gi < a.ng; ++gi) {
+        if (gw0 + gi >= a.G) break;  // wave-uniform
+        const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
+        float A[5], B[5];
+#pragma unroll
+        for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+          for (int u = 0; u < 5; ++u) {
+            if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = 16 * u + 4 * lq + r;
+              const bool inb = row >= lo && row < hi;
+#pragma unroll
+              for (int w = 0; w < 5; ++w) {
+                const bool own = inb && cvv[w] && (int)cimv[w] == c;
+                Tw[own ? wb[w] + (row - lo) * XP : XDUM + l] = acc[u][w][r];
+              }
+            }
+          }
+          xwave_sync();
+#pragma unroll
+          for (int u = 0; u < 5; ++u) {
+            if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = 16 * u + 4 * lq + r;
+              const bool inb = row >= lo && row < hi;
+#pragma unroll
+              for (int w = 0; w < 5; ++w) {
+                const bool ok = inb && cvv[w];
+                const float x = ok ? acc[u][w][r] : 0.f;
+                const float pv = Tw[ok ? rb[w] + row - lo : XDUM + l];
+                if ((int)cimv[w] == c) A[w] = fmaf(x, pv, A[w]);
+                else B[w] = fmaf(x, pv, B[w]);
+              }
+            }
+          }
+          xwave_sync();
+        }
+#pragma unroll
+        for (int w = 0; w < 5; ++w) {
+          cA[w] += (double)A[w];
+          cB[w] += (double)B[w];
+        }
+      }
     }
     // per walker: the lane's columns in a fixed order, then the wave
     for (int k = 0; k < a.nw; ++k) {  // uniform
@@ -1255,19 +1324,22 @@ __global__ __launch_bounds__(XNT, 2) void k_exch_bf3(Bf3Exch a) {
       }
     }
     __syncthreads();
-    // the two row waves of a column hold disjoint g ranges of the same walkers
-    if (t < 2 * a.nw) {
+    // the row waves of a column hold disjoint g ranges of the same walkers
+    if (t < XWC * a.nw) {
       const int wcc = t / a.nw, wi = t % a.nw;
-      const int w = (tc * 2 + wcc) * a.nw + wi;
+      const int w = (tc * XWC + wcc) * a.nw + wi;
       if (w < a.W) {
-        const double re = wsum[((wcc * a.nw) + wi) * 2] + wsum[(((2 + wcc) * a.nw) + wi) * 2];
-        const double im =
-            wsum[((wcc * a.nw) + wi) * 2 + 1] + wsum[(((2 + wcc) * a.nw) + wi) * 2 + 1];
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int r = 0; r < XWR; ++r) {
+          re += wsum[((r * XWC + wcc) * a.nw + wi) * 2];
+          im += wsum[((r * XWC + wcc) * a.nw + wi) * 2 + 1];
+        }
         a.part[((long)tr * a.W + w) * 2] = re;
         a.part[((long)tr * a.W + w) * 2 + 1] = im;
       }
     }
-    __syncthreads();  // the epilogue area is the staging of the next tile
+    v = vn;  // the barrier at the loop top protects wsum and the epilogue blocks
   }
 }
 
@@ -1327,13 +1399,14 @@ __global__ __launch_bounds__(256) void k_coul_parts(int W, int G, int nsplit,
 void exch_geom(int N, int G, int W, int* ng, int* nw, int* ntr, int* ntc) {
   *ng = N > 0 ? XWT / N : 1;
   *nw = N > 0 ? std::max(1, XWT / (2 * N)) : 1;
-  *ntr = cdiv_int(G, 2 * *ng);
-  *ntc = cdiv_int(W, 2 * *nw);
+  *ntr = cdiv_int(G, XWR * *ng);
+  *ntc = cdiv_int(W, XWC * *nw);
 }
 
 size_t exch_lds_bytes() {
   const size_t stages = (size_t)2 * XSTAGE * sizeof(bf16_t);
-  const size_t epi = (size_t)XNW * XEPI * sizeof(float) + (size_t)XNW * XWT * 2 * sizeof(double);
+  const size_t epi = (size_t)XSTAGE * sizeof(bf16_t) + (size_t)XNW * XEPI * sizeof(float) +
+                     (size_t)XNW * (XWT / 2) * 2 * sizeof(double);
   return std::max(stages, epi);
 }
 
@@ -1423,8 +1496,8 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     a.mN = (65536 + a.N - 1) / a.N;
     a.m2N = (65536 + 2 * a.N - 1) / (2 * a.N);
     if (a.ldq % GK) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: row stride %d", a.ldq);
-    // persistent: two workgroups per CU, a multiple of 8 (one share per XCD)
-    const long nwg = std::min<long>((long)num_cus() * 2, (nb + 7) / 8 * 8);
+    // persistent: one workgroup per CU, a multiple of 8 (one share per XCD)
+    const long nwg = std::min<long>((long)num_cus() * XWPC, (nb + 7) / 8 * 8);
     const dim3 grid((unsigned)std::max<long>(8, nwg / 8 * 8));
     if (a.N % 4 == 0)
       hipLaunchKernelGGL(k_exch_bf3<true>, grid, dim3(XNT), sm, s, a);
