@@ -959,32 +959,46 @@ int bf3_propagate(hipStream_t s, const Bf3System& b, int W, const bf16_t* Vb, fl
 // ----------------------------------------------------------------------------
 namespace {
 
-// Workgroup: XWR x XWC waves of XWT x XWT, i.e. XRT rows (g, i) by XCT columns
-// (walker, re/im, j).
-constexpr int XWT = 80, XWR = 4, XWC = 2, XNW = XWR * XWC, XNT = 64 * XNW;
-constexpr int XWPC = 1;  // workgroups per CU (131 KB of LDS each)
-constexpr int XRT = XWR * XWT, XCT = XWC * XWT, XGR = 8;
+// A wave owns an XWT x XWT tile of the GEMM with rows (g, i) and columns
+// (walker, re/im, j); a workgroup holds XCfg::WR x XWC waves.  Two shapes:
+//   wide (4 x 2 waves: 320 x 160, one workgroup per CU, 131 KB of LDS), for
+//     long K with at most two g blocks per wave: the epilogue runs in two
+//     plane passes in blocks after staging buffer 0, which receives the next
+//     tile's first K step meanwhile;
+//   narrow (2 x 2 waves: 160 x 160, two workgroups per CU, 80 KB each), for
+//     short K or many g blocks per wave: one epilogue pass per g block with
+//     both planes, over the staging buffers, while the other workgroup
+//     computes.
+constexpr int XWT = 80, XWC = 2, XGR = 8;
 constexpr int XMAXN = XWT / 2;
-// 16-row blocks of one staging buffer: A hi, A lo (XRT rows each), then B hi,
-// B lo (XCT rows each); wave wv fills blocks wv + XNW i
-constexpr int XABLK = 2 * XRT / 16, XNBLK = XABLK + 2 * XCT / 16;
-constexpr int XSLOT = (XNBLK + XNW - 1) / XNW;
-// One wave's epilogue block: one plane c (real or imaginary) of the T blocks
-// of one g block, T_c[wi][i][j] at (wi N + i) XP + j (nw N <= XWT / 2 rows of
-// XP floats), then a dummy area of 256 floats for the lanes whose values
-// belong to the other plane or to columns outside the tile.  The epilogue
-// blocks follow staging buffer 0, which receives the next tile's first K step
-// while the epilogue runs.
+// epilogue row pitch in floats: 4 mod 8 keeps the column writes of 16 lanes
+// and the 16-byte row reads on distinct banks
 constexpr int XP = 44;
-constexpr int XDUM = (XWT / 2) * XP;  // floats
-constexpr int XEPI = XDUM + 256;
-static_assert(XP >= XMAXN && XDUM % 4 == 0 && XEPI % 4 == 0, "epilogue layout");
-// one staging buffer: planes (A hi, A lo) of XRT rows and (B hi, B lo) of XCT
-// rows, each row GK bf16 (64 bytes), in blocks of 16 rows written by one
-// LDS-DMA instruction each
-constexpr int XSTAGE = XNBLK * 16 * GK;  // bf16
-
-
+template <bool WIDE>
+struct XCfg {
+  static constexpr int WR = WIDE ? 4 : 2, NW = WR * XWC, NT = 64 * NW, WPC = WIDE ? 1 : 2;
+  static constexpr int RT = WR * XWT, CT = XWC * XWT;
+  // 16-row blocks of one staging buffer: A hi, A lo (RT rows each), then B hi,
+  // B lo (CT rows each), rows of GK bf16 (64 bytes), each block written by one
+  // LDS-DMA instruction; wave wv fills blocks wv + NW i
+  static constexpr int ABLK = 2 * RT / 16, NBLK = ABLK + 2 * CT / 16;
+  static constexpr int SLOT = (NBLK + NW - 1) / NW;
+  static constexpr int STAGE = NBLK * 16 * GK;  // bf16
+  // one wave's epilogue block: the T planes of one g block, T_c[wi][i][j] at
+  // ((2 wi + c) N + i) XP + j (narrow: both planes, 2 nw N <= XWT rows) or
+  // one plane at (wi N + i) XP + j (wide: nw N <= XWT / 2 rows), then a dummy
+  // area of 256 floats for lanes whose values are not stored
+  static constexpr int EROWS = WIDE ? XWT / 2 : XWT;
+  static constexpr int DUM = EROWS * XP, EPI = DUM + 256;
+  static constexpr int EOFF = WIDE ? STAGE : 0;  // bf16 offset of the epilogue blocks
+  static size_t lds_bytes() {
+    const size_t stages = (size_t)2 * STAGE * sizeof(bf16_t);
+    const size_t epi = (size_t)EOFF * sizeof(bf16_t) + (size_t)NW * EPI * sizeof(float) +
+                       (size_t)NW * XMAXN * 2 * sizeof(double);  // + wsum [NW][nw <= XMAXN][2]
+    return std::max(stages, epi);
+  }
+};
+static_assert(XP >= XMAXN && XP % 8 == 4 && (XWT / 2) * XP % 4 == 0, "epilogue layout");
 
 struct Bf3Exch {
   const bf16_t* a;  // Lhalf hi plane (lo plane at +aplane): row (g, i) at g ka + i ldq
@@ -1004,19 +1018,17 @@ __device__ __forceinline__ void xwave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroup tile: 320 rows (g, i) x 160 columns (walker, re/im, j), 8 waves of
-// 80 x 80 in a 4 x 2 grid, K step 32; one workgroup per CU.  The operand
-// bytes moved into LDS per output bound this kernel (half the DMA volume runs
-// it a quarter faster), so the tile is as large as the LDS holds: two stages
-// of 60 KB.  The grid is persistent: XCD x owns a contiguous range of logical
-// tiles and its workgroups take them in turn, so at any time an XCD works on
-// neighbouring tiles (inside a group of XGR row tiles the row tile runs
-// fastest, so the L2 holds both operands).  Operand chunks go from global
-// memory straight into two LDS stages by buffer-load DMA (no staging
-// registers): the next K step's chunks are issued at the start of this one,
-// one barrier per step.  The epilogue reuses both stages.
-template <bool VEC>
-__global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
+// Persistent grid: XCD x owns a contiguous range of logical tiles and its
+// workgroups take them in turn, so at any time an XCD works on neighbouring
+// tiles (inside a group of XGR row tiles the row tile runs fastest, so the L2
+// holds both operands).  Operand chunks go from global memory straight into
+// two LDS stages by buffer-load DMA (no staging registers), one barrier per
+// K step; the next K step's chunks are issued between the first product
+// groups of this one (issued all at once they back up the vector memory queue
+// and stall the wave before its products).
+template <bool VEC, bool WIDE>
+__global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf3Exch a) {
+  using C = XCfg<WIDE>;
   extern __shared__ __align__(16) bf16_t lds[];
   const int t = threadIdx.x, l = t & 63, wv = t >> 6;
   const int wr = wv / XWC, wc = wv % XWC;
@@ -1044,31 +1056,31 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
 
   const i32x4 rsa = buf_rsrc(a.a, 4u * a.aplane), rsb = buf_rsrc(a.b, 4u * a.bplane);
 
-  // DMA slots: wave wv's instruction i fills 16-row block b = wv + XNW i of
-  // the stage (blocks < XABLK: A rows (plane, row), then B rows); lane l
+  // DMA slots: wave wv's instruction i fills 16-row block b = wv + C::NW i of
+  // the stage (blocks < C::ABLK: A rows (plane, row), then B rows); lane l
   // fetches row l / 4 of the block, K chunk xswz(l / 4, l % 4).  An empty
   // slot (row outside the tile's g blocks or walkers) reads offset 0: it only
   // feeds outputs the epilogue ignores.
   const int wvu = __builtin_amdgcn_readfirstlane(wv);
   const int lrow = l >> 2, lkc = xswz(lrow, l & 3);
-  int off[XSLOT];
+  int off[C::SLOT];
   auto slots = [&](int tr, int tc) {
     int tt = t;  // opaque per tile: keeps the index arithmetic out of registers between tiles
     asm volatile("" : "+v"(tt));
     const int lr = (tt & 63) >> 2;
 #pragma unroll
-    for (int i = 0; i < XSLOT; ++i) {
-      const int b = wvu + XNW * i;  // uniform
+    for (int i = 0; i < C::SLOT; ++i) {
+      const int b = wvu + C::NW * i;  // uniform
       off[i] = 0;
-      if (b < XABLK) {
-        const int p = b >= XRT / 16, r = 16 * (b - p * (XRT / 16)) + lr;
+      if (b < C::ABLK) {
+        const int p = b >= C::RT / 16, r = 16 * (b - p * (C::RT / 16)) + lr;
         const int wrr = r / XWT, rr = r - wrr * XWT;  // row wave, row in it
         const int gq = (rr * a.mN) >> 16, ii = rr - gq * N;
-        const int g = (tr * XWR + wrr) * a.ng + gq;
+        const int g = (tr * C::WR + wrr) * a.ng + gq;
         const bool ok = rr < NR && g < a.G;
         off[i] = ok ? 2 * (p * a.aplane + g * a.ka + ii * a.ldq + 8 * lkc) : 0;
-      } else if (b < XNBLK) {
-        const int bb = b - XABLK, p = bb >= XCT / 16, r = 16 * (bb - p * (XCT / 16)) + lr;
+      } else if (b < C::NBLK) {
+        const int bb = b - C::ABLK, p = bb >= C::CT / 16, r = 16 * (bb - p * (C::CT / 16)) + lr;
         const int wcc = r / XWT, rr = r - wcc * XWT;  // column wave, column in it
         const int wq = (rr * a.m2N) >> 16, rem = rr - wq * 2 * N;
         const int ri = rem >= N, j = rem - ri * N;
@@ -1081,24 +1093,24 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
   // chunks at K offset k0 into stage st (rows are ldq long, a multiple of GK,
   // zero past M: chunks never run into the next row)
   auto fetch_slot = [&](int st, int k0, int i) {
-    const unsigned base = (unsigned)(uintptr_t)(lds + st * XSTAGE);
-    const int b = wvu + XNW * i;  // 16-row block, uniform
-    if (b < XABLK) dma16(rsa, base + b * 1024, off[i], 2 * k0);
-    else if (b < XNBLK) dma16(rsb, base + b * 1024, off[i], 2 * k0);
+    const unsigned base = (unsigned)(uintptr_t)(lds + st * C::STAGE);
+    const int b = wvu + C::NW * i;  // 16-row block, uniform
+    if (b < C::ABLK) dma16(rsa, base + b * 1024, off[i], 2 * k0);
+    else if (b < C::NBLK) dma16(rsb, base + b * 1024, off[i], 2 * k0);
   };
   auto fetch = [&](int st, int k0) {
 #pragma unroll
-    for (int i = 0; i < XSLOT; ++i) fetch_slot(st, k0, i);
+    for (int i = 0; i < C::SLOT; ++i) fetch_slot(st, k0, i);
   };
   auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
   const int nk = a.ldq / GK;
 
-  float* Tw = (float*)(lds + XSTAGE) + wv * XEPI;
-  double* wsum = (double*)((float*)(lds + XSTAGE) + XNW * XEPI);  // [XNW][nw][2]
+  float* Tw = (float*)(lds + C::EOFF) + wv * C::EPI;
+  double* wsum = (double*)((float*)(lds + C::EOFF) + C::NW * C::EPI);  // [NW][nw][2]
   // lane offset of (row li, chunk lq) inside a 16-row block, in bf16
   const int frag = li * GK + 8 * xswz(li, lq);
 
-  {
+  if (WIDE) {
     int tr, tc;
     tile_of(v, tr, tc);
     slots(tr, tc);
@@ -1107,8 +1119,13 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
   while (v < v1) {
     int tr, tc;
     tile_of(v, tr, tc);
-    // stage 0 holds this tile's first K step; the barrier also ends the
-    // previous tile's epilogue, whose area the K loop's DMA overwrites
+    if (!WIDE) {
+      slots(tr, tc);
+      fetch(0, 0);
+    }
+    // stage 0 holds this tile's first K step; in the wide shape the barrier
+    // also ends the previous tile's epilogue, whose blocks the K loop's DMA
+    // overwrites
     dma_wait();
     __syncthreads();
 
@@ -1118,36 +1135,33 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
 #pragma unroll
       for (int w = 0; w < 5; ++w) acc[u][w] = f4v{0.f, 0.f, 0.f, 0.f};
     for (int kt = 0; kt < nk; ++kt) {
-      const bf16_t* st = lds + (kt & 1) * XSTAGE + frag;
+      const bf16_t* st = lds + (kt & 1) * C::STAGE + frag;
       uint4 ah[5], al[5];
 #pragma unroll
       for (int u = 0; u < 5; ++u) {
         const int r = XWT * wr + 16 * u;  // block-aligned row
         ah[u] = *(const uint4*)(st + r * GK);
-        al[u] = *(const uint4*)(st + (XRT + r) * GK);
+        al[u] = *(const uint4*)(st + (C::RT + r) * GK);
       }
 #pragma unroll
       for (int w = 0; w < 5; ++w) {
         const int c = XWT * wc + 16 * w;
-        const uint4 bh = *(const uint4*)(st + (2 * XRT + c) * GK);
-        const uint4 bl = *(const uint4*)(st + (2 * XRT + XCT + c) * GK);
+        const uint4 bh = *(const uint4*)(st + (2 * C::RT + c) * GK);
+        const uint4 bl = *(const uint4*)(st + (2 * C::RT + C::CT + c) * GK);
 #pragma unroll
         for (int u = 0; u < 5; ++u) {
           acc[u][w] = mfma3(ah[u], al[u], bh, bl, acc[u][w]);
-          // the next K step's DMA slots, one after each of the first product
-          // groups: issued all at once they back up the vector memory queue
-          // and stall the wave before its products
-          const int q = 5 * w + u;  // static
-          if (q < XSLOT && kt + 1 < nk) fetch_slot((kt + 1) & 1, (kt + 1) * GK, q);
+          const int q = 5 * w + u;  // static: one DMA slot after each of the first groups
+          if (q < C::SLOT && kt + 1 < nk) fetch_slot((kt + 1) & 1, (kt + 1) * GK, q);
         }
       }
       dma_wait();
       __syncthreads();
     }
 
-    // the next tile's first K step goes to stage 0 while the epilogue runs
+    // wide: the next tile's first K step goes to stage 0 while the epilogue runs
     const int vn = next_tile(v + nx);
-    if (vn < v1) {
+    if (WIDE && vn < v1) {
       int trn, tcn;
       tile_of(vn, trn, tcn);
       slots(trn, tcn);
@@ -1157,18 +1171,18 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
     // Epilogue: E(g, w) = sum_ij T_ij T_ji for the g blocks and walkers of
     // the wave tile.  With T = Tr + i Ti,
     //   Re E = sum Tr_ij Tr_ji - sum Ti_ij Ti_ji,
-    //   Im E = sum Tr_ij Ti_ji + sum Ti_ij Tr_ji.
-    // Pass (gi, c) covers plane c of g block gi: the lanes holding values of
-    // that plane write them into the wave's block T_c[wi][i][j] (a lane holds
-    // rows 4 lq .. 4 lq + 3 of one column), and after a wave barrier every
-    // lane reads the transposed partners T_c[wi][j][i .. i + 3] of its rows,
-    // one 16-byte load (one load per element when N % 4 != 0, where a lane's
-    // rows can span two g blocks).  A lane of plane c adds the products to
-    // its own-plane sum A, a lane of the other plane to its cross sum B.
-    // Products and a lane's partial sums over one g block (at most 16 terms
-    // per plane) are FP32, the sums over g blocks, lanes and tiles FP64, all
-    // in a fixed order.
-    int wb[5], rb[5], wiv[5];
+    //   Im E = sum Tr_ij Ti_ji + sum Ti_ij Tr_ji,
+    // so a lane needs the transposed partners of its elements in its own
+    // plane (sum A) and in the other one (sum B).  A lane holds rows
+    // 4 lq .. 4 lq + 3 of one column; it writes them into the wave's block,
+    // and after a wave barrier reads the partners T[wi][j][i .. i + 3], one
+    // 16-byte load (one load per element when N % 4 != 0, where a lane's rows
+    // can span two g blocks).  Wide: pass (gi, c) stores plane c of g block gi
+    // only, a lane of plane c adds its products to A, one of the other plane
+    // to B; narrow: pass gi stores both planes.  Products and a lane's partial
+    // sums over one g block (at most 16 terms per sum) are FP32, the sums over
+    // g blocks, lanes and tiles FP64, all in a fixed order.
+    int wiv[5], jv[5];
     bool cvv[5], cimv[5];
     {
       int lj = li;  // opaque per tile, as in slots()
@@ -1180,39 +1194,163 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
         const int wi = cv ? (col * a.m2N) >> 16 : 0;
         const int rem = col - wi * 2 * N;
         const bool cim = cv && rem >= N;
-        const int j = cv ? rem - (cim ? N : 0) : 0;
         cvv[w] = cv;
         cimv[w] = cim;
         wiv[w] = wi;
-        wb[w] = wi * N * XP + j;    // + i XP: this lane's element (i, j)
-        rb[w] = (wi * N + j) * XP;  // + i: its partner (j, i)
+        jv[w] = cv ? rem - (cim ? N : 0) : 0;
       }
     }
     double cA[5], cB[5];
 #pragma unroll
     for (int w = 0; w < 5; ++w) cA[w] = cB[w] = 0.0;
-    const int gw0 = (tr * XWR + wr) * a.ng;
-    if (VEC) {
-      // a lane's rows 4 lq + 16 u + (0..3) are rows i0 .. i0 + 3 of g block gi
-      // (i0 = 16 u + 4 lq - gi N, whole when N % 4 == 0); its element (i, j)
-      // sits at wpos + (16 u - gi N) XP, the partners (j, i0 .. i0 + 3) at
-      // rpos + 16 u - gi N
-      int wpos[5], rpos[5];
+    const int gw0 = (tr * C::WR + wr) * a.ng;
+    if constexpr (WIDE) {
+      int wb[5], rb[5];
 #pragma unroll
       for (int w = 0; w < 5; ++w) {
-        wpos[w] = wb[w] + 4 * lq * XP;
-        rpos[w] = cvv[w] ? rb[w] + 4 * lq : 0;
+        wb[w] = wiv[w] * N * XP + jv[w];    // + i XP: this lane's element (i, j)
+        rb[w] = (wiv[w] * N + jv[w]) * XP;  // + i: its partner (j, i)
       }
-      for (int gi = 0; gi < a.ng; ++gi) {
-        if (gw0 + gi >= a.G) break;  // wave-uniform
-        const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
-        float S[2][5];  // S[c][w]: the pass-c sums (plane c partners)
+      if (VEC) {
+        // a lane's rows 4 lq + 16 u + (0..3) are rows i0 .. i0 + 3 of g block gi
+        // (i0 = 16 u + 4 lq - gi N, whole when N % 4 == 0); its element (i, j)
+        // sits at wpos + (16 u - gi N) XP, the partners (j, i0 .. i0 + 3) at
+        // rpos + 16 u - gi N
+        int wpos[5], rpos[5];
+  #pragma unroll
+        for (int w = 0; w < 5; ++w) {
+          wpos[w] = wb[w] + 4 * lq * XP;
+          rpos[w] = cvv[w] ? rb[w] + 4 * lq : C::DUM + 64;  // outside the tile: dummy area
+        }
+        for (int gi = 0; gi < a.ng; ++gi) {
+          if (gw0 + gi >= a.G) break;  // wave-uniform
+          const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
+          float S[2][5];  // S[c][w]: the pass-c sums (plane c partners)
+  #pragma unroll
+          for (int c = 0; c < 2; ++c)
+  #pragma unroll
+            for (int w = 0; w < 5; ++w) S[c][w] = 0.f;
+  #pragma unroll
+          for (int c = 0; c < 2; ++c) {
+  #pragma unroll
+            for (int u = 0; u < 5; ++u) {
+              if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+              const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
+              if (r0 >= lo && r0 < hi) {
+  #pragma unroll
+                for (int w = 0; w < 5; ++w) {
+                  if (cvv[w] && (int)cimv[w] == c) {  // only the lanes of plane c write
+                    float* d = Tw + wpos[w] + du * XP;
+  #pragma unroll
+                    for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
+                  }
+                }
+              }
+            }
+            xwave_sync();
+  #pragma unroll
+            for (int u = 0; u < 5; ++u) {
+              if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+              const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
+              if (r0 >= lo && r0 < hi) {
+                float4 ps[5];  // all five partner loads in flight together
+  #pragma unroll
+                for (int w = 0; w < 5; ++w) ps[w] = *(const float4*)(Tw + rpos[w] + du);
+  #pragma unroll
+                for (int w = 0; w < 5; ++w) {
+                  const f4v x = acc[u][w];
+                  S[c][w] = fmaf(x[0], ps[w].x, S[c][w]);
+                  S[c][w] = fmaf(x[1], ps[w].y, S[c][w]);
+                  S[c][w] = fmaf(x[2], ps[w].z, S[c][w]);
+                  S[c][w] = fmaf(x[3], ps[w].w, S[c][w]);
+                }
+              }
+            }
+            xwave_sync();
+          }
+          // own-plane partners feed A, the other plane's feed B
+  #pragma unroll
+          for (int w = 0; w < 5; ++w) {
+            cA[w] += (double)(cimv[w] ? S[1][w] : S[0][w]);
+            cB[w] += (double)(cimv[w] ? S[0][w] : S[1][w]);
+          }
+        }
+      } else {
+        for (int gi = 0; gi < a.ng; ++gi) {
+          if (gw0 + gi >= a.G) break;  // wave-uniform
+          const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
+          float A[5], B[5];
+  #pragma unroll
+          for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
+  #pragma unroll
+          for (int c = 0; c < 2; ++c) {
+  #pragma unroll
+            for (int u = 0; u < 5; ++u) {
+              if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+  #pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int row = 16 * u + 4 * lq + r;
+                const bool inb = row >= lo && row < hi;
+  #pragma unroll
+                for (int w = 0; w < 5; ++w) {
+                  const bool own = inb && cvv[w] && (int)cimv[w] == c;
+                  Tw[own ? wb[w] + (row - lo) * XP : C::DUM + l] = acc[u][w][r];
+                }
+              }
+            }
+            xwave_sync();
+  #pragma unroll
+            for (int u = 0; u < 5; ++u) {
+              if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
+  #pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int row = 16 * u + 4 * lq + r;
+                const bool inb = row >= lo && row < hi;
+  #pragma unroll
+                for (int w = 0; w < 5; ++w) {
+                  const bool ok = inb && cvv[w];
+                  const float x = ok ? acc[u][w][r] : 0.f;
+                  const float pv = Tw[ok ? rb[w] + row - lo : C::DUM + l];
+                  if ((int)cimv[w] == c) A[w] = fmaf(x, pv, A[w]);
+                  else B[w] = fmaf(x, pv, B[w]);
+                }
+              }
+            }
+            xwave_sync();
+          }
+  #pragma unroll
+          for (int w = 0; w < 5; ++w) {
+            cA[w] += (double)A[w];
+            cB[w] += (double)B[w];
+          }
+        }
+      }
+    } else {
+      // this lane's element (i, j) at wb + i XP, its partners (j, i) in its
+      // own plane at ro + i and in the other one at rx + i
+      int wb[5], ro[5], rx[5];
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+      for (int w = 0; w < 5; ++w) {
+        const int c = cimv[w] ? 1 : 0;
+        wb[w] = (2 * wiv[w] + c) * N * XP + jv[w];
+        ro[w] = ((2 * wiv[w] + c) * N + jv[w]) * XP;
+        rx[w] = ((2 * wiv[w] + 1 - c) * N + jv[w]) * XP;
+      }
+      if (VEC) {
+        // partner reads of columns outside the tile go to the dummy area
+        int wpos[5], opos[5], xpos[5];
 #pragma unroll
-          for (int w = 0; w < 5; ++w) S[c][w] = 0.f;
+        for (int w = 0; w < 5; ++w) {
+          wpos[w] = wb[w] + 4 * lq * XP;
+          opos[w] = cvv[w] ? ro[w] + 4 * lq : C::DUM + 64;
+          xpos[w] = cvv[w] ? rx[w] + 4 * lq : C::DUM + 64;
+        }
+        for (int gi = 0; gi < a.ng; ++gi) {
+          if (gw0 + gi >= a.G) break;  // wave-uniform
+          const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
+          float A[5], B[5];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+          for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
 #pragma unroll
           for (int u = 0; u < 5; ++u) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
@@ -1220,7 +1358,7 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
             if (r0 >= lo && r0 < hi) {
 #pragma unroll
               for (int w = 0; w < 5; ++w) {
-                if (cvv[w] && (int)cimv[w] == c) {  // only the lanes of plane c write
+                if (cvv[w]) {
                   float* d = Tw + wpos[w] + du * XP;
 #pragma unroll
                   for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
@@ -1234,37 +1372,41 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
             const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
             if (r0 >= lo && r0 < hi) {
-              float4 ps[5];  // all five partner loads in flight together
+              float4 po[5], px[5];  // all ten partner loads in flight together
 #pragma unroll
-              for (int w = 0; w < 5; ++w) ps[w] = *(const float4*)(Tw + rpos[w] + du);
+              for (int w = 0; w < 5; ++w) {
+                po[w] = *(const float4*)(Tw + opos[w] + du);
+                px[w] = *(const float4*)(Tw + xpos[w] + du);
+              }
 #pragma unroll
               for (int w = 0; w < 5; ++w) {
                 const f4v x = acc[u][w];
-                S[c][w] = fmaf(x[0], ps[w].x, S[c][w]);
-                S[c][w] = fmaf(x[1], ps[w].y, S[c][w]);
-                S[c][w] = fmaf(x[2], ps[w].z, S[c][w]);
-                S[c][w] = fmaf(x[3], ps[w].w, S[c][w]);
+                A[w] = fmaf(x[0], po[w].x, A[w]);
+                A[w] = fmaf(x[1], po[w].y, A[w]);
+                A[w] = fmaf(x[2], po[w].z, A[w]);
+                A[w] = fmaf(x[3], po[w].w, A[w]);
+                B[w] = fmaf(x[0], px[w].x, B[w]);
+                B[w] = fmaf(x[1], px[w].y, B[w]);
+                B[w] = fmaf(x[2], px[w].z, B[w]);
+                B[w] = fmaf(x[3], px[w].w, B[w]);
               }
             }
           }
           xwave_sync();
+#pragma unroll
+          for (int w = 0; w < 5; ++w) {
+            cA[w] += (double)A[w];
+            cB[w] += (double)B[w];
+          }
         }
-        // own-plane partners feed A, the other plane's feed B
+      } else {
 #pragma unroll
         for (int w = 0; w < 5; ++w) {
-          cA[w] += (double)(cimv[w] ? S[1][w] : S[0][w]);
-          cB[w] += (double)(cimv[w] ? S[0][w] : S[1][w]);
+          if (!cvv[w]) ro[w] = rx[w] = C::DUM + 4 * l;
         }
-      }
-    } else {
-      for (int gi = 0; gi < a.ng; ++gi) {
-        if (gw0 + gi >= a.G) break;  // wave-uniform
-        const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
-        float A[5], B[5];
-#pragma unroll
-        for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int gi = 0; gi < a.ng; ++gi) {
+          if (gw0 + gi >= a.G) break;  // wave-uniform
+          const int lo = gi * N, hi = lo + N;  // rows of this g block in the wave tile
 #pragma unroll
           for (int u = 0; u < 5; ++u) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
@@ -1273,13 +1415,14 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
               const int row = 16 * u + 4 * lq + r;
               const bool inb = row >= lo && row < hi;
 #pragma unroll
-              for (int w = 0; w < 5; ++w) {
-                const bool own = inb && cvv[w] && (int)cimv[w] == c;
-                Tw[own ? wb[w] + (row - lo) * XP : XDUM + l] = acc[u][w][r];
-              }
+              for (int w = 0; w < 5; ++w)
+                Tw[inb && cvv[w] ? wb[w] + (row - lo) * XP : C::DUM + l] = acc[u][w][r];
             }
           }
           xwave_sync();
+          float A[5], B[5];
+#pragma unroll
+          for (int w = 0; w < 5; ++w) A[w] = B[w] = 0.f;
 #pragma unroll
           for (int u = 0; u < 5; ++u) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
@@ -1290,19 +1433,19 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
 #pragma unroll
               for (int w = 0; w < 5; ++w) {
                 const bool ok = inb && cvv[w];
+                const int di = ok ? row - lo : 0;
                 const float x = ok ? acc[u][w][r] : 0.f;
-                const float pv = Tw[ok ? rb[w] + row - lo : XDUM + l];
-                if ((int)cimv[w] == c) A[w] = fmaf(x, pv, A[w]);
-                else B[w] = fmaf(x, pv, B[w]);
+                A[w] = fmaf(x, Tw[ro[w] + di], A[w]);
+                B[w] = fmaf(x, Tw[rx[w] + di], B[w]);
               }
             }
           }
-          xwave_sync();
-        }
 #pragma unroll
-        for (int w = 0; w < 5; ++w) {
-          cA[w] += (double)A[w];
-          cB[w] += (double)B[w];
+          for (int w = 0; w < 5; ++w) {
+            cA[w] += (double)A[w];
+            cB[w] += (double)B[w];
+          }
+          xwave_sync();
         }
       }
     }
@@ -1331,7 +1474,7 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
       if (w < a.W) {
         double re = 0.0, im = 0.0;
 #pragma unroll
-        for (int r = 0; r < XWR; ++r) {
+        for (int r = 0; r < C::WR; ++r) {
           re += wsum[((r * XWC + wcc) * a.nw + wi) * 2];
           im += wsum[((r * XWC + wcc) * a.nw + wi) * 2 + 1];
         }
@@ -1339,7 +1482,9 @@ __global__ __launch_bounds__(XNT, XWPC) void k_exch_bf3(Bf3Exch a) {
         a.part[((long)tr * a.W + w) * 2 + 1] = im;
       }
     }
-    v = vn;  // the barrier at the loop top protects wsum and the epilogue blocks
+    // narrow: the epilogue blocks are the staging of the next tile
+    if (!WIDE) __syncthreads();
+    v = vn;  // wide: the barrier at the loop top protects wsum and the epilogue blocks
   }
 }
 
@@ -1396,18 +1541,17 @@ __global__ __launch_bounds__(256) void k_coul_parts(int W, int G, int nsplit,
   }
 }
 
-void exch_geom(int N, int G, int W, int* ng, int* nw, int* ntr, int* ntc) {
+// the wide shape for long K (>= 8 K steps) and at most two g blocks per wave
+// with N % 4 == 0; measured at M=300, N=40 (wide 10.7 ms per spin, narrow
+// 11.7) and M=150, N=20 / 18 (wide 1.04 / 1.87 ms, narrow 0.88 / 1.20)
+bool exch_wide(int N, int ldq) { return N > 0 && N % 4 == 0 && XWT / N <= 2 && ldq / GK >= 8; }
+
+void exch_geom(int N, int G, int W, int ldq, int* ng, int* nw, int* ntr, int* ntc) {
+  const int wr = exch_wide(N, ldq) ? XCfg<true>::WR : XCfg<false>::WR;
   *ng = N > 0 ? XWT / N : 1;
   *nw = N > 0 ? std::max(1, XWT / (2 * N)) : 1;
-  *ntr = cdiv_int(G, XWR * *ng);
+  *ntr = cdiv_int(G, wr * *ng);
   *ntc = cdiv_int(W, XWC * *nw);
-}
-
-size_t exch_lds_bytes() {
-  const size_t stages = (size_t)2 * XSTAGE * sizeof(bf16_t);
-  const size_t epi = (size_t)XSTAGE * sizeof(bf16_t) + (size_t)XNW * XEPI * sizeof(float) +
-                     (size_t)XNW * (XWT / 2) * 2 * sizeof(double);
-  return std::max(stages, epi);
 }
 
 }  // namespace
@@ -1452,8 +1596,8 @@ bool bf3_exchange_supported(const Bf3System& b) { return b.Na <= XMAXN && b.Nb <
 
 size_t bf3_exchange_part_bytes(const Bf3System& b, int W) {
   int ng, nw, ntra = 0, ntrb = 0, ntc;
-  if (b.Na) exch_geom(b.Na, b.G, W, &ng, &nw, &ntra, &ntc);
-  if (b.Nb) exch_geom(b.Nb, b.G, W, &ng, &nw, &ntrb, &ntc);
+  if (b.Na) exch_geom(b.Na, b.G, W, b.ldq, &ng, &nw, &ntra, &ntc);
+  if (b.Nb) exch_geom(b.Nb, b.G, W, b.ldq, &ng, &nw, &ntrb, &ntc);
   return (size_t)(ntra + ntrb) * W * 2 * sizeof(double);
 }
 
@@ -1462,8 +1606,8 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
   if (W <= 0) return AFQ_OK;
   if (!bf3_exchange_supported(b))
     return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: nocc %d/%d > %d", b.Na, b.Nb, XMAXN);
-  const size_t sm = exch_lds_bytes();
-  if (sm > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: LDS %zu", sm);
+  const size_t smax = std::max(XCfg<true>::lds_bytes(), XCfg<false>::lds_bytes());
+  if (smax > max_lds_bytes()) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: LDS %zu", smax);
   const int Ns[2] = {b.Na, b.Nb};
   const long Ks[2] = {b.Ka, b.Kb};
   const bf16_t* gh[2] = {gha, ghb};
@@ -1486,7 +1630,7 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     a.G = b.G;
     a.W = W;
     a.ldq = b.ldq;
-    exch_geom(a.N, b.G, W, &a.ng, &a.nw, &a.ntr, &a.ntc);
+    exch_geom(a.N, b.G, W, b.ldq, &a.ng, &a.nw, &a.ntr, &a.ntc);
     a.part = part + (sp == 1 ? (size_t)ntrs[0] * W * 2 : 0);
     parts[sp] = a.part;
     ntrs[sp] = a.ntr;
@@ -1496,13 +1640,18 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     a.mN = (65536 + a.N - 1) / a.N;
     a.m2N = (65536 + 2 * a.N - 1) / (2 * a.N);
     if (a.ldq % GK) return fail(AFQ_EUNSUPPORTED, "bf16x3 exchange: row stride %d", a.ldq);
-    // persistent: one workgroup per CU, a multiple of 8 (one share per XCD)
-    const long nwg = std::min<long>((long)num_cus() * XWPC, (nb + 7) / 8 * 8);
+    // persistent: WPC workgroups per CU, a multiple of 8 (one share per XCD)
+    const bool wide = exch_wide(a.N, b.ldq);
+    const int wpc = wide ? XCfg<true>::WPC : XCfg<false>::WPC;
+    const long nwg = std::min<long>((long)num_cus() * wpc, (nb + 7) / 8 * 8);
     const dim3 grid((unsigned)std::max<long>(8, nwg / 8 * 8));
-    if (a.N % 4 == 0)
-      hipLaunchKernelGGL(k_exch_bf3<true>, grid, dim3(XNT), sm, s, a);
+    const bool vec = a.N % 4 == 0;
+    if (wide)
+      hipLaunchKernelGGL((k_exch_bf3<true, true>), grid, dim3(XCfg<true>::NT), XCfg<true>::lds_bytes(), s, a);
+    else if (vec)
+      hipLaunchKernelGGL((k_exch_bf3<true, false>), grid, dim3(XCfg<false>::NT), XCfg<false>::lds_bytes(), s, a);
     else
-      hipLaunchKernelGGL(k_exch_bf3<false>, grid, dim3(XNT), sm, s, a);
+      hipLaunchKernelGGL((k_exch_bf3<false, false>), grid, dim3(XCfg<false>::NT), XCfg<false>::lds_bytes(), s, a);
     AFQ_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 64)), dim3(256), 0, s, W, ntrs[0], parts[0],
