@@ -57,13 +57,15 @@ bool bf3_exchange_supported(const Bf3System& b);
 size_t bf3_exchange_part_bytes(const Bf3System& b, int W);
 int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, const bf16_t* ghb,
                  double* part, double* ex);
-// Exchange of one determinant of an occupation-string trial: Lhalf = rows
-// occ_s of every L_g (chol (G, M, M) FP64), Ghalf_s (W, N_s, M) complex FP64;
-// both are split into scratch planes (bf3_md_exchange_bytes) first.
-size_t bf3_md_exchange_bytes(const Bf3System& b, int W);
-int bf3_md_exchange(hipStream_t s, const Bf3System& b, int W, const int* occa, const int* occb,
+// Two-body energy of one determinant of an occupation-string trial: Lhalf =
+// rows occ_s of every L_g (chol (G, M, M) FP64), Ghalf_s (W, N_s, M) complex
+// FP64, both split into scratch planes (bf3_md_two_body_bytes) once; the
+// Coulomb X_g on the split-BF16 force-bias GEMM (ec, as bf3_coulomb), the
+// exchange on k_exch_bf3 (ex).
+size_t bf3_md_two_body_bytes(const Bf3System& b, int W);
+int bf3_md_two_body(hipStream_t s, const Bf3System& b, int W, const int* occa, const int* occb,
                     const double* chol, const double* gha, const double* ghb, void* scratch,
-                    double* ex);
+                    double* ec, double* ex);
 // Coulomb energy 1/2 sum_g X_g^2 from the force-bias partials part[nsplit][W][G].
 int bf3_coulomb(hipStream_t s, int W, int G, int nsplit, const float* part, double* ec);
 size_t bf3_vhs_bytes(const Bf3System& b, int W);

@@ -1660,17 +1660,22 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
   return AFQ_OK;
 }
 
-size_t bf3_md_exchange_bytes(const Bf3System& b, int W) {
+// force-bias partials of one determinant's Coulomb term: one split per spin
+static size_t md_coul_part_bytes(const Bf3System& b, int W) {
+  return (size_t)2 * W * std::max(b.G, 1) * 2 * sizeof(float);
+}
+
+size_t bf3_md_two_body_bytes(const Bf3System& b, int W) {
   return align256((size_t)2 * b.G * b.Ka * sizeof(bf16_t)) +
          align256((size_t)2 * b.G * b.Kb * sizeof(bf16_t)) +
          align256((size_t)2 * 2 * W * b.Ka * sizeof(bf16_t)) +
          align256((size_t)2 * 2 * W * b.Kb * sizeof(bf16_t)) +
-         align256(bf3_exchange_part_bytes(b, W));
+         align256(bf3_exchange_part_bytes(b, W)) + align256(md_coul_part_bytes(b, W));
 }
 
-int bf3_md_exchange(hipStream_t s, const Bf3System& b, int W, const int* occa, const int* occb,
+int bf3_md_two_body(hipStream_t s, const Bf3System& b, int W, const int* occa, const int* occb,
                     const double* chol, const double* gha, const double* ghb, void* scratch,
-                    double* ex) {
+                    double* ec, double* ex) {
   if (W <= 0) return AFQ_OK;
   char* p = (char*)scratch;
   auto take = [&](size_t n) {
@@ -1684,6 +1689,7 @@ int bf3_md_exchange(hipStream_t s, const Bf3System& b, int W, const int* occa, c
   bf16_t* pa = (bf16_t*)take((size_t)2 * 2 * W * b.Ka * sizeof(bf16_t));
   bf16_t* pb = (bf16_t*)take((size_t)2 * 2 * W * b.Kb * sizeof(bf16_t));
   double* part = (double*)take(bf3_exchange_part_bytes(b, W));
+  float* cpart = (float*)take(md_coul_part_bytes(b, W));
   const int Ns[2] = {b.Na, b.Nb};
   const long Ks[2] = {b.Ka, b.Kb};
   const int* occ[2] = {occa, occb};
@@ -1700,6 +1706,17 @@ int bf3_md_exchange(hipStream_t s, const Bf3System& b, int W, const int* occa, c
     hipLaunchKernelGGL(k_prep_gh, dim3(2048), dim3(256), 0, s, ng, Ns[sp], b.M, b.ldq, Ks[sp], ng,
                        gh[sp], gp[sp]);
     AFQ_CHECK_LAUNCH();
+  }
+  // Coulomb: X_g = sum_s sum_(i,q) Lhalf_s[g][i q] Ghalf_s[w][i q], the force
+  // bias contraction of this determinant's planes (no vectors or no
+  // electrons: X = 0)
+  int nsplit = 0;
+  int rc = bf3_force_bias(s, d, W, pa, pb, cpart, &nsplit, 1);
+  if (rc) return rc;
+  if (nsplit > 0) {
+    if ((rc = bf3_coulomb(s, W, b.G, nsplit, cpart, ec))) return rc;
+  } else {
+    if ((rc = zero_async(s, ec, (size_t)2 * W * sizeof(double)))) return rc;
   }
   return bf3_exchange(s, d, W, pa, pb, part, ex);
 }

@@ -520,8 +520,9 @@ int md_g_from_b(hipStream_t s, int W, int M, int Na, int Nb, const double* phia,
 // FP64 multi-det local energy given the buffers of md_greens(MD_GHALF) with G.
 // scratch: E2 (nd W complex) | rchol_a | rchol_b | X (W G complex) | ec | ex
 //          | exchange slab (gc N W N complex)
-// lp: the split-BF16 operands of a BF16x3 system (its exchange then runs on
-// k_exch_bf3 per determinant; NULL: FP64 exchange)
+// lp: the split-BF16 operands of a BF16x3 system (each determinant's Coulomb
+// and exchange then run on the split-BF16 kernels, as in the single-det
+// BF16x3 estimator; NULL: FP64 two-body terms)
 static bool md_bf3_exchange(const Bf3System* lp) { return lp && bf3_exchange_supported(*lp); }
 
 static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int* gc_out,
@@ -529,7 +530,7 @@ static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int
   const long N = std::max(Na, Nb);
   long gc = N > 0 ? std::max(1L, std::min((long)G, (256L << 20) / (16L * N * W * N))) : 1;
   if (gc_out) *gc_out = (int)gc;
-  const size_t slab = md_bf3_exchange(lp) ? a256(bf3_md_exchange_bytes(*lp, W))
+  const size_t slab = md_bf3_exchange(lp) ? a256(bf3_md_two_body_bytes(*lp, W))
                                           : a256(16UL * gc * N * W * N);
   return a256(16UL * nd * W) + a256(8UL * G * Na * M) + a256(8UL * G * Nb * M) +
          a256(16UL * W * G) + 2 * a256(16UL * W) + slab;
@@ -554,11 +555,22 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
   double* X = (double*)take(16UL * W * G);
   double* ec = (double*)take(16UL * W);
   double* ex = (double*)take(16UL * W);
-  double* slab = (double*)take(bx ? bf3_md_exchange_bytes(*lp, W)
+  double* slab = (double*)take(bx ? bf3_md_two_body_bytes(*lp, W)
                                   : 16UL * gc * std::max(Na, Nb) * W * std::max(Na, Nb));
   int rc = e1b_fullg_launch(s, W, M, h1e, b.Ga, b.Gb, enuc, e_1b);
   if (rc) return rc;
   for (int I = 0; I < nd; ++I) {
+    const double* ga = b.gha + 2L * I * W * Na * M;
+    const double* gb = b.ghb + 2L * I * W * Nb * M;
+    if (bx) {
+      if ((rc = bf3_md_two_body(s, *lp, W, occa + (long)I * Na, occb + (long)I * Nb, chol, ga, gb,
+                                slab, ec, ex)))
+        return rc;
+      hipLaunchKernelGGL(k_md_e2b_store, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W,
+                         (const cplx*)ec, (const cplx*)ex, (cplx*)E2 + (long)I * W);
+      AFQ_CHECK_LAUNCH();
+      continue;
+    }
     if (Na && G) {
       const long n = (long)G * Na * M;
       hipLaunchKernelGGL(k_md_rchol, dim3(2048), dim3(256), 0, s, n, M, Na, occa + (long)I * Na,
@@ -571,17 +583,9 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
                          chol, rb);
       AFQ_CHECK_LAUNCH();
     }
-    const double* ga = b.gha + 2L * I * W * Na * M;
-    const double* gb = b.ghb + 2L * I * W * Nb * M;
     if ((rc = coulomb_launch(s, W, M, Na, Nb, G, ra, rb, false, ga, gb, X, ec))) return rc;
-    if (bx) {
-      if ((rc = bf3_md_exchange(s, *lp, W, occa + (long)I * Na, occb + (long)I * Nb, chol, ga, gb,
-                                slab, ex)))
-        return rc;
-    } else {
-      if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
-      if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
-    }
+    if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
+    if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
     hipLaunchKernelGGL(k_md_e2b_store, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W,
                        (const cplx*)ec, (const cplx*)ex, (cplx*)E2 + (long)I * W);
     AFQ_CHECK_LAUNCH();

@@ -176,8 +176,8 @@ def test_multidet_system_energy(afq):
 def _per_det_two_body(s, pa, pb, tu, td, ci, emulate):
     """sum_I w_I (E_coul,I - E_exch,I) / ovlp as local_energy_multidet forms
     it, and the bound scale sum_I |w_I| (|E_coul,I| + |E_exch,I|) / |ovlp|;
-    emulate: each determinant's exchange products as the split BF16 ones
-    (float64; the Coulomb term stays exact, as on the GPU)."""
+    emulate: each determinant's Coulomb and exchange products as the split
+    BF16 ones (float64)."""
     from afq_cases import bf16x3_two_body
     from oracle import afqmc as O
     omax = MD.det_overlap_max(pa, pb, tu, td)
@@ -193,7 +193,7 @@ def _per_det_two_body(s, pa, pb, tu, td, ci, emulate):
         rb = np.einsum('pi,gpq->giq', td[d].conj(), s.chol).real
         ec, ex = O.local_energy_2body(gha[0], ghb[0], ra, rb)
         if emulate:
-            ex = bf16x3_two_body(gha[0], ghb[0], ra, rb)[1]
+            ec, ex = bf16x3_two_body(gha[0], ghb[0], ra, rb)
         e2 += w * (ec - ex)
         scale += np.abs(w) * (np.abs(ec) + np.abs(ex))
         ovlp += w
@@ -201,8 +201,8 @@ def _per_det_two_body(s, pa, pb, tu, td, ci, emulate):
 
 
 def test_multidet_system_energy_bf16x3(afq):
-    """BF16x3 system: every determinant's exchange on the split-BF16 kernel
-    (the determinant algebra, Coulomb and E_1b stay FP64).  Each
+    """BF16x3 system: every determinant's Coulomb and exchange on the
+    split-BF16 kernels (the determinant algebra and E_1b stay FP64).  Each
     determinant's two-body energy carries at most 3e-5 of its component
     scale (the single-determinant bound), so the estimator's error is bounded
     by 3e-5 sum_I |w_I| (|E_coul,I| + |E_exch,I|) / |ovlp| -- the weights
