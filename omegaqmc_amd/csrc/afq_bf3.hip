@@ -1026,7 +1026,7 @@ __device__ __forceinline__ void xwave_sync() {
 // K step; the next K step's chunks are issued between the first product
 // groups of this one (issued all at once they back up the vector memory queue
 // and stall the wave before its products).
-template <bool VEC, bool WIDE>
+template <int VW, bool WIDE>
 __global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf3Exch a) {
   using C = XCfg<WIDE>;
   extern __shared__ __align__(16) bf16_t lds[];
@@ -1211,7 +1211,7 @@ __global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf
         wb[w] = wiv[w] * N * XP + jv[w];    // + i XP: this lane's element (i, j)
         rb[w] = (wiv[w] * N + jv[w]) * XP;  // + i: its partner (j, i)
       }
-      if (VEC) {
+      if (VW == 4) {
         // a lane's rows 4 lq + 16 u + (0..3) are rows i0 .. i0 + 3 of g block gi
         // (i0 = 16 u + 4 lq - gi N, whole when N % 4 == 0); its element (i, j)
         // sits at wpos + (16 u - gi N) XP, the partners (j, i0 .. i0 + 3) at
@@ -1336,8 +1336,12 @@ __global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf
         ro[w] = ((2 * wiv[w] + c) * N + jv[w]) * XP;
         rx[w] = ((2 * wiv[w] + 1 - c) * N + jv[w]) * XP;
       }
-      if (VEC) {
-        // partner reads of columns outside the tile go to the dummy area
+      if (VW > 1) {
+        // a lane's rows 4 lq .. 4 lq + 3 in groups of VW (N % VW == 0, so a
+        // group never spans two g blocks); partner reads of columns outside
+        // the tile go to the dummy area
+        constexpr int NGR = 4 / VW;
+        typedef float fvw __attribute__((ext_vector_type(VW)));
         int wpos[5], opos[5], xpos[5];
 #pragma unroll
         for (int w = 0; w < 5; ++w) {
@@ -1354,14 +1358,17 @@ __global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf
 #pragma unroll
           for (int u = 0; u < 5; ++u) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
-            const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
-            if (r0 >= lo && r0 < hi) {
 #pragma unroll
-              for (int w = 0; w < 5; ++w) {
-                if (cvv[w]) {
-                  float* d = Tw + wpos[w] + du * XP;
+            for (int h = 0; h < NGR; ++h) {
+              const int rg = 16 * u + 4 * lq + VW * h, du = 16 * u - lo + VW * h;
+              if (rg >= lo && rg < hi) {
 #pragma unroll
-                  for (int r = 0; r < 4; ++r) d[r * XP] = acc[u][w][r];
+                for (int w = 0; w < 5; ++w) {
+                  if (cvv[w]) {
+                    float* d = Tw + wpos[w] + du * XP;
+#pragma unroll
+                    for (int r = 0; r < VW; ++r) d[r * XP] = acc[u][w][VW * h + r];
+                  }
                 }
               }
             }
@@ -1370,25 +1377,25 @@ __global__ __launch_bounds__(XCfg<WIDE>::NT, XCfg<WIDE>::WPC) void k_exch_bf3(Bf
 #pragma unroll
           for (int u = 0; u < 5; ++u) {
             if (16 * u + 16 <= lo || 16 * u >= hi) continue;  // uniform
-            const int r0 = 16 * u + 4 * lq, du = 16 * u - lo;
-            if (r0 >= lo && r0 < hi) {
-              float4 po[5], px[5];  // all ten partner loads in flight together
 #pragma unroll
-              for (int w = 0; w < 5; ++w) {
-                po[w] = *(const float4*)(Tw + opos[w] + du);
-                px[w] = *(const float4*)(Tw + xpos[w] + du);
-              }
+            for (int h = 0; h < NGR; ++h) {
+              const int rg = 16 * u + 4 * lq + VW * h, du = 16 * u - lo + VW * h;
+              if (rg >= lo && rg < hi) {
+                fvw po[5], px[5];  // all ten partner loads in flight together
 #pragma unroll
-              for (int w = 0; w < 5; ++w) {
-                const f4v x = acc[u][w];
-                A[w] = fmaf(x[0], po[w].x, A[w]);
-                A[w] = fmaf(x[1], po[w].y, A[w]);
-                A[w] = fmaf(x[2], po[w].z, A[w]);
-                A[w] = fmaf(x[3], po[w].w, A[w]);
-                B[w] = fmaf(x[0], px[w].x, B[w]);
-                B[w] = fmaf(x[1], px[w].y, B[w]);
-                B[w] = fmaf(x[2], px[w].z, B[w]);
-                B[w] = fmaf(x[3], px[w].w, B[w]);
+                for (int w = 0; w < 5; ++w) {
+                  po[w] = *(const fvw*)(Tw + opos[w] + du);
+                  px[w] = *(const fvw*)(Tw + xpos[w] + du);
+                }
+#pragma unroll
+                for (int w = 0; w < 5; ++w) {
+#pragma unroll
+                  for (int r = 0; r < VW; ++r) {
+                    const float x = acc[u][w][VW * h + r];
+                    A[w] = fmaf(x, po[w][r], A[w]);
+                    B[w] = fmaf(x, px[w][r], B[w]);
+                  }
+                }
               }
             }
           }
@@ -1645,13 +1652,17 @@ int bf3_exchange(hipStream_t s, const Bf3System& b, int W, const bf16_t* gha, co
     const int wpc = wide ? XCfg<true>::WPC : XCfg<false>::WPC;
     const long nwg = std::min<long>((long)num_cus() * wpc, (nb + 7) / 8 * 8);
     const dim3 grid((unsigned)std::max<long>(8, nwg / 8 * 8));
-    const bool vec = a.N % 4 == 0;
+    // epilogue row groups: 4 rows when N % 4 == 0, 2 when N is even, else 1
+    const dim3 bn(XCfg<false>::NT);
+    const size_t sn = XCfg<false>::lds_bytes();
     if (wide)
-      hipLaunchKernelGGL((k_exch_bf3<true, true>), grid, dim3(XCfg<true>::NT), XCfg<true>::lds_bytes(), s, a);
-    else if (vec)
-      hipLaunchKernelGGL((k_exch_bf3<true, false>), grid, dim3(XCfg<false>::NT), XCfg<false>::lds_bytes(), s, a);
+      hipLaunchKernelGGL((k_exch_bf3<4, true>), grid, dim3(XCfg<true>::NT), XCfg<true>::lds_bytes(), s, a);
+    else if (a.N % 4 == 0)
+      hipLaunchKernelGGL((k_exch_bf3<4, false>), grid, bn, sn, s, a);
+    else if (a.N % 2 == 0)
+      hipLaunchKernelGGL((k_exch_bf3<2, false>), grid, bn, sn, s, a);
     else
-      hipLaunchKernelGGL((k_exch_bf3<false, false>), grid, dim3(XCfg<false>::NT), XCfg<false>::lds_bytes(), s, a);
+      hipLaunchKernelGGL((k_exch_bf3<1, false>), grid, bn, sn, s, a);
     AFQ_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_exch_reduce, dim3(cdiv_int(W, 64)), dim3(256), 0, s, W, ntrs[0], parts[0],
