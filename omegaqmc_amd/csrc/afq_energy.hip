@@ -13,6 +13,7 @@
 // k_exch_contract in a fixed order.
 #include "afq_gemm.h"
 #include "afq_kernels.h"
+#include "afq_mfma.h"
 
 namespace afq {
 
@@ -185,11 +186,36 @@ int exchange_fp64(hipStream_t s, int W, int M, int N, int G, const double* Ghalf
   return AFQ_OK;
 }
 
-// X (W,G) = sum_s Ghalf_s . rchol_s^T (rchol real or complex)
+// x[0] += x[1] + ... + x[ns - 1], slices of n doubles, in slice order
+__global__ void k_sum_slices(long n, int ns, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double v = x[i];
+    for (int k = 1; k < ns; ++k) v += x[k * n + i];
+    x[i] = v;
+  }
+}
+
+// X (W,G) = sum_s Ghalf_s . rchol_s^T (rchol real or complex); X holds nx
+// slices of (W,G) complex.  Real vectors: the force-bias contraction on FP64
+// MFMA (k_fb, one split per spin into slices 0, 1, summed into slice 0);
+// complex vectors (or fewer slices than spins): the generic FP64 GEMM.
 static int coulomb_x(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                      const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
-                     double* X) {
+                     double* X, int nx) {
   if (G == 0) return AFQ_OK;  // no vectors: X is empty
+  if (!rc_cplx) {
+    const FbSplits sp = fb_splits(M, Na, Nb, Gha, Ghb, ra, rb, 1);
+    if (sp.n == 0) return zero_async(s, X, sizeof(cplx) * (size_t)W * G);
+    if (sp.n <= nx) {
+      int rc = fb_mfma_f64(s, W, G, sp, X);
+      if (rc || sp.n == 1) return rc;
+      const long n = 2L * W * G;
+      hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)std::min<long>(cdiv_int(n, 256), 8192)),
+                         dim3(256), 0, s, n, sp.n, X);
+      AFQ_CHECK_LAUNCH();
+      return AFQ_OK;
+    }
+  }
   const int Ns[2] = {Na, Nb};
   const double* R[2] = {ra, rb};
   const double* Gh[2] = {Gha, Ghb};
@@ -226,7 +252,7 @@ static int e2b_scratch(afq_ctx* ctx, int W, int Na, int Nb, int G, int gchunk_re
   int gc = per_g > 0 ? (int)std::max<long>(1, (256L << 20) / per_g) : std::max(G, 1);
   gc = std::min(gc, std::max(G, 1));
   if (gchunk_req > 0) gc = std::min(gc, gchunk_req);
-  size_t sx = align256(16L * W * G), sw = align256(16L * W);
+  size_t sx = align256(2 * 16L * W * G), sw = align256(16L * W);  // X: two (W,G) slices
   size_t need = sx + 3 * sw + align256((long)gc * per_g);
   char* sc = (char*)scratch(ctx, need);
   if (!sc) return AFQ_ENOMEM;
@@ -242,7 +268,7 @@ static int e2b_scratch(afq_ctx* ctx, int W, int Na, int Nb, int G, int gchunk_re
 static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, int Nb, int G,
                     const double* Gha, const double* Ghb, const double* ra, const double* rb,
                     bool rc_cplx, double* e_coul, double* e_exch, const double* sign = nullptr) {
-  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, sc.X);
+  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, sc.X, 2);
   if (rc) return rc;
   hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)sc.X, (cplx*)e_coul,
                      sign);
@@ -254,8 +280,8 @@ static int two_body(hipStream_t s, const E2bScratch& sc, int W, int M, int Na, i
 
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                    const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
-                   double* X, double* e_coul, const double* sign) {
-  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, X);
+                   double* X, int nx, double* e_coul, const double* sign) {
+  int rc = coulomb_x(s, W, M, Na, Nb, G, ra, rb, rc_cplx, Gha, Ghb, X, nx);
   if (rc) return rc;
   hipLaunchKernelGGL(k_coulomb, dim3(W), dim3(NT), 0, s, G, (const cplx*)X, (cplx*)e_coul, sign);
   AFQ_CHECK_LAUNCH();

@@ -89,6 +89,6 @@ int e1b_fullg_launch(hipStream_t s, int W, int M, const double* h1e, const doubl
                      const double* Gb, double enuc, double* e1b);
 int coulomb_launch(hipStream_t s, int W, int M, int Na, int Nb, int G, const double* ra,
                    const double* rb, bool rc_cplx, const double* Gha, const double* Ghb,
-                   double* X, double* e_coul, const double* sign = nullptr);
+                   double* X, int nx, double* e_coul, const double* sign = nullptr);
 
 }  // namespace afq

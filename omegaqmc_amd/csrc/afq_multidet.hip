@@ -533,7 +533,7 @@ static size_t md_energy_scratch(int W, int M, int Na, int Nb, int G, int nd, int
   const size_t slab = md_bf3_exchange(lp) ? a256(bf3_md_two_body_bytes(*lp, W))
                                           : a256(16UL * gc * N * W * N);
   return a256(16UL * nd * W) + a256(8UL * G * Na * M) + a256(8UL * G * Nb * M) +
-         a256(16UL * W * G) + 2 * a256(16UL * W) + slab;
+         a256(2 * 16UL * W * G) + 2 * a256(16UL * W) + slab;
 }
 
 static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd, int chunk,
@@ -552,7 +552,7 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
   double* E2 = (double*)take(16UL * nd * W);
   double* ra = (double*)take(8UL * G * Na * M);
   double* rb = (double*)take(8UL * G * Nb * M);
-  double* X = (double*)take(16UL * W * G);
+  double* X = (double*)take(2 * 16UL * W * G);  // two (W,G) slices
   double* ec = (double*)take(16UL * W);
   double* ex = (double*)take(16UL * W);
   double* slab = (double*)take(bx ? bf3_md_two_body_bytes(*lp, W)
@@ -583,7 +583,7 @@ static int md_energy(hipStream_t s, int W, int M, int Na, int Nb, int G, int nd,
                          chol, rb);
       AFQ_CHECK_LAUNCH();
     }
-    if ((rc = coulomb_launch(s, W, M, Na, Nb, G, ra, rb, false, ga, gb, X, ec))) return rc;
+    if ((rc = coulomb_launch(s, W, M, Na, Nb, G, ra, rb, false, ga, gb, X, 2, ec))) return rc;
     if ((rc = exchange_fp64(s, W, M, Na, G, ga, ra, false, gc, slab, ex, false))) return rc;
     if ((rc = exchange_fp64(s, W, M, Nb, G, gb, rb, false, gc, slab, ex, true))) return rc;
     hipLaunchKernelGGL(k_md_e2b_store, dim3(cdiv_int(W, 256)), dim3(256), 0, s, W,

@@ -357,8 +357,9 @@ int system_energy_fp64(afq_system* sys, int W, const double* phia, const double*
   EnergySlots es;
   int rc = energy_prologue(sys, W, phia, phib, h1e, enuc, nullptr, e_1b, &es);
   if (rc) return rc;
+  // es.X: the force-bias partial slot, max(1, fb_nsplit) slices of (W,G)
   if ((rc = coulomb_launch(s, W, M, Na, Nb, G, sys->rchol_a, sys->rchol_b, false, es.gha, es.ghb,
-                           es.X, es.ec, sys->chol_sign)))
+                           es.X, std::max(1, sys->fb_nsplit), es.ec, sys->chol_sign)))
     return rc;
   // T slab in the VHS region (W*M*M complex): gchunk with gchunk*N*W*N <= W*M*M
   const int N = std::max(Na, Nb);
