@@ -85,17 +85,20 @@ def test_trajectory_energy_within_budget(afq):
 
 
 @pytest.mark.parametrize("M,Na,Nb,G,W", [(40, 5, 4, 120, 64), (100, 40, 38, 130, 37),
-                                         (60, 16, 1, 90, 21), (64, 40, 40, 200, 130)])
+                                         (60, 16, 1, 90, 21), (64, 40, 40, 200, 130),
+                                         (256, 36, 32, 97, 23)])
 def test_block_energy_close_to_oracle(afq, M, Na, Nb, G, W):
     """Mixed-estimator local energy with the split-BF16 Coulomb and exchange
-    contractions (ragged row/walker tiles, N = 1 .. 40) against the FP64
-    oracle.  E_2b = E_coul - E_exch cancels two large terms, so the bound is
+    contractions (ragged row/walker tiles, N = 1 .. 40; M = 256 with N = 36, 32
+    runs the exchange's wide shape, the others its narrow one) against the
+    FP64 oracle.  E_2b = E_coul - E_exch cancels two large terms, so the bound is
     per walker relative to |E_coul| + |E_exch|: 3e-5 (BF16x3 products carry
     ~2^-17 relative error, FP32 partial sums over K = N M add ~1e-6; these
     walkers are far from the trial, |Ghalf| up to 35).  The
     error itself must be the split's own: a float64 emulation of the same
-    hi/lo products lands within a third of it (plus 2e-7 of the scale for
-    the FP32 accumulation)."""
+    hi/lo products lands within a third of the emulated Coulomb and exchange
+    errors (summed: they can cancel in E_2b), plus 2e-7 of the scale for the
+    FP32 accumulation."""
     s, d64, dbf = _drivers(M, Na, Nb, G)
     od = O.Driver(s.h1e, s.h1e_mod, s.chol, s.enuc, Na, Nb)
     rng = np.random.default_rng(M + W)
@@ -112,7 +115,8 @@ def test_block_energy_close_to_oracle(afq, M, Na, Nb, G, W):
     assert np.all(np.abs(_n(et) - rt) < 3e-5 * scale + 1e-12 * np.abs(r1))
     mc, mx = bf16x3_two_body(Gha, Ghb, od.rchol_a, od.rchol_b)
     emu = mc - mx
-    assert np.all(np.abs(_n(e2) - emu) < 0.34 * np.abs(emu - r2) + 2e-7 * scale)
+    split_err = np.abs(mc - ec) + np.abs(mx - ex)  # the components' errors; they can cancel in E_2b
+    assert np.all(np.abs(_n(e2) - emu) < 0.34 * split_err + 2e-7 * scale)
     ft, _, f2 = d64.system.local_energy(_t(pa), _t(pb), _t(s.h1e), s.enuc)
     assert rel_err(_n(f2), r2) < 1e-11
 
