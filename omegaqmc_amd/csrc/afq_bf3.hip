@@ -164,6 +164,13 @@ __device__ __forceinline__ i32x4 buf_rsrc(const void* p, unsigned bytes) {
   return r;
 }
 
+// LDS writes of this wavefront visible to its own later reads (no other wave)
+__device__ __forceinline__ void xwave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ----------------------------------------------------------------------------
 // the GEMM core: 256 x 256 tile, K step 32, 8 waves of 64 x 128
 // ----------------------------------------------------------------------------
@@ -283,43 +290,61 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
 
   if (EPI == 0) {
     // one 16x16 block (P,Q) of VHS per column tile; this wave holds block
-    // rows i = 8 wc + w (w < 8), column j = li
+    // rows i = 8 wc + x (x < 8), column j = li, for the walkers of its rows.
+    // The transposed block (Q,P) takes a lane's 8 rows as one 16-byte store;
+    // the block (P,Q) itself goes through a wave-private LDS area
+    // [lq][plane][x][j] (the stages are free after the K loop), read back as
+    // 16-byte row chunks: 4 stores of 16 bytes per lane instead of 32 of 2.
     const int2 PQ = a.blk_pq[tn];
     const long offd = ((long)PQ.x * a.RT + PQ.y) * 1024;
     const long offt = ((long)PQ.y * a.RT + PQ.x) * 1024;
+    bf16_t* stg = lds + wv * 2048;  // 4 walkers x 4 planes x 8 rows x 16 columns
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int w = (row0 + 64 * wr + 16 * u + 4 * lq) / 2 + h;
-        if (w >= a.W) continue;
-        bf16_t* base = a.Vb + (long)w * a.vstride;
-        bf16_t pl[4][8];
+        if (w < a.W) {
+          bf16_t* base = a.Vb + (long)w * a.vstride;
+          bf16_t pl[4][8];
 #pragma unroll
-        for (int x = 0; x < 8; ++x) {
-          // FP32 product (the nearest float to the exact one) and its exact
-          // FP32 split
-          const float re = -a.sqdt * acc[u][x][2 * h + 1];
-          const float im = a.sqdt * acc[u][x][2 * h];
-          pl[0][x] = bf_bits(re);
-          pl[1][x] = bf_bits(re - bf_val(pl[0][x]));
-          pl[2][x] = bf_bits(im);
-          pl[3][x] = bf_bits(im - bf_val(pl[2][x]));
-          const int i = 8 * wc + x;
+          for (int x = 0; x < 8; ++x) {
+            // FP32 product (the nearest float to the exact one) and its exact
+            // FP32 split
+            const float re = -a.sqdt * acc[u][x][2 * h + 1];
+            const float im = a.sqdt * acc[u][x][2 * h];
+            pl[0][x] = bf_bits(re);
+            pl[1][x] = bf_bits(re - bf_val(pl[0][x]));
+            pl[2][x] = bf_bits(im);
+            pl[3][x] = bf_bits(im - bf_val(pl[2][x]));
 #pragma unroll
-          for (int p = 0; p < 4; ++p) base[offd + p * 256 + i * 16 + li] = pl[p][x];
-        }
-        if (PQ.x != PQ.y) {
+            for (int p = 0; p < 4; ++p) stg[((lq * 4 + p) * 8 + x) * 16 + li] = pl[p][x];
+          }
+          if (PQ.x != PQ.y) {
 #pragma unroll
-          for (int p = 0; p < 4; ++p) {
-            uint4 pk;
-            pk.x = pl[p][0] | ((unsigned)pl[p][1] << 16);
-            pk.y = pl[p][2] | ((unsigned)pl[p][3] << 16);
-            pk.z = pl[p][4] | ((unsigned)pl[p][5] << 16);
-            pk.w = pl[p][6] | ((unsigned)pl[p][7] << 16);
-            *(uint4*)(base + offt + p * 256 + li * 16 + 8 * wc) = pk;
+            for (int p = 0; p < 4; ++p) {
+              uint4 pk;
+              pk.x = pl[p][0] | ((unsigned)pl[p][1] << 16);
+              pk.y = pl[p][2] | ((unsigned)pl[p][3] << 16);
+              pk.z = pl[p][4] | ((unsigned)pl[p][5] << 16);
+              pk.w = pl[p][6] | ((unsigned)pl[p][7] << 16);
+              *(uint4*)(base + offt + p * 256 + li * 16 + 8 * wc) = pk;
+            }
           }
         }
+        xwave_sync();
+        // chunk c = ((q * 4 + p) * 8 + x) * 2 + half of the area: walker
+        // group q, plane p, block row 8 wc + x, columns 8 half .. 8 half + 7
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int c = l + 64 * k;
+          const int q = c >> 6, p = (c >> 4) & 3, x = (c >> 1) & 7, half = c & 1;
+          const int wq = (row0 + 64 * wr + 16 * u + 4 * q) / 2 + h;
+          const uint4 v = *(const uint4*)(stg + c * 8);
+          if (wq < a.W)
+            *(uint4*)(a.Vb + (long)wq * a.vstride + offd + p * 256 + (8 * wc + x) * 16 + 8 * half) = v;
+        }
+        xwave_sync();
       }
     }
   } else {
@@ -1011,12 +1036,6 @@ struct Bf3Exch {
   int mN, m2N;   // ceil(2^16 / N), ceil(2^16 / 2N): x / N = (x mN) >> 16 for x < 512
   double* part;  // [ntr][W] complex
 };
-
-__device__ __forceinline__ void xwave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Persistent grid: XCD x owns a contiguous range of logical tiles and its
 // workgroups take them in turn, so at any time an XCD works on neighbouring
