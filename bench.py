@@ -217,6 +217,10 @@ def lockstep_energy_error(sys_np, W, nsteps, precision, seed=42, every=10, trial
         "block_end_err_mHa": {str(k): mha(d) for k, d, _, _, _ in ends},
         "block_end_fp64_stat_err_mHa": {str(k): mha(sg) for k, _, _, _, sg in ends},
         "block_end_capped_err_mHa": {str(k): mha(c) for k, _, c, _, _ in ends},
+        "block_end_signed_err_mHa": {str(k): mha(d) for k, d, _ in r['block_ends_signed']},
+        # the error of the run's mean block energy (the reported estimate)
+        "mean_block_energy_err_mHa": mha(r['mean_block_raw']),
+        "mean_block_energy_capped_err_mHa": mha(r['mean_block_capped']),
         "trace_err_mHa": {str(k): mha(d) for k, d, _, _, _ in r['trace']},
         "fp64_walkers_beyond_cap": {str(k): n for k, _, _, n, _ in r['trace']},
         # the first two blocks; later the FP64 local energies spread over

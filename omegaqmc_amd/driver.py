@@ -286,12 +286,15 @@ def lockstep_energy_error(ref, low, num_walkers, num_steps, seed=42, every=10,
     deviation of the local energies over the square root of the Kish size
     (sum |w|)^2 / sum w^2); 'block_ends' the same at block ends; 'max_raw',
     'max_capped' over all checkpoints; 'max_block_raw', 'max_block_capped'
-    over block ends."""
+    over block ends; 'block_ends_signed' [(step, dE raw, dE capped)] with
+    sign (low minus ref) and 'mean_block_raw', 'mean_block_capped' the error
+    of the run's mean block energy, the estimate a driver run reports
+    (afqmc_gto.py:1517-1525 averages the production block energies)."""
     W = int(num_walkers)
     sr, sl = ref.init_walkers(W), low.init_walkers(W)
     eshift = ref.ehf - ref.e_const
     ebound = float(np.sqrt(2.0 / ref.dt))
-    trace, ends = [], []
+    trace, ends, signed = [], [], []
     iblock = 0
     for step in range(1, int(num_steps) + 1):
         if (step - 1) % num_steps_per_block == 0:
@@ -321,6 +324,7 @@ def lockstep_energy_error(ref, low, num_walkers, num_steps, seed=42, every=10,
             trace.append(rec)
             if end:
                 ends.append(rec)
+                signed.append((step, vals[1][0] - vals[0][0], vals[1][1] - vals[0][1]))
         if end:
             acc = sr['acc'].cpu().numpy()
             if acc[0] > 1e-10:
@@ -330,7 +334,10 @@ def lockstep_energy_error(ref, low, num_walkers, num_steps, seed=42, every=10,
            'max_raw': max((r[1] for r in trace), default=0.0),
            'max_capped': max((r[2] for r in trace), default=0.0),
            'max_block_raw': max((r[1] for r in ends), default=0.0),
-           'max_block_capped': max((r[2] for r in ends), default=0.0)}
+           'max_block_capped': max((r[2] for r in ends), default=0.0),
+           'block_ends_signed': signed,
+           'mean_block_raw': abs(float(np.mean([r[1] for r in signed]))) if signed else 0.0,
+           'mean_block_capped': abs(float(np.mean([r[2] for r in signed]))) if signed else 0.0}
     return out
 
 
