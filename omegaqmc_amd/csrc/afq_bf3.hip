@@ -102,8 +102,9 @@ __global__ void k_prep_L(long n, int M, int G, int Gp, long NC, const int2* __re
     if (g < G && p < M && q < M) v = L[(long)g * MM + (long)p * M + q];
     bf16_t h, l;
     split2(v, h, l);
-    Lb[e] = h;
-    Lb[NC * Gp + e] = l;
+    const long o = ((c >> 4) * (Gp >> 5) + (g >> 5)) * 512 + (c & 15) * 32 + (g & 31);
+    Lb[o] = h;
+    Lb[NC * Gp + o] = l;
   }
 }
 
@@ -176,12 +177,19 @@ __device__ __forceinline__ void xwave_sync() {
 // ----------------------------------------------------------------------------
 constexpr int GB = 256, GK = 32, GT = 512;
 
+// Operand planes are row-major (element (r, k) at r ld + k) or K-tiled: rows
+// in blocks of 16, each block's K in steps of 32, every 16 x 32 piece 1 KB of
+// contiguous memory, (r, k) at ((r / 16) (ld / 32) + k / 32) 512 + (r % 16) 32
+// + k % 32 (ld a multiple of 32, rows padded to 16).  A K-tiled K step of 16
+// rows is one contiguous kilobyte for the DMA: 42.6 against 23.8 B per clock
+// per CU for 16 separate 64-byte rows, measured on gfx950.
 struct Bf3Split {
   const bf16_t* a[2];  // A planes (rows x lda)
   const bf16_t* b[2];  // B planes (cols x ldb)
   long lda, ldb;
   int kbeg, kend;
   unsigned abytes, bbytes;  // bytes of one A / B plane (< 2^32)
+  int tiled;                // both operands K-tiled
 };
 
 struct Bf3Gemm {
@@ -235,13 +243,18 @@ __global__ __launch_bounds__(GT) void k_bf3_gemm(Bf3Gemm a) {
     const bool isa = q < 2;
     const int gr = (isa ? row0 : col0) + r;
     const bool ok = gr < (isa ? a.rows : a.cols);
-    off[i] = (unsigned)((ok ? (long)gr : 0L) * (isa ? S.lda : S.ldb) * 2 + 16 * lkc);
+    const long ld = isa ? S.lda : S.ldb;
+    // K-tiled: row block gr / 16 is 32 ld bytes, the lane's row (gr % 16 = lrow)
+    // 64 bytes into each kilobyte
+    off[i] = S.tiled ? (unsigned)((ok ? (long)(gr >> 4) : 0L) * ld * 32 + lrow * 64 + 16 * lkc)
+                     : (unsigned)((ok ? (long)gr : 0L) * ld * 2 + 16 * lkc);
   }
+  const int kbytes = S.tiled ? 32 : 2;  // byte offset of K step k0 per unit of k0
   auto fetch1 = [&](int buf, int k0, int i) {
     const int q = i >> 1, b = wv + 8 * (i & 1);
     const unsigned dst =
         (unsigned)(uintptr_t)(lds + buf * GSTAGE + q * GB * GK + b * 16 * GK);
-    dma16(rs[q], dst, (int)off[i], 2 * k0);
+    dma16(rs[q], dst, (int)off[i], kbytes * k0);
   };
   auto dma_wait = []() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); };
   const int frag = li * GK + 8 * xswz(li, lq);  // (row li, chunk lq) of a 16-row block
@@ -737,7 +750,8 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
 // host side
 // ----------------------------------------------------------------------------
 size_t bf3_vhs_bytes(const Bf3System& b, int W) { return (size_t)W * b.RT * b.RT * 1024 * 2; }
-size_t bf3_x_bytes(const Bf3System& b, int W) { return (size_t)2 * 2 * W * b.Gp * 2; }
+static size_t x_rows(int W) { return (size_t)(2 * W + 15) / 16 * 16; }
+size_t bf3_x_bytes(const Bf3System& b, int W) { return (size_t)2 * x_rows(W) * b.Gp * 2; }
 size_t bf3_gh_bytes(const Bf3System& b, int W) {
   return (size_t)2 * 2 * W * ((size_t)b.Ka + b.Kb) * 2;
 }
@@ -818,14 +832,15 @@ int bf3_vhs(hipStream_t s, const Bf3System& b, int W, const bf16_t* X, double dt
   Bf3Gemm a{};
   a.nsplit = 1;
   a.sp[0].a[0] = X;
-  a.sp[0].a[1] = X + (size_t)2 * W * b.Gp;
+  a.sp[0].a[1] = X + x_rows(W) * b.Gp;
   a.sp[0].lda = b.Gp;
+  a.sp[0].tiled = 1;
   a.sp[0].b[0] = b.Lb;
   a.sp[0].b[1] = b.Lb + (size_t)b.nblk * 256 * b.Gp;
   a.sp[0].ldb = b.Gp;
   a.sp[0].kbeg = 0;
   a.sp[0].kend = b.Gp;
-  const size_t ab = (size_t)2 * W * b.Gp * 2, bb = (size_t)b.nblk * 256 * b.Gp * 2;
+  const size_t ab = x_rows(W) * b.Gp * 2, bb = (size_t)b.nblk * 256 * b.Gp * 2;
   if (ab >= (1ull << 32) || bb >= (1ull << 32))
     return fail(AFQ_EUNSUPPORTED, "bf16x3 VHS: operand plane of %zu / %zu bytes (limit 4 GiB)", ab, bb);
   a.sp[0].abytes = (unsigned)ab;

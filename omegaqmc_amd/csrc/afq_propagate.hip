@@ -16,7 +16,9 @@ constexpr int NT = 256;
 // afqmc_gto.py:687-706 per walker (one workgroup): cap, shift, cfb, cmf.
 // vbias comes as nsplit FP64 partials (vbias) or FP32 partials (vpart), summed
 // in FP64 in split order.  Optional outputs: xbar, xshifted (FP64) and the
-// split-bf16 planes of xshifted, Xp[2][2W][Gp] with rows (w, re/im).
+// split-bf16 planes of xshifted, Xp[2][2W][Gp] with rows (w, re/im), K-tiled:
+// element (r, g) at ((r / 16) (Gp / 32) + g / 32) 512 + (r % 16) 32 + g % 32, each
+// plane 2W rounded up to 16 rows.
 // sign (optional, (G,)): the electron-gas form (afqmc_pw_heg.py:895-922), HS
 // phase i for sign +1 and 1 for sign -1: xbar = -sqrt(dt) (phase vbias - mf)
 // and VHS = sqrt(dt) sum_g xs_g phase_g L_g, written here as the coefficient
@@ -60,12 +62,14 @@ __global__ __launch_bounds__(NT) void k_shift(int G, int nsplit, long sstride,
     if (xs_out) xs_out[w * G + g] = xv;
     if (Xp) {
       bf16_t h, l;
+      const long r = 2 * w;  // even: rows r, r + 1 share a 16-row block
+      const long o = ((r >> 4) * (Gp >> 5) + (g >> 5)) * 512 + (r & 15) * 32 + (g & 31);
       split2(xv.x, h, l);
-      Xp[2 * w * Gp + g] = h;
-      Xp[Xplane + 2 * w * Gp + g] = l;
+      Xp[o] = h;
+      Xp[Xplane + o] = l;
       split2(xv.y, h, l);
-      Xp[(2 * w + 1) * Gp + g] = h;
-      Xp[Xplane + (2 * w + 1) * Gp + g] = l;
+      Xp[o + 32] = h;
+      Xp[Xplane + o + 32] = l;
     }
     s_mf = cadd(s_mf, cmul(xs, m));
     s_xixb = cadd(s_xixb, cscale(xb, x));
@@ -169,7 +173,8 @@ int shift_launch(hipStream_t s, int W, int G, const double* vbiasT, const double
   if (W <= 0) return AFQ_OK;
   hipLaunchKernelGGL(k_shift, dim3(W), dim3(NT), 0, s, G, nsplit, (long)W * G,
                      (const cplx*)vbiasT, (const float2*)vpart, (const cplx*)mf, xi, sqrt(dt), fb,
-                     (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf, Xp, Gp, 2L * W * Gp, sign);
+                     (cplx*)xbar, (cplx*)xs, (cplx*)cfb, (cplx*)cmf, Xp, Gp, (2L * W + 15) / 16 * 16 * Gp,
+                     sign);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
