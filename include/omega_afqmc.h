@@ -420,8 +420,10 @@ int afq_system_local_energy(afq_system* sys, int nwalkers, const double* phia,
  * stream (no counterpart in the reference, which times whole blocks with
  * perf_counter at afqmc_gto.py:1380-1382).  Phases, in order: greens,
  * one-body, force-bias GEMM, shift, VHS build, Taylor, overlap, weights,
- * aux-field RNG.  read() synchronises the stream and returns the summed
- * milliseconds and the number of timed intervals per phase. */
+ * aux-field RNG.  on: 0 stops timing, 1 times every phase, a negative value
+ * -m times only the phases p with bit p of m set (two event records per timed
+ * phase and step on the stream).  read() synchronises the stream and returns
+ * the summed milliseconds and the number of timed intervals per phase. */
 int afq_system_timing_enable(afq_system* sys, int on);
 int afq_system_timing_read(afq_system* sys, double* total_ms, int64_t* launches, int nphase,
                            int reset);

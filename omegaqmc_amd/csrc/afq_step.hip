@@ -22,6 +22,7 @@
 
 struct Timing {
   bool on = false;
+  unsigned mask = ~0u;  // phases timed while on
   std::vector<hipEvent_t> ev[PH_NPHASE];  // pairs (start, stop)
   int used[PH_NPHASE] = {};
   double total_ms[PH_NPHASE] = {};
@@ -32,7 +33,7 @@ namespace afq {
 
 void phase_begin(afq_system* s, int ph) {
   Timing* t = s->timing;
-  if (!t || !t->on) return;
+  if (!t || !t->on || !((t->mask >> ph) & 1u)) return;
   if ((int)t->ev[ph].size() < 2 * (t->used[ph] + 1)) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;
@@ -45,7 +46,8 @@ void phase_begin(afq_system* s, int ph) {
 
 void phase_end(afq_system* s, int ph) {
   Timing* t = s->timing;
-  if (!t || !t->on || (int)t->ev[ph].size() < 2 * (t->used[ph] + 1)) return;
+  if (!t || !t->on || !((t->mask >> ph) & 1u) || (int)t->ev[ph].size() < 2 * (t->used[ph] + 1))
+    return;
   (void)hipEventRecord(t->ev[ph][2 * t->used[ph] + 1], s->ctx->stream);
   t->used[ph] += 1;
 }
@@ -456,6 +458,7 @@ int afq_system_timing_enable(afq_system* s, int on) {
     if (rc) return rc;
   }
   s->timing->on = (on != 0);
+  s->timing->mask = on < 0 ? (unsigned)(-(long)on) : ~0u;
   return AFQ_OK;
 }
 

@@ -151,3 +151,33 @@ def test_full_size_step_properties(afq):
     q = _n(qa)
     eye = np.einsum('wpi,wpj->wij', q.conj(), q)
     assert np.max(np.abs(eye - np.eye(40)[None])) < 1e-10
+
+
+def test_phase_timing_mask(afq):
+    """afq_system_timing_enable: 1 times every phase of each step, -(1 << p)
+    only phase p (the other phases record no events), 0 stops; the timed
+    intervals count one per phase and step."""
+    import torch
+    from omegaqmc_amd import _lib
+    s, od, gd = _driver_pair(30, 4, 3, 90)
+    st = gd.init_walkers(8)
+    L = _lib.load()
+    h = gd.system.handle
+    tot = (_lib.ctypes.c_double * 16)()
+    cnt = (_lib.ctypes.c_int64 * 16)()
+    eshift = gd.ehf - gd.e_const
+    tay = _lib.PHASES.index("taylor")
+    counts = {}
+    for mode, on in (("taylor", -(1 << tay)), ("all", 1)):
+        _lib.check(L.afq_system_timing_enable(h, on), "timing")
+        L.afq_system_timing_read(h, tot, cnt, 16, 1)
+        for k in range(1, 4):
+            gd.step(st, k, 42, eshift, 5, 10)
+        torch.cuda.synchronize()
+        _lib.check(L.afq_system_timing_read(h, tot, cnt, 16, 1), "timing")
+        counts[mode] = {n: int(cnt[i]) for i, n in enumerate(_lib.PHASES)}
+        assert tot[tay] > 0.0
+    _lib.check(L.afq_system_timing_enable(h, 0), "timing")
+    assert counts["taylor"] == {n: (3 if n == "taylor" else 0) for n in _lib.PHASES}
+    for n in ("greens", "force_bias", "vhs", "taylor", "overlap", "weights"):
+        assert counts["all"][n] == 3, counts["all"]
