@@ -109,18 +109,21 @@ __global__ void k_prep_L(long n, int M, int G, int Gp, long NC, const int2* __re
 }
 
 // rows x cols -> [plane][rows_p][cols_p] with zero padding; diag_sub is
-// taken off the diagonal first (A - diag_sub I)
+// taken off the diagonal first (A - diag_sub I).  tiled: K-tiled planes,
+// (r, c) at ((r / 16) (cols_p / 32) + c / 32) 512 + (r % 16) 32 + c % 32
+// (rows_p a multiple of 16, cols_p of 32)
 __global__ void k_prep_mat(long n, int rows, int cols, int cols_p, long plane,
                            const double* __restrict__ A, double diag_sub,
-                           bf16_t* __restrict__ out) {
+                           bf16_t* __restrict__ out, int tiled) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const long r = e / cols_p;
     const int c = (int)(e % cols_p);
     const double v = (r < rows && c < cols) ? A[r * cols + c] - (r == c ? diag_sub : 0.0) : 0.0;
     bf16_t h, l;
     split2(v, h, l);
-    out[e] = h;
-    out[plane + e] = l;
+    const long o = tiled ? ((r >> 4) * (cols_p >> 5) + (c >> 5)) * 512 + (r & 15) * 32 + (c & 31) : e;
+    out[o] = h;
+    out[plane + o] = l;
   }
 }
 
@@ -500,14 +503,15 @@ __global__ __launch_bounds__(PT) void k_prop_bf3(int W, int M, int RT, int Kp, i
     const v4u x = *(const gv4u*)p;
     return make_uint4(x.x, x.y, x.z, x.w);
   };
-  // acc = E . T (E real, planes [2][16 RT][Kp]); row tiles this wave does not
-  // own read a valid row and are not used
+  // acc = E . T (E real, K-tiled planes [2][RT][Kp / 32][16][32]: a wave's
+  // fragment of one row tile and k-step is one contiguous kilobyte); row
+  // tiles this wave does not own read a valid row and are not used
   auto ldE = [&](uint4* eh, uint4* el, int ks) {
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
-      const long r = 16 * min(wv + PW * u, RT - 1) + li;
-      eh[u] = gld(E + r * Kp + 32 * ks + 8 * lq);
-      el[u] = gld(E + Eplane + r * Kp + 32 * ks + 8 * lq);
+      const long o = ((long)min(wv + PW * u, RT - 1) * (Kp >> 5) + ks) * 512 + li * 32 + 8 * lq;
+      eh[u] = gld(E + o);
+      el[u] = gld(E + Eplane + o);
     }
   };
   auto estep = [&](const uint4* eh, const uint4* el, int ks) {
@@ -796,7 +800,7 @@ int bf3_prepare(afq_system* s, Bf3System* b) {
     // part D = expH1 - I goes through the split-BF16 products (exact in FP64:
     // the diagonal of expH1 lies within a factor 2 of 1)
     hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, st, n, M, M, b->Kp, n, s->expH1, 1.0,
-                       b->E);
+                       b->E, 1);
     AFQ_CHECK_LAUNCH();
   }
   if (G > 0 && s->Na > 0 && s->rchol_a) {
@@ -911,7 +915,7 @@ int bf3_force_bias(hipStream_t s, const Bf3System& b, int W, const bf16_t* Gha, 
 int bf3_split_rows(hipStream_t s, const double* A, int rows, int cols, int cols_p, bf16_t* out) {
   const long n = (long)rows * cols_p;
   if (n <= 0) return AFQ_OK;
-  hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, s, n, rows, cols, cols_p, n, A, 0.0, out);
+  hipLaunchKernelGGL(k_prep_mat, dim3(1024), dim3(256), 0, s, n, rows, cols, cols_p, n, A, 0.0, out, 0);
   AFQ_CHECK_LAUNCH();
   return AFQ_OK;
 }
